@@ -1,0 +1,200 @@
+/*
+ * sfhydro: MI355X (gfx950) implementation of Stonefish's geometry-based
+ * hydrodynamics behind a C ABI.
+ *
+ * The entry points replace the per-face loops of SolidEntity and the body-level
+ * driver around them.  Each declaration names the reference interface it
+ * stands in for (paths relative to the reference checkout).  All entry points
+ * return an int status: SFH_OK (0) or a negative SFH_ERR_* code; the message of
+ * the last error on the calling thread is returned by sfh_last_error().
+ *
+ * Conventions (as in the reference):
+ *   - world frame NED, z down, gravity (0, 0, +g);
+ *   - transforms are 16 doubles in OpenGL column-major order, exactly what
+ *     btTransform::getOpenGLMatrix writes (basis columns in [0..2], [4..6],
+ *     [8..10], origin in [12..14]);
+ *   - velocities are world-frame doubles;
+ *   - drag outputs of the two per-face entry points are raw integrals (before
+ *     CorrectHydrodynamicForces), as in the reference.
+ */
+#ifndef SFHYDRO_H
+#define SFHYDRO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFH_OK 0
+#define SFH_ERR_INVALID (-1)     /* null handle, bad argument */
+#define SFH_ERR_SIZE (-2)        /* size out of range, buffer too small */
+#define SFH_ERR_HIP (-3)         /* HIP runtime error */
+#define SFH_ERR_NODEVICE (-4)    /* no gfx950 device */
+#define SFH_ERR_UNSUPPORTED (-5) /* configuration the kernels do not implement */
+
+typedef struct sfh_mesh_t* sfh_mesh;
+typedef struct sfh_ocean_t* sfh_ocean;
+typedef struct sfh_batch_t* sfh_batch;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* sfh_last_error(void);
+
+/* Selects the HIP device used by this thread's later calls. */
+int sfh_init(int device);
+
+/* ---- meshes ------------------------------------------------------------ */
+
+/* Uploads a triangle mesh to device memory.
+ * Replaces: the `const Mesh*` argument of the per-face loops
+ * (OpenGLDataStructs.h:190-310; Mesh::getVertexPos at 207-210 / 259-262).
+ * vertex_pos points at Vertex::pos of vertex 0; vertex_stride is the byte
+ * distance between vertices: sizeof(Vertex) = 24, sizeof(TexturableVertex) =
+ * 44, or 12 for packed xyz.  face_ids holds Face::vertexID, 3 per face.  The
+ * device copy is packed: 12 B per vertex, 12 B per face. */
+int sfh_mesh_create(const void* vertex_pos, size_t vertex_stride, uint32_t nv,
+                    const uint32_t* face_ids, uint32_t nf, sfh_mesh* out);
+int sfh_mesh_destroy(sfh_mesh mesh);
+/* Vertex/face counts and the float AABB of the vertices (mesh frame). */
+int sfh_mesh_info(sfh_mesh mesh, uint32_t* nv, uint32_t* nf, float aabb_min[3], float aabb_max[3]);
+
+/* ---- ocean ------------------------------------------------------------- */
+
+/* Liquid density and gravity.
+ * Replaces: Ocean::getLiquid().density (Ocean.h) and
+ * SimulationManager::getGravity() (SimulationManager.cpp:662-665). */
+int sfh_ocean_create(double density, const double gravity[3], sfh_ocean* out);
+int sfh_ocean_destroy(sfh_ocean ocean);
+
+/* Wave height data for Ocean::GetDepth (Ocean.cpp:149-168) through
+ * OpenGLRealOcean::ComputeWaveHeight (OpenGLRealOcean.cpp:280-331): n x n
+ * texels, `channels` interleaved floats per texel (4 for the reference's RGBA
+ * fftData), layer l reads channel l and spans lengths[l] metres (the
+ * reference: 2 layers, gridSizes.x and .y).  height(x,y) = - sum over layers
+ * of the periodic bilinear interpolation.  data == NULL or layers == 0 makes
+ * the surface flat (hasWaves() false).  host_data: 1 if `data` is host memory,
+ * 0 if device memory. */
+int sfh_ocean_set_waves(sfh_ocean ocean, const float* data, int host_data, uint32_t n,
+                        uint32_t channels, uint32_t layers, const float* lengths);
+
+/* Velocity fields summed by Ocean::GetFluidVelocity (Ocean.cpp:183-201).
+ * type 0 = Uniform (params[0..2] = v, Uniform.cpp:51-54).  Jet and Pipe
+ * (types 1, 2) are accepted by the oracle only; here they return
+ * SFH_ERR_UNSUPPORTED. */
+int sfh_ocean_add_current(sfh_ocean ocean, int type, int enabled, const double params[12]);
+/* Ocean::EnableCurrents / DisableCurrents (Ocean.cpp:203-211). */
+int sfh_ocean_enable_currents(sfh_ocean ocean, int enabled);
+
+/* ---- reference-shaped per-body calls (host memory, synchronous) ----------- */
+
+typedef struct {
+    int32_t damping_forces;      /* HydrodynamicsSettings::dampingForces */
+    int32_t realistic_buoyancy;  /* HydrodynamicsSettings::reallisticBuoyancy */
+} sfh_settings;
+
+/* Replaces: SolidEntity::ComputeHydrodynamicForcesSurface (SolidEntity.h:135-137,
+ * SolidEntity.cpp:1289-1721).  Outputs the reference leaves untouched (buoyancy
+ * when the submerged volume is <= 1e-9, drag when damping is off) keep the
+ * caller's values.  mesh == NULL zeroes the outputs as the reference does. */
+int sfh_forces_surface(const sfh_settings* settings, sfh_mesh mesh, sfh_ocean ocean,
+                       const double T_CG[16], const double T_C[16],
+                       const double v[3], const double omega[3],
+                       double Fb[3], double Tb[3], double Fdq[3], double Tdq[3],
+                       double Fdf[3], double Tdf[3], double* Swet, double* Vsub);
+
+/* Replaces: SolidEntity::ComputeHydrodynamicForcesSubmerged (SolidEntity.h:152-153,
+ * SolidEntity.cpp:1723-1797). */
+int sfh_forces_submerged(sfh_mesh mesh, sfh_ocean ocean,
+                         const double T_CG[16], const double T_C[16],
+                         const double v[3], const double omega[3],
+                         double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3]);
+
+/* Replaces: SolidEntity::CorrectHydrodynamicForces (SolidEntity.h:113-114,
+ * SolidEntity.cpp:1263-1287).  In/out drag vectors; T_O = getOTransform(). */
+int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],
+                const double Cd[3], const double Cf[3], const double T_O[16]);
+
+/* ---- batched body-level evaluation ------------------------------------- */
+
+/* Static description of one body: what SolidEntity::ComputeHydrodynamicForces
+ * (SolidEntity.cpp:1799-1850) reads from the entity besides its state. */
+typedef struct {
+    sfh_mesh mesh;            /* getPhysicsMesh() */
+    double T_CG2C[16];        /* getCTransform() = getCGTransform() * T_CG2C */
+    double T_CG2O[16];        /* getOTransform() = getCGTransform() * T_CG2O */
+    double aabb_min[3];       /* collision-shape AABB in the CG frame: the  */
+    double aabb_max[3];       /* btCompoundShape local AABB (getAABB, 215-226) */
+    double Cd[3], Cf[3];      /* fdCd, fdCf */
+    double volume, surface;   /* volume, surface */
+    double P_CB[3];           /* P_CB (CG frame) */
+    int32_t buoyant;          /* isBuoyant() */
+    int32_t hydro;            /* phy.mode is FLOATING or SUBMERGED */
+} sfh_body_desc;
+
+/* Per-body state: 22 doubles, T_CG[16] (getCGTransform), v[3], omega[3]. */
+#define SFH_STATE_DOUBLES 22
+/* Per-body output: 24 doubles: Fb[3] Tb[3] Fdq[3] Tdq[3] Fdf[3] Tdf[3] Swet
+ * Vsub, then the written mask (bit 0 Fb, 1 Tb, 2 Fdq, 3 Tdq, 4 Fdf, 5 Tdf,
+ * 6 Swet, 7 Vsub) and the body/fluid position (0 outside, 1 inside,
+ * 2 crossing), both stored as doubles, and 2 spare. */
+#define SFH_OUT_DOUBLES 24
+
+typedef struct {
+    uint32_t faces_per_lane;  /* chunk = 64 lanes x faces_per_lane faces; 0 = default */
+    uint32_t waves;           /* resident wavefronts of the persistent launch; 0 = default */
+} sfh_batch_opts;
+
+/* Replaces the per-body pool fan-out of SimulationManager::SimulationTickCallback
+ * (SimulationManager.cpp:1619-1655) + Ocean::ApplyFluidForces (Ocean.cpp:219-268)
+ * + SolidEntity::ComputeHydrodynamicForces: one batch holds many bodies and
+ * evaluates all of them in one pass on one stream.  Builds the face-prefix
+ * work list (chunks of faces, each owned by one body). */
+int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const sfh_batch_opts* opts, sfh_batch* out);
+int sfh_batch_destroy(sfh_batch batch);
+int sfh_batch_set_opts(sfh_batch batch, const sfh_batch_opts* opts);
+int sfh_batch_info(sfh_batch batch, uint32_t* nbodies, uint32_t* nchunks, uint64_t* nfaces);
+
+/* Evaluates every body (classification, per-face loop, FP64 reduction,
+ * buoyancy, CorrectHydrodynamicForces).  d_states: n x SFH_STATE_DOUBLES,
+ * d_out: n x SFH_OUT_DOUBLES, both device memory; `stream` is a hipStream_t
+ * (NULL = default stream).  Asynchronous.  Outputs the reference leaves
+ * untouched keep the values d_out held. */
+int sfh_batch_compute(sfh_batch batch, sfh_ocean ocean, const double* d_states, double* d_out, void* stream);
+
+/* Same with host buffers; synchronous. */
+int sfh_batch_compute_host(sfh_batch batch, sfh_ocean ocean, const double* states, double* out);
+
+/* Captures sfh_batch_compute(batch, ocean, d_states, d_out) on `stream` into a
+ * HIP graph held by the batch, then sfh_batch_replay launches it. */
+int sfh_batch_capture(sfh_batch batch, sfh_ocean ocean, const double* d_states, double* d_out, void* stream);
+int sfh_batch_replay(sfh_batch batch, void* stream);
+
+/* Average duration (ms) of the per-face kernel over the last `count`
+ * sfh_batch_compute calls made with timing on (HIP events on the launch
+ * stream). */
+int sfh_batch_timing(sfh_batch batch, int enable);
+int sfh_batch_kernel_ms(sfh_batch batch, double* face_ms, double* total_ms, uint32_t* count);
+
+/* ---- wave field builder ------------------------------------------------ */
+
+/* One sinusoid of the wave elevation field:
+ *   eta(x, y) = sum_k a_k cos(2 pi (mx_k x + my_k y) / L + phase_k)
+ * with integer wavenumbers so that the field is periodic on the L x L tile. */
+typedef struct {
+    float amplitude;
+    float phase;
+    int32_t mx, my;
+} sfh_wave_component;
+
+/* Builds the n x n elevation grid (1 channel, texel (i, j) at x = i L / n,
+ * y = j L / n) into device memory d_grid.  This is the input generator of the
+ * wave-field workload; it feeds sfh_ocean_set_waves like the reference's
+ * fftData. */
+int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_t n,
+                        float* d_grid, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,128 @@
+"""ctypes binding of ``include/sfhydro.h`` (libsfhydro.so).
+
+The library is the only compute path: if it is missing the import fails loudly,
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIBDIR = Path(__file__).resolve().parent / "_lib"
+
+SFH_OK = 0
+SFH_ERR_INVALID = -1
+SFH_ERR_SIZE = -2
+SFH_ERR_HIP = -3
+SFH_ERR_NODEVICE = -4
+SFH_ERR_UNSUPPORTED = -5
+
+SFH_STATE_DOUBLES = 22
+SFH_OUT_DOUBLES = 24
+
+c_double_p = C.POINTER(C.c_double)
+c_float_p = C.POINTER(C.c_float)
+c_uint32_p = C.POINTER(C.c_uint32)
+
+
+class sfh_settings(C.Structure):
+    _fields_ = [("damping_forces", C.c_int32), ("realistic_buoyancy", C.c_int32)]
+
+
+class sfh_body_desc(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_void_p),
+        ("T_CG2C", C.c_double * 16),
+        ("T_CG2O", C.c_double * 16),
+        ("aabb_min", C.c_double * 3),
+        ("aabb_max", C.c_double * 3),
+        ("Cd", C.c_double * 3),
+        ("Cf", C.c_double * 3),
+        ("volume", C.c_double),
+        ("surface", C.c_double),
+        ("P_CB", C.c_double * 3),
+        ("buoyant", C.c_int32),
+        ("hydro", C.c_int32),
+    ]
+
+
+class sfh_batch_opts(C.Structure):
+    _fields_ = [("faces_per_lane", C.c_uint32), ("waves", C.c_uint32)]
+
+
+class sfh_wave_component(C.Structure):
+    _fields_ = [("amplitude", C.c_float), ("phase", C.c_float), ("mx", C.c_int32), ("my", C.c_int32)]
+
+
+# name -> (restype, argtypes)
+_PROTOS = {
+    "sfh_last_error": (C.c_char_p, []),
+    "sfh_init": (C.c_int, [C.c_int]),
+    "sfh_mesh_create": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint32, c_uint32_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sfh_mesh_destroy": (C.c_int, [C.c_void_p]),
+    "sfh_mesh_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint32_p, c_float_p, c_float_p]),
+    "sfh_ocean_create": (C.c_int, [C.c_double, c_double_p, C.POINTER(C.c_void_p)]),
+    "sfh_ocean_destroy": (C.c_int, [C.c_void_p]),
+    "sfh_ocean_set_waves": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, c_float_p]),
+    "sfh_ocean_add_current": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p]),
+    "sfh_ocean_enable_currents": (C.c_int, [C.c_void_p, C.c_int]),
+    "sfh_forces_surface": (C.c_int, [C.POINTER(sfh_settings), C.c_void_p, C.c_void_p, c_double_p, c_double_p,
+                                     c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                     c_double_p, c_double_p, c_double_p, c_double_p]),
+    "sfh_forces_submerged": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                       c_double_p, c_double_p, c_double_p, c_double_p]),
+    "sfh_correct": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                              c_double_p]),
+    "sfh_batch_create": (C.c_int, [C.POINTER(sfh_body_desc), C.c_uint32, C.POINTER(sfh_batch_opts),
+                                   C.POINTER(C.c_void_p)]),
+    "sfh_batch_destroy": (C.c_int, [C.c_void_p]),
+    "sfh_batch_set_opts": (C.c_int, [C.c_void_p, C.POINTER(sfh_batch_opts)]),
+    "sfh_batch_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint32_p, C.POINTER(C.c_uint64)]),
+    "sfh_batch_compute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfh_batch_compute_host": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
+    "sfh_batch_capture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfh_batch_replay": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sfh_batch_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sfh_batch_kernel_ms": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_uint32_p]),
+    "sfh_wave_grid_build": (C.c_int, [C.POINTER(sfh_wave_component), C.c_uint32, C.c_uint32, C.c_void_p,
+                                      C.c_void_p]),
+}
+
+EXPORTED = tuple(_PROTOS)
+
+_lib = None
+
+
+def lib_path() -> Path:
+    return LIBDIR / "libsfhydro.so"
+
+
+def load() -> C.CDLL:
+    """Loads libsfhydro.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: the HIP library has not been built "
+            "(run `python -m stonefish_amd.build` or __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SfhError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sfhydro error {code}: {msg}")
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != SFH_OK:
+        msg = load().sfh_last_error()
+        raise SfhError(rc, msg.decode() if msg else "")

@@ -1,0 +1,221 @@
+// Per-face hydrodynamics integrand, FP32, evaluated in the body's C (mesh)
+// frame.  Shared by the gfx950 kernels (device) and the host-side unit harness.
+//
+// Reference: SolidEntity::ComputeHydrodynamicForcesSurface (SolidEntity.cpp:1289-1721)
+// and ComputeHydrodynamicForcesSubmerged (SolidEntity.cpp:1723-1797).  The
+// reference transforms every vertex to world coordinates in float; here the
+// same quantities are formed in the mesh frame, where a rigid transform leaves
+// cross products, dot products, lengths, triple products and affine
+// combinations (clip points, centroids) unchanged.  World quantities are
+// recovered once per body in FP64: forces and moments are rotated by R, the
+// moment about the centre of gravity is M + dt x F.  This keeps every FP32
+// operand at the scale of the body instead of its world position.
+#pragma once
+
+#ifdef __HIPCC__
+#define SFH_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define SFH_HD inline
+#endif
+
+namespace sfh {
+
+struct f3 { float x, y, z; };
+
+SFH_HD f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+SFH_HD f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+SFH_HD f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+SFH_HD f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+SFH_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+SFH_HD f3 cross(f3 a, f3 b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+// Thresholds spelled as in the reference (float literals).
+#define SFH_DEGENERATE_N2 1e-12f   // |fn|^2 below this: face skipped (1386 etc.)
+#define SFH_APPROACH_VN  -1e-12f   // quadratic drag when vc.n below this (1776)
+#define SFH_SKIN_VT2      1e-9f    // skin friction when |vt|^2 above this (1784)
+
+// Per-body, per-step constants of the face loop, all in the mesh frame.
+struct FaceBody {
+    f3 v;     // R^T * linear velocity of the body
+    f3 w;     // R^T * angular velocity
+    f3 u;     // R^T * fluid velocity (uniform part)
+    f3 dt;    // R^T * (t_C - t_CG): mesh origin relative to the CoG
+    f3 o;     // R^T * (p0 - t_C), p0 = (t_CG.x, t_CG.y, 0): volume apex
+};
+
+// Accumulator element type: double on the device (FP64 reduction), float or
+// double in host harnesses.
+template <typename T>
+struct Acc {
+    T Fb[3], Mb[3];      // pressure buoyancy (waves): force, moment about the mesh origin
+    T Fdq[3], Mdq[3];    // quadratic drag
+    T Fdf[3], Mdf[3];    // skin friction
+    T CB[3];             // sum of tetra centroid (relative to o) * 6V
+    T V6;                // sum of 6V
+    T Swet;              // wetted area
+};
+
+template <typename T>
+SFH_HD void acc_zero(Acc<T>& a)
+{
+    for (int k = 0; k < 3; ++k) a.Fb[k] = a.Mb[k] = a.Fdq[k] = a.Mdq[k] = a.Fdf[k] = a.Mdf[k] = a.CB[k] = T(0);
+    a.V6 = T(0);
+    a.Swet = T(0);
+}
+
+template <typename T>
+SFH_HD void add3(T* acc, f3 x)
+{
+    acc[0] += (T)x.x; acc[1] += (T)x.y; acc[2] += (T)x.z;
+}
+
+// Drag integrand of one (possibly clipped) face: centroid c (mesh frame), unit
+// normal n1, area A (SolidEntity.cpp:1660-1683, 1767-1790).
+template <typename T>
+SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, Acc<T>& acc)
+{
+    f3 r = c + B.dt;
+    f3 vc = B.u - (B.v + cross(B.w, r));
+    float vcn = dot(vc, n1);
+    if (vcn < SFH_APPROACH_VN) {
+        float m = sqrtf(dot(vc, vc));
+        f3 q = vc * (m * (-vcn) * A);
+        add3(acc.Fdq, q);
+        add3(acc.Mdq, cross(c, q));
+    }
+    f3 vt = vc - n1 * vcn;
+    if (dot(vt, vt) > SFH_SKIN_VT2) {
+        f3 s = vt * A;
+        add3(acc.Fdf, s);
+        add3(acc.Mdf, cross(c, s));
+    }
+}
+
+// Submerged face (no clipping, no buoyancy): SolidEntity.cpp:1749-1791.
+template <typename T>
+SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
+{
+    f3 n = cross(p2 - p1, p3 - p1);
+    float n2 = dot(n, n);
+    if (n2 < SFH_DEGENERATE_N2) return;
+    float len = sqrtf(n2);
+    f3 n1 = n * (1.0f / len);
+    float A = 0.5f * len;
+    f3 c = (p1 + p2 + p3) * (1.0f / 3.0f);
+    face_drag(B, c, n1, A, acc);
+}
+
+// Tetra (p0; a, b, c) of the submerged-volume integral.  6V = (a-p0).((b-a)x(c-a)),
+// equal to the reference's p0a.(p0b x p0c) (SolidEntity.cpp:1373-1379) and free
+// of its cancellation; the centroid term is (p0a+p0b+p0c)/4 * 6V.
+template <typename T>
+SFH_HD void face_tetra(const FaceBody& B, f3 a, f3 b, f3 c, Acc<T>& acc)
+{
+    f3 ao = a - B.o;
+    float v6 = dot(ao, cross(b - a, c - a));
+    f3 g = (ao + (b - B.o) + (c - B.o)) * 0.25f;
+    add3(acc.CB, g * v6);
+    acc.V6 += (T)v6;
+}
+
+// Clip point on the edge from wet vertex `w` (depth dw >= 0) to dry vertex `d`
+// (depth dd < 0).
+SFH_HD f3 clip_pt(f3 w, f3 d, float dw, float dd)
+{
+    return w + (d - w) * (dw / (fabsf(dd) + dw));
+}
+
+// Result of clipping one face against the waterline: the wet polygon's
+// centroid, unit normal and area, and whether the face contributes.
+struct Wet {
+    f3 c, n1;
+    float A;
+    bool ok;
+};
+
+// Surface face: waterline clip (7 cases, SolidEntity.cpp:1362-1647), tetra
+// volume, face properties.  d0..d2 are the vertex depths (>= 0 is wet).  The
+// tetra volume is accumulated before the degeneracy test, as in the reference.
+template <typename T>
+SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1, float d2, Acc<T>& acc)
+{
+    Wet r;
+    r.ok = false;
+    f3 p4 = p1, fv1, fv2, fv3 = p1, fv4 = p1;
+    bool quad = false;
+    if (d0 < 0.f) {
+        if (d1 < 0.f) {                 // A: only p3 wet
+            p1 = clip_pt(p3, p1, d2, d0);
+            p2 = clip_pt(p3, p2, d2, d1);
+            face_tetra(B, p1, p2, p3, acc);
+            fv1 = p2 - p1; fv2 = p3 - p1;
+        } else if (d2 < 0.f) {          // B: only p2 wet
+            p1 = clip_pt(p2, p1, d1, d0);
+            p3 = clip_pt(p2, p3, d1, d2);
+            face_tetra(B, p1, p2, p3, acc);
+            fv1 = p2 - p1; fv2 = p3 - p1;
+        } else {                        // C: only p1 dry -> quad p1 p2 p3 p4
+            p4 = clip_pt(p3, p1, d2, d0);
+            p1 = clip_pt(p2, p1, d1, d0);
+            face_tetra(B, p1, p2, p3, acc);
+            face_tetra(B, p1, p3, p4, acc);
+            fv1 = p2 - p1; fv2 = p4 - p1; fv3 = p2 - p3; fv4 = p4 - p3;
+            quad = true;
+        }
+    } else if (d1 < 0.f) {
+        if (d2 < 0.f) {                 // D: only p1 wet
+            p2 = clip_pt(p1, p2, d0, d1);
+            p3 = clip_pt(p1, p3, d0, d2);
+            face_tetra(B, p1, p2, p3, acc);
+            fv1 = p2 - p1; fv2 = p3 - p1;
+        } else {                        // E: only p2 dry
+            p4 = clip_pt(p3, p2, d2, d1);
+            p2 = clip_pt(p1, p2, d0, d1);
+            face_tetra(B, p1, p2, p3, acc);
+            face_tetra(B, p2, p4, p3, acc);
+            fv1 = p2 - p1; fv2 = p3 - p1; fv3 = p2 - p3; fv4 = p4 - p3;
+            quad = true;
+        }
+    } else if (d2 < 0.f) {              // F: only p3 dry
+        p4 = clip_pt(p1, p3, d0, d2);
+        p3 = clip_pt(p2, p3, d1, d2);
+        face_tetra(B, p1, p2, p3, acc);
+        face_tetra(B, p1, p3, p4, acc);
+        fv1 = p2 - p1; fv2 = p4 - p1; fv3 = p2 - p3; fv4 = p4 - p3;
+        quad = true;
+    } else {                            // G: all wet
+        fv1 = p2 - p1; fv2 = p3 - p1;
+        f3 n = cross(fv1, fv2);
+        f3 ao = p1 - B.o;
+        float v6 = dot(ao, n);
+        f3 g = (ao + (p2 - B.o) + (p3 - B.o)) * 0.25f;
+        add3(acc.CB, g * v6);
+        acc.V6 += (T)v6;
+        float n2 = dot(n, n);
+        if (n2 < SFH_DEGENERATE_N2) return r;
+        float len = sqrtf(n2);
+        r.n1 = n * (1.0f / len);
+        r.A = 0.5f * len;
+        r.c = (p1 + p2 + p3) * (1.0f / 3.0f);
+        r.ok = true;
+        return r;
+    }
+    f3 n = cross(fv1, fv2);
+    float n2 = dot(n, n);
+    if (n2 < SFH_DEGENERATE_N2) return r;
+    float len = sqrtf(n2);
+    r.n1 = n * (1.0f / len);
+    if (quad) {
+        f3 n34 = cross(fv3, fv4);
+        r.A = 0.5f * (len + sqrtf(dot(n34, n34)));
+        r.c = (p1 + p2 + p3 + p4) * 0.25f;
+    } else {
+        r.A = 0.5f * len;
+        r.c = (p1 + p2 + p3) * (1.0f / 3.0f);
+    }
+    r.ok = true;
+    return r;
+}
+
+}  // namespace sfh

@@ -1,0 +1,1295 @@
+// sfhydro: gfx950 kernels and C-ABI host code for Stonefish's geometry-based
+// hydrodynamics (SolidEntity.cpp:1239-1850, Ocean.cpp:149-201,
+// OpenGLRealOcean.cpp:280-331, Compound-free body path).
+//
+// One evaluation pass over a batch of bodies is three launches on one stream:
+//   k_prologue  one lane per body, FP64: C transform, body/fluid classification
+//               (8 AABB corners against the surface), per-body FP32 constants of
+//               the face loop in the mesh frame, FP64 constants of the epilogue;
+//   k_faces     one wavefront per chunk of 64*K faces of one body, FP32 face
+//               arithmetic, FP64 per-lane accumulation and FP64 wavefront
+//               reduction; one 24-double partial per chunk;
+//   k_epilogue  one lane per body, FP64: sums the body's chunk partials in chunk
+//               order (deterministic), rotates to world, moment about the CoG,
+//               buoyancy, CorrectHydrodynamicForces, output write mask.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/sfhydro.h"
+#include "sfh_face.h"
+
+using namespace sfh;
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SFH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+extern "C" const char* sfh_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------
+// device-side data
+// ---------------------------------------------------------------------------
+
+enum : uint32_t {
+    BF_HYDRO = 1u,       // physics mode FLOATING or SUBMERGED
+    BF_BUOYANT = 2u,     // isBuoyant()
+    BF_DAMPING = 4u,     // dampingForces (per-face calls); body level: always on
+    BF_MODE_SURF = 8u,   // static Surface call: no classification, no Correct
+    BF_MODE_SUBM = 16u,  // static Submerged call: no classification, no Correct
+    BF_EXPLICIT_TC = 32u // state carries T_C explicitly (per-face calls)
+};
+
+static constexpr int kMaxLayers = 4;
+static constexpr int kChunkDoubles = 24;
+
+struct BodyStatic {
+    const float* verts;
+    const uint32_t* faces;
+    uint32_t nf;
+    uint32_t flags;
+    uint32_t chunk0;
+    uint32_t nchunks;
+    double cg2c[12];   // basis row-major, origin
+    double cg2o[9];    // basis row-major
+    double aabb_c[3];  // CG-frame AABB centre and half extents
+    double aabb_h[3];
+    double Cd[3], Cf[3];
+    double volume, surface;
+    double P_CB[3];
+};
+
+struct LayerK {
+    float fu, fv;       // fractional texel coordinate of the mesh origin
+    int ku, kv;         // integer texel coordinate of the mesh origin, mod n
+    f3 ru, rv;          // rows 0/1 of R scaled by n / length: texels per mesh unit
+};
+
+struct BodyDyn {
+    int32_t pos;        // 0 outside, 1 inside (submerged loop), 2 crossing (surface loop)
+    uint32_t flags;     // effective settings of the face loop
+    FaceBody fb;
+    f3 rz;              // row 2 of R: world z of a mesh-frame vector
+    float tz;           // world z of the mesh origin
+    LayerK layer[kMaxLayers];
+    // FP64 epilogue constants
+    double R[9];        // basis of T_C, row-major
+    double dt[3];       // R^T (t_C - t_CG)
+    double Rcg[9];      // basis of T_CG, row-major
+    double tcg[3];      // origin of T_CG
+};
+
+struct OceanK {
+    double rho;
+    double g[3];
+    double len[kMaxLayers];
+    double u[3];                      // sum of enabled uniform currents
+    const float4* quad[kMaxLayers];   // per layer: (h00, h10, h01, h11) per texel
+    int32_t nlayers;                  // 0 = flat surface
+    int32_t n;
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int wrap_n(int i, int n)
+{
+    // i = k + floor(u) with k in [0, n) and |floor(u)| below n for any body
+    // smaller than the wave tile: one conditional step wraps it.  The clamp
+    // keeps the texel read in bounds for anything else (non-finite states).
+    if (i < 0) i += n;
+    if (i >= n) i -= n;
+    return min(max(i, 0), n - 1);
+}
+
+// Sum over layers of the bilinear height at mesh-frame point p (FP32).  The
+// texel coordinate of p is u = ku + fu + ru.p (the integer part of the body
+// position is carried exactly in ku).  ComputeInterpolatedWaveData in exact
+// arithmetic: i0 = floor(u) mod n, i1 = i0 + 1 mod n, alpha = frac(u).
+__device__ __forceinline__ float wave_sum(const BodyDyn& D, const OceanK& oc, f3 p)
+{
+    float h = 0.f;
+    for (int l = 0; l < oc.nlayers; ++l) {
+        const LayerK& L = D.layer[l];
+        float u = L.fu + dot(L.ru, p);
+        float v = L.fv + dot(L.rv, p);
+        float fi = floorf(u), fj = floorf(v);
+        float a = u - fi, b = v - fj;
+        int i = wrap_n(L.ku + (int)fi, oc.n);
+        int j = wrap_n(L.kv + (int)fj, oc.n);
+        float4 t = oc.quad[l][(size_t)j * oc.n + i];
+        h += (1.f - a) * (1.f - b) * t.x + a * (1.f - b) * t.y + (1.f - a) * b * t.z + a * b * t.w;
+    }
+    return h;
+}
+
+// depth = z - H(x, y) with H = -sum of layer heights (Ocean::GetDepth).
+__device__ __forceinline__ float depth_local(const BodyDyn& D, const OceanK& oc, f3 p)
+{
+    float z = D.tz + dot(D.rz, p);
+    if (oc.nlayers) z += wave_sum(D, oc, p);
+    return z;
+}
+
+// FP64 wave height sum at world (x, y), for the classification corners.
+__device__ double wave_sum64(const OceanK& oc, double x, double y)
+{
+    double h = 0.0;
+    for (int l = 0; l < oc.nlayers; ++l) {
+        double s = (double)oc.n / oc.len[l];
+        double u = x * s - 0.5, v = y * s - 0.5;
+        double fi = floor(u), fj = floor(v);
+        double a = u - fi, b = v - fj;
+        long long ii = (long long)fi % oc.n, jj = (long long)fj % oc.n;
+        if (ii < 0) ii += oc.n;
+        if (jj < 0) jj += oc.n;
+        float4 t = oc.quad[l][(size_t)jj * oc.n + (size_t)ii];
+        h += (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
+    }
+    return h;
+}
+
+__device__ __forceinline__ void gl_basis(const double* m, double* B)  // OpenGL -> row-major
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) B[3 * r + c] = m[4 * c + r];
+}
+
+__device__ __forceinline__ void mat_t_vec(const double* B, const double* x, double* y)  // B^T x
+{
+    for (int i = 0; i < 3; ++i) y[i] = B[i] * x[0] + B[3 + i] * x[1] + B[6 + i] * x[2];
+}
+
+__device__ __forceinline__ void mat_vec(const double* B, const double* x, double* y)  // B x
+{
+    for (int i = 0; i < 3; ++i) y[i] = B[3 * i] * x[0] + B[3 * i + 1] * x[1] + B[3 * i + 2] * x[2];
+}
+
+__device__ __forceinline__ void cross64(const double* a, const double* b, double* c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ f3 to_f3(const double* x) { return mk3((float)x[0], (float)x[1], (float)x[2]); }
+
+// ---------------------------------------------------------------------------
+// k_prologue: one lane per body
+// ---------------------------------------------------------------------------
+
+// states: kind 0 = body state (T_CG[16], v, w; T_C = T_CG * T_CG2C),
+//         kind 1 = explicit (T_CG[16], T_C[16], v, w).
+__global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
+                           uint32_t n, OceanK oc, BodyDyn* __restrict__ dyn)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const BodyStatic& S = st[b];
+    BodyDyn D;
+    const double* s = states + (size_t)b * (state_kind ? 38 : 22);
+    double Rcg[9], tcg[3], R[9], tc[3];
+    gl_basis(s, Rcg);
+    tcg[0] = s[12]; tcg[1] = s[13]; tcg[2] = s[14];
+    const double* v = s + (state_kind ? 32 : 16);
+    const double* w = v + 3;
+    if (state_kind) {
+        gl_basis(s + 16, R);
+        tc[0] = s[28]; tc[1] = s[29]; tc[2] = s[30];
+    } else {
+        // btTransform operator*: basis Rcg * B, origin Rcg * o + tcg
+        const double* B = S.cg2c;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                R[3 * i + j] = Rcg[3 * i] * B[j] + Rcg[3 * i + 1] * B[3 + j] + Rcg[3 * i + 2] * B[6 + j];
+        for (int i = 0; i < 3; ++i)
+            tc[i] = (Rcg[3 * i] * B[9] + Rcg[3 * i + 1] * B[10] + Rcg[3 * i + 2] * B[11]) + tcg[i];
+    }
+
+    uint32_t flags = S.flags;
+    int pos;
+    if (flags & BF_MODE_SURF) pos = 2;
+    else if (flags & BF_MODE_SUBM) pos = 1;
+    else if (!(flags & BF_HYDRO)) pos = -1;
+    else {
+        // CheckBodyFluidPosition (SolidEntity.cpp:1239-1261) on the world AABB
+        // btCompoundShape::getAabb(T_CG) of the CG-frame AABB.
+        double c[3], e[3], mn[3], mx[3], d[3];
+        for (int i = 0; i < 3; ++i) {
+            c[i] = (Rcg[3 * i] * S.aabb_c[0] + Rcg[3 * i + 1] * S.aabb_c[1] + Rcg[3 * i + 2] * S.aabb_c[2]) + tcg[i];
+            e[i] = fabs(Rcg[3 * i]) * S.aabb_h[0] + fabs(Rcg[3 * i + 1]) * S.aabb_h[1] + fabs(Rcg[3 * i + 2]) * S.aabb_h[2];
+            mn[i] = c[i] - e[i];
+            mx[i] = c[i] + e[i];
+            d[i] = mx[i] - mn[i];
+        }
+        int under = 0;
+        for (int k = 0; k < 8; ++k) {
+            double q[3];
+            if (k == 1) { q[0] = mx[0]; q[1] = mx[1]; q[2] = mx[2]; }
+            else {
+                // corners min, +x, +y, +xy, +z, +xz, +yz in the order of the reference
+                const int sx[8] = {0, 0, 1, 0, 1, 0, 1, 0};
+                const int sy[8] = {0, 0, 0, 1, 1, 0, 0, 1};
+                const int sz[8] = {0, 0, 0, 0, 0, 1, 1, 1};
+                q[0] = mn[0] + (sx[k] ? d[0] : 0.0);
+                q[1] = mn[1] + (sy[k] ? d[1] : 0.0);
+                q[2] = mn[2] + (sz[k] ? d[2] : 0.0);
+            }
+            double depth = q[2];
+            if (oc.nlayers) depth += wave_sum64(oc, q[0], q[1]);
+            if (depth > 0.0) ++under;
+        }
+        pos = under == 0 ? 0 : (under == 8 ? 1 : 2);
+        // body level: damping always on (Ocean::ApplyFluidForces, Ocean.cpp:248-252)
+        flags |= BF_DAMPING;
+    }
+    D.pos = pos;
+    D.flags = flags;
+
+    // mesh-frame FP32 constants
+    double x[3], y[3];
+    mat_t_vec(R, v, y);   D.fb.v = to_f3(y);
+    mat_t_vec(R, w, y);   D.fb.w = to_f3(y);
+    mat_t_vec(R, oc.u, y); D.fb.u = to_f3(y);
+    for (int i = 0; i < 3; ++i) x[i] = tc[i] - tcg[i];
+    mat_t_vec(R, x, y);
+    D.fb.dt = to_f3(y);
+    D.dt[0] = y[0]; D.dt[1] = y[1]; D.dt[2] = y[2];
+    x[0] = tcg[0] - tc[0]; x[1] = tcg[1] - tc[1]; x[2] = -tc[2];
+    mat_t_vec(R, x, y);
+    D.fb.o = to_f3(y);
+    D.rz = mk3((float)R[6], (float)R[7], (float)R[8]);
+    D.tz = (float)tc[2];
+    for (int l = 0; l < kMaxLayers; ++l) {
+        LayerK L;
+        L.fu = L.fv = 0.f; L.ku = L.kv = 0;
+        L.ru = L.rv = mk3(0.f, 0.f, 0.f);
+        if (l < oc.nlayers) {
+            double s = (double)oc.n / oc.len[l];
+            double bu = tc[0] * s - 0.5, bv = tc[1] * s - 0.5;
+            double fu = floor(bu), fv = floor(bv);
+            L.fu = (float)(bu - fu);
+            L.fv = (float)(bv - fv);
+            long long ku = (long long)fu % oc.n, kv = (long long)fv % oc.n;
+            if (ku < 0) ku += oc.n;
+            if (kv < 0) kv += oc.n;
+            L.ku = (int)ku; L.kv = (int)kv;
+            L.ru = mk3((float)(R[0] * s), (float)(R[1] * s), (float)(R[2] * s));
+            L.rv = mk3((float)(R[3] * s), (float)(R[4] * s), (float)(R[5] * s));
+        }
+        D.layer[l] = L;
+    }
+    for (int i = 0; i < 9; ++i) { D.R[i] = R[i]; D.Rcg[i] = Rcg[i]; }
+    for (int i = 0; i < 3; ++i) D.tcg[i] = tcg[i];
+    dyn[b] = D;
+}
+
+// ---------------------------------------------------------------------------
+// k_faces: one wavefront per chunk of 64*K faces
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum_f64(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ __forceinline__ void load_face(const BodyStatic& S, uint32_t f, f3& p1, f3& p2, f3& p3)
+{
+    const uint32_t* fi = S.faces + 3 * (size_t)f;
+    uint32_t i0 = fi[0], i1 = fi[1], i2 = fi[2];
+    const float* v0 = S.verts + 3 * (size_t)i0;
+    const float* v1 = S.verts + 3 * (size_t)i1;
+    const float* v2 = S.verts + 3 * (size_t)i2;
+    p1 = mk3(v0[0], v0[1], v0[2]);
+    p2 = mk3(v1[0], v1[1], v1[2]);
+    p3 = mk3(v2[0], v2[1], v2[2]);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_faces(const uint2* __restrict__ chunks, uint32_t nchunks,
+                                               const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
+                                               OceanK oc, double* __restrict__ partials)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t c = wave; c < nchunks; c += nwaves) {
+        const uint2 ch = chunks[c];
+        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.x);
+        const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.y);
+        const BodyDyn& D = dyn[b];
+        const int pos = __builtin_amdgcn_readfirstlane(D.pos);
+        if (pos <= 0) continue;
+        const BodyStatic& S = st[b];
+        const uint32_t fend = min(f0 + 64u * K, S.nf);
+        const FaceBody fb = D.fb;
+        const uint32_t flags = D.flags;
+        Acc<double> acc;
+        acc_zero(acc);
+        if (pos == 1) {
+            for (int j = 0; j < K; ++j) {
+                uint32_t f = f0 + 64u * j + lane;
+                if (f < fend) {
+                    f3 p1, p2, p3;
+                    load_face(S, f, p1, p2, p3);
+                    face_submerged(fb, p1, p2, p3, acc);
+                }
+            }
+        } else {
+            const bool damping = flags & BF_DAMPING;
+            const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
+            for (int j = 0; j < K; ++j) {
+                uint32_t f = f0 + 64u * j + lane;
+                if (f < fend) {
+                    f3 p1, p2, p3;
+                    load_face(S, f, p1, p2, p3);
+                    float d0 = depth_local(D, oc, p1);
+                    float d1 = depth_local(D, oc, p2);
+                    float d2 = depth_local(D, oc, p3);
+                    if (!(d0 < 0.f && d1 < 0.f && d2 < 0.f)) {
+                        Wet wt = face_clip(fb, p1, p2, p3, d0, d1, d2, acc);
+                        if (wt.ok) {
+                            if (wave_buoy) {
+                                float dc = depth_local(D, oc, wt.c);
+                                f3 fbi = wt.n1 * (-wt.A * dc);
+                                add3(acc.Fb, fbi);
+                                add3(acc.Mb, cross(wt.c, fbi));
+                            }
+                            if (damping) face_drag(fb, wt.c, wt.n1, wt.A, acc);
+                            acc.Swet += (double)wt.A;
+                        }
+                    }
+                }
+            }
+        }
+        // FP64 wavefront reduction; chunk partial layout:
+        // Fdq Mdq Fdf Mdf Fb Mb CB (3 each), V6, Swet
+        double* out = partials + (size_t)c * kChunkDoubles;
+        double vals[kChunkDoubles - 1];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            vals[k] = acc.Fdq[k]; vals[3 + k] = acc.Mdq[k];
+            vals[6 + k] = acc.Fdf[k]; vals[9 + k] = acc.Mdf[k];
+            vals[12 + k] = acc.Fb[k]; vals[15 + k] = acc.Mb[k];
+            vals[18 + k] = acc.CB[k];
+        }
+        vals[21] = acc.V6;
+        vals[22] = acc.Swet;
+        if (pos == 1) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) vals[k] = wave_sum_f64(vals[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kChunkDoubles - 1; ++k) vals[k] = wave_sum_f64(vals[k]);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < kChunkDoubles - 1; ++k) out[k] = vals[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_epilogue: one lane per body
+// ---------------------------------------------------------------------------
+
+// CorrectHydrodynamicForces (SolidEntity.cpp:1263-1287): M = inverse(basis of
+// T_O) by cofactors (btMatrix3x3.h:1093-1103); safeNormalize (btVector3.h:286-299).
+__device__ void correct64(double rho, const double* BO, const double* Cd, const double* Cf,
+                          double* Fdq, double* Tdq, double* Fdf, double* Tdf)
+{
+#define CF(r1, c1, r2, c2) (BO[3 * (r1) + (c1)] * BO[3 * (r2) + (c2)] - BO[3 * (r1) + (c2)] * BO[3 * (r2) + (c1)])
+    double co0 = CF(1, 1, 2, 2), co1 = CF(1, 2, 2, 0), co2 = CF(1, 0, 2, 1);
+    double det = BO[0] * co0 + BO[1] * co1 + BO[2] * co2;
+    double s = 1.0 / det;
+    double M[9] = {co0 * s, CF(0, 2, 2, 1) * s, CF(0, 1, 1, 2) * s,
+                   co1 * s, CF(0, 0, 2, 2) * s, CF(0, 2, 1, 0) * s,
+                   co2 * s, CF(0, 1, 2, 0) * s, CF(0, 0, 1, 1) * s};
+#undef CF
+    double* vecs[4] = {Fdq, Tdq, Fdf, Tdf};
+    for (int q = 0; q < 4; ++q) {
+        double* x = vecs[q];
+        double d[3];
+        mat_vec(M, x, d);
+        double l2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (l2 >= 2.220446049250313e-16 * 2.220446049250313e-16) {
+            double l = sqrt(l2);
+            d[0] /= l; d[1] /= l; d[2] /= l;
+        } else {
+            d[0] = 1.0; d[1] = 0.0; d[2] = 0.0;
+        }
+        const double* C = (q < 2) ? Cd : Cf;
+        double c = fabs(d[0]) * C[0] + fabs(d[1]) * C[1] + fabs(d[2]) * C[2];
+        double k = (q < 2) ? 0.5 * rho * c : rho * c;
+        x[0] *= k; x[1] *= k; x[2] *= k;
+    }
+}
+
+__global__ void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t n,
+                           OceanK oc, const double* __restrict__ partials, double* __restrict__ out)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const BodyStatic& S = st[b];
+    const BodyDyn& D = dyn[b];
+    double* o = out + (size_t)b * SFH_OUT_DOUBLES;
+    const int pos = D.pos;
+    const uint32_t flags = D.flags;
+    if (pos < 0) {   // not a hydrodynamic body: nothing assigned
+        o[20] = 0.0;
+        o[21] = -1.0;
+        return;
+    }
+    if (pos == 0) {  // OUTSIDE: everything zero (SolidEntity.cpp:1810-1821)
+        for (int k = 0; k < 20; ++k) o[k] = 0.0;
+        o[20] = 255.0;
+        o[21] = 0.0;
+        return;
+    }
+    double P[23];
+#pragma unroll
+    for (int k = 0; k < 23; ++k) P[k] = 0.0;
+    for (uint32_t c = 0; c < S.nchunks; ++c) {
+        const double* p = partials + (size_t)(S.chunk0 + c) * kChunkDoubles;
+#pragma unroll
+        for (int k = 0; k < 23; ++k) P[k] += p[k];
+    }
+    const bool body_level = !(flags & (BF_MODE_SURF | BF_MODE_SUBM));
+    const bool damping = flags & BF_DAMPING;
+    const bool buoyant = flags & BF_BUOYANT;
+    uint32_t written = 0;
+    // drag: world = R * F_local; torque = R * (M_local + dt x F_local)
+    double drag[12];
+    for (int q = 0; q < 2; ++q) {
+        const double* F = P + 6 * q;
+        const double* M = P + 6 * q + 3;
+        double t[3], m[3];
+        cross64(D.dt, F, t);
+        for (int k = 0; k < 3; ++k) m[k] = M[k] + t[k];
+        mat_vec(D.R, F, drag + 6 * q);
+        mat_vec(D.R, m, drag + 6 * q + 3);
+    }
+    double* Fdq = drag;
+    double* Tdq = drag + 3;
+    double* Fdf = drag + 6;
+    double* Tdf = drag + 9;
+    double Fb[3] = {0, 0, 0}, Tb[3] = {0, 0, 0};
+    double Swet = 0.0, Vsub = 0.0;
+    if (pos == 1) {
+        if (body_level) {
+            if (buoyant) {   // analytic buoyancy of a submerged body (1828-1836)
+                double cb[3];
+                for (int k = 0; k < 3; ++k) Fb[k] = -S.volume * oc.rho * oc.g[k];
+                mat_vec(D.Rcg, S.P_CB, cb);
+                cross64(cb, Fb, Tb);
+                written |= 3u;
+            }
+            Swet = S.surface;
+            written |= 64u;
+        }
+        written |= 4u | 8u | 16u | 32u;
+    } else {
+        const double V6 = P[21];
+        if (buoyant && V6 > (double)1e-9f) {
+            Vsub = V6 / 6.0;
+            written |= 1u | 2u | 128u;
+            if (oc.nlayers) {   // pressure integration (1694-1700)
+                double s = oc.rho * oc.g[2];
+                double t[3], m[3];
+                cross64(D.dt, P + 12, t);
+                for (int k = 0; k < 3; ++k) m[k] = P[15 + k] + t[k];
+                mat_vec(D.R, P + 12, Fb);
+                mat_vec(D.R, m, Tb);
+                for (int k = 0; k < 3; ++k) { Fb[k] *= s; Tb[k] *= s; }
+            } else {            // volume and centre of buoyancy (1701-1707)
+                double cbl[3], cb[3], r[3];
+                for (int k = 0; k < 3; ++k) cbl[k] = P[18 + k] / V6;
+                mat_vec(D.R, cbl, cb);
+                // CB - T_CG.origin = (p0 - t_CG) + R cbl, p0 - t_CG = (0, 0, -t_CG.z)
+                r[0] = cb[0]; r[1] = cb[1]; r[2] = cb[2] - D.tcg[2];
+                for (int k = 0; k < 3; ++k) Fb[k] = -Vsub * oc.rho * oc.g[k];
+                cross64(r, Fb, Tb);
+            }
+        }
+        if (damping) written |= 4u | 8u | 16u | 32u;
+        Swet = P[22];
+        written |= 64u;
+    }
+    if (body_level && damping) {
+        double BO[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                BO[3 * i + j] = D.Rcg[3 * i] * S.cg2o[j] + D.Rcg[3 * i + 1] * S.cg2o[3 + j] + D.Rcg[3 * i + 2] * S.cg2o[6 + j];
+        correct64(oc.rho, BO, S.Cd, S.Cf, Fdq, Tdq, Fdf, Tdf);
+    }
+    if (written & 1u) for (int k = 0; k < 3; ++k) o[k] = Fb[k];
+    if (written & 2u) for (int k = 0; k < 3; ++k) o[3 + k] = Tb[k];
+    if (written & 4u) for (int k = 0; k < 3; ++k) o[6 + k] = Fdq[k];
+    if (written & 8u) for (int k = 0; k < 3; ++k) o[9 + k] = Tdq[k];
+    if (written & 16u) for (int k = 0; k < 3; ++k) o[12 + k] = Fdf[k];
+    if (written & 32u) for (int k = 0; k < 3; ++k) o[15 + k] = Tdf[k];
+    if (written & 64u) o[18] = Swet;
+    if (written & 128u) o[19] = Vsub;
+    o[20] = (double)written;
+    o[21] = (double)pos;
+}
+
+// Standalone CorrectHydrodynamicForces for the per-body entry point.
+__global__ void k_correct(double rho, const double* __restrict__ in, double* __restrict__ io)
+{
+    // in: BO[9] row-major, Cd[3], Cf[3]; io: Fdq Tdq Fdf Tdf
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    correct64(rho, in, in + 9, in + 12, io, io + 3, io + 6, io + 9);
+}
+
+// ---------------------------------------------------------------------------
+// wave grid builder
+// ---------------------------------------------------------------------------
+
+__global__ void k_wave_grid(const sfh_wave_component* __restrict__ comps, uint32_t ncomp, uint32_t n,
+                            float* __restrict__ grid)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t j = blockIdx.y;
+    if (i >= n) return;
+    const float two_pi_n = 6.283185307179586f / (float)n;
+    float h = 0.f;
+    for (uint32_t k = 0; k < ncomp; ++k) {
+        long long m = (long long)comps[k].mx * i + (long long)comps[k].my * j;
+        m %= (long long)n;
+        if (m < 0) m += n;
+        h += comps[k].amplitude * cosf(two_pi_n * (float)m + comps[k].phase);
+    }
+    grid[(size_t)j * n + i] = h;
+}
+
+// quad texture: per texel (h[j][i], h[j][i+1], h[j+1][i], h[j+1][i+1]), wrapping
+__global__ void k_wave_quad(const float* __restrict__ data, uint32_t n, uint32_t channels, uint32_t channel,
+                            float4* __restrict__ quad)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t j = blockIdx.y;
+    if (i >= n) return;
+    uint32_t i1 = (i + 1 == n) ? 0 : i + 1;
+    uint32_t j1 = (j + 1 == n) ? 0 : j + 1;
+    float4 q;
+    q.x = data[((size_t)j * n + i) * channels + channel];
+    q.y = data[((size_t)j * n + i1) * channels + channel];
+    q.z = data[((size_t)j1 * n + i) * channels + channel];
+    q.w = data[((size_t)j1 * n + i1) * channels + channel];
+    quad[(size_t)j * n + i] = q;
+}
+
+// ---------------------------------------------------------------------------
+// host objects
+// ---------------------------------------------------------------------------
+
+struct sfh_mesh_t {
+    float* d_verts = nullptr;
+    uint32_t* d_faces = nullptr;
+    uint32_t nv = 0, nf = 0;
+    float aabb_min[3], aabb_max[3];
+};
+
+struct Current {
+    int type, enabled;
+    double p[12];
+};
+
+struct sfh_ocean_t {
+    double rho = 1000.0;
+    double g[3] = {0, 0, 9.81};
+    int nlayers = 0;
+    uint32_t n = 0;
+    double len[kMaxLayers] = {0, 0, 0, 0};
+    float4* d_quad[kMaxLayers] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<Current> currents;
+    bool currents_enabled = false;
+};
+
+struct sfh_batch_t {
+    uint32_t n = 0;
+    uint32_t nchunks = 0;
+    uint64_t nfaces = 0;
+    uint32_t K = 8;
+    uint32_t waves = 0;
+    std::vector<BodyStatic> h_static;
+    BodyStatic* d_static = nullptr;
+    uint2* d_chunks = nullptr;
+    BodyDyn* d_dyn = nullptr;
+    double* d_partials = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool timing = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool ev_pending = false;
+    double face_ms_sum = 0.0, total_ms_sum = 0.0;
+    uint32_t timed = 0;
+    int device = 0;
+};
+
+static OceanK make_oceank(const sfh_ocean_t* o)
+{
+    OceanK k;
+    memset(&k, 0, sizeof(k));
+    k.rho = o->rho;
+    for (int i = 0; i < 3; ++i) k.g[i] = o->g[i];
+    k.nlayers = o->nlayers;
+    k.n = (int32_t)o->n;
+    for (int l = 0; l < kMaxLayers; ++l) {
+        k.len[l] = o->len[l];
+        k.quad[l] = o->d_quad[l];
+    }
+    if (o->currents_enabled)
+        for (const Current& c : o->currents)
+            if (c.enabled && c.type == 0)
+                for (int i = 0; i < 3; ++i) k.u[i] += c.p[i];
+    return k;
+}
+
+extern "C" int sfh_init(int device)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(SFH_ERR_NODEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(SFH_ERR_INVALID, "device %d out of range", device);
+    HIPCHK(hipSetDevice(device));
+    return SFH_OK;
+}
+
+// ---- meshes -----------------------------------------------------------------
+
+extern "C" int sfh_mesh_create(const void* vertex_pos, size_t vertex_stride, uint32_t nv,
+                               const uint32_t* face_ids, uint32_t nf, sfh_mesh* out)
+{
+    if (!out || (!vertex_pos && nv) || (!face_ids && nf)) return fail(SFH_ERR_INVALID, "null argument");
+    if (vertex_stride < 12) return fail(SFH_ERR_INVALID, "vertex stride %zu < 12", vertex_stride);
+    if (nf > (1u << 30)) return fail(SFH_ERR_SIZE, "too many faces");
+    std::vector<float> v(3 * (size_t)nv);
+    const char* src = (const char*)vertex_pos;
+    float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < nv; ++i) {
+        float p[3];
+        memcpy(p, src + (size_t)i * vertex_stride, 12);
+        for (int k = 0; k < 3; ++k) {
+            v[3 * (size_t)i + k] = p[k];
+            if (i == 0 || p[k] < mn[k]) mn[k] = p[k];
+            if (i == 0 || p[k] > mx[k]) mx[k] = p[k];
+        }
+    }
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i)
+        if (face_ids[i] >= nv) return fail(SFH_ERR_INVALID, "face %zu references vertex %u >= %u", i / 3, face_ids[i], nv);
+    sfh_mesh_t* m = new sfh_mesh_t();
+    m->nv = nv;
+    m->nf = nf;
+    for (int k = 0; k < 3; ++k) { m->aabb_min[k] = mn[k]; m->aabb_max[k] = mx[k]; }
+    if (nv) {
+        hipError_t e = hipMalloc(&m->d_verts, v.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(m->d_verts, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hipFree(m->d_verts); delete m; return fail(SFH_ERR_HIP, "mesh upload: %s", hipGetErrorString(e)); }
+    }
+    if (nf) {
+        hipError_t e = hipMalloc(&m->d_faces, 3 * (size_t)nf * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemcpy(m->d_faces, face_ids, 3 * (size_t)nf * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hipFree(m->d_verts); hipFree(m->d_faces); delete m; return fail(SFH_ERR_HIP, "mesh upload: %s", hipGetErrorString(e)); }
+    }
+    *out = m;
+    return SFH_OK;
+}
+
+extern "C" int sfh_mesh_destroy(sfh_mesh m)
+{
+    if (!m) return fail(SFH_ERR_INVALID, "null mesh");
+    hipFree(m->d_verts);
+    hipFree(m->d_faces);
+    delete m;
+    return SFH_OK;
+}
+
+extern "C" int sfh_mesh_info(sfh_mesh m, uint32_t* nv, uint32_t* nf, float aabb_min[3], float aabb_max[3])
+{
+    if (!m) return fail(SFH_ERR_INVALID, "null mesh");
+    if (nv) *nv = m->nv;
+    if (nf) *nf = m->nf;
+    for (int k = 0; k < 3; ++k) {
+        if (aabb_min) aabb_min[k] = m->aabb_min[k];
+        if (aabb_max) aabb_max[k] = m->aabb_max[k];
+    }
+    return SFH_OK;
+}
+
+// ---- ocean ------------------------------------------------------------------
+
+extern "C" int sfh_ocean_create(double density, const double gravity[3], sfh_ocean* out)
+{
+    if (!out || !gravity) return fail(SFH_ERR_INVALID, "null argument");
+    sfh_ocean_t* o = new sfh_ocean_t();
+    o->rho = density;
+    for (int k = 0; k < 3; ++k) o->g[k] = gravity[k];
+    *out = o;
+    return SFH_OK;
+}
+
+static void ocean_free_waves(sfh_ocean_t* o)
+{
+    for (int l = 0; l < kMaxLayers; ++l) {
+        if (o->d_quad[l]) hipFree(o->d_quad[l]);
+        o->d_quad[l] = nullptr;
+    }
+    o->nlayers = 0;
+    o->n = 0;
+}
+
+extern "C" int sfh_ocean_destroy(sfh_ocean o)
+{
+    if (!o) return fail(SFH_ERR_INVALID, "null ocean");
+    ocean_free_waves(o);
+    delete o;
+    return SFH_OK;
+}
+
+extern "C" int sfh_ocean_set_waves(sfh_ocean o, const float* data, int host_data, uint32_t n,
+                                   uint32_t channels, uint32_t layers, const float* lengths)
+{
+    if (!o) return fail(SFH_ERR_INVALID, "null ocean");
+    HIPCHK(hipDeviceSynchronize());
+    ocean_free_waves(o);
+    if (!data || layers == 0) return SFH_OK;
+    if (layers > (uint32_t)kMaxLayers || channels < layers || n < 2 || n > 16384 || !lengths)
+        return fail(SFH_ERR_INVALID, "bad wave layout (n %u, channels %u, layers %u)", n, channels, layers);
+    for (uint32_t l = 0; l < layers; ++l)
+        if (!(lengths[l] > 0.f)) return fail(SFH_ERR_INVALID, "layer %u length must be positive", l);
+    size_t count = (size_t)n * n * channels;
+    float* d_data = nullptr;
+    const float* src = data;
+    if (host_data) {
+        HIPCHK(hipMalloc(&d_data, count * sizeof(float)));
+        HIPCHK(hipMemcpy(d_data, data, count * sizeof(float), hipMemcpyHostToDevice));
+        src = d_data;
+    }
+    for (uint32_t l = 0; l < layers; ++l) {
+        hipError_t e = hipMalloc(&o->d_quad[l], (size_t)n * n * sizeof(float4));
+        if (e != hipSuccess) { hipFree(d_data); ocean_free_waves(o); return fail(SFH_ERR_HIP, "wave alloc: %s", hipGetErrorString(e)); }
+        dim3 grid((n + 255) / 256, n);
+        hipLaunchKernelGGL(k_wave_quad, grid, dim3(256), 0, 0, src, n, channels, l, o->d_quad[l]);
+        o->len[l] = (double)lengths[l];
+    }
+    hipError_t e = hipDeviceSynchronize();
+    if (d_data) hipFree(d_data);
+    if (e != hipSuccess) { ocean_free_waves(o); return fail(SFH_ERR_HIP, "wave setup: %s", hipGetErrorString(e)); }
+    o->nlayers = (int)layers;
+    o->n = n;
+    return SFH_OK;
+}
+
+extern "C" int sfh_ocean_add_current(sfh_ocean o, int type, int enabled, const double params[12])
+{
+    if (!o || !params) return fail(SFH_ERR_INVALID, "null argument");
+    if (type != 0) return fail(SFH_ERR_UNSUPPORTED, "only uniform currents are implemented on the device");
+    Current c;
+    c.type = type;
+    c.enabled = enabled;
+    memcpy(c.p, params, sizeof(c.p));
+    o->currents.push_back(c);
+    return SFH_OK;
+}
+
+extern "C" int sfh_ocean_enable_currents(sfh_ocean o, int enabled)
+{
+    if (!o) return fail(SFH_ERR_INVALID, "null ocean");
+    o->currents_enabled = enabled != 0;
+    return SFH_OK;
+}
+
+// ---- batches ----------------------------------------------------------------
+
+static void gl_to_rows(const double* m, double* B, double* t)
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) B[3 * r + c] = m[4 * c + r];
+    if (t) { t[0] = m[12]; t[1] = m[13]; t[2] = m[14]; }
+}
+
+static void batch_free(sfh_batch_t* b)
+{
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (int i = 0; i < 3; ++i)
+        if (b->ev[i]) hipEventDestroy(b->ev[i]);
+    hipFree(b->d_static);
+    hipFree(b->d_chunks);
+    hipFree(b->d_dyn);
+    hipFree(b->d_partials);
+}
+
+static uint32_t default_waves()
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
+    }
+    return (uint32_t)cus * 32u;
+}
+
+// (Re)builds the face-prefix chunk list for chunk size 64*K.
+static int batch_build_chunks(sfh_batch_t* b)
+{
+    std::vector<uint2> chunks;
+    const uint32_t csize = 64u * b->K;
+    for (uint32_t i = 0; i < b->n; ++i) {
+        BodyStatic& s = b->h_static[i];
+        s.chunk0 = (uint32_t)chunks.size();
+        uint32_t nc = (s.nf + csize - 1) / csize;
+        s.nchunks = nc;
+        for (uint32_t c = 0; c < nc; ++c) chunks.push_back(make_uint2(i, c * csize));
+    }
+    hipFree(b->d_static);
+    hipFree(b->d_chunks);
+    hipFree(b->d_partials);
+    b->d_static = nullptr; b->d_chunks = nullptr; b->d_partials = nullptr;
+    b->nchunks = (uint32_t)chunks.size();
+    HIPCHK(hipMalloc(&b->d_static, sizeof(BodyStatic) * std::max<size_t>(1, b->n)));
+    HIPCHK(hipMemcpy(b->d_static, b->h_static.data(), sizeof(BodyStatic) * b->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b->d_chunks, sizeof(uint2) * std::max<size_t>(1, chunks.size())));
+    if (!chunks.empty())
+        HIPCHK(hipMemcpy(b->d_chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b->d_partials, sizeof(double) * kChunkDoubles * std::max<size_t>(1, chunks.size())));
+    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    return SFH_OK;
+}
+
+static int apply_opts(sfh_batch_t* b, const sfh_batch_opts* opts)
+{
+    uint32_t K = opts && opts->faces_per_lane ? opts->faces_per_lane : 8u;
+    if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16 && K != 32)
+        return fail(SFH_ERR_INVALID, "faces_per_lane must be 1, 2, 4, 8, 16 or 32");
+    b->waves = opts && opts->waves ? opts->waves : default_waves();
+    if (K != b->K || !b->d_chunks) {
+        b->K = K;
+        return batch_build_chunks(b);
+    }
+    return SFH_OK;
+}
+
+static BodyStatic make_static(const sfh_mesh_t* m, uint32_t flags)
+{
+    BodyStatic s;
+    memset(&s, 0, sizeof(s));
+    s.verts = m ? m->d_verts : nullptr;
+    s.faces = m ? m->d_faces : nullptr;
+    s.nf = m ? m->nf : 0;
+    s.flags = flags;
+    for (int i = 0; i < 3; ++i) s.cg2c[4 * i] = s.cg2o[4 * i] = 1.0;
+    return s;
+}
+
+extern "C" int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const sfh_batch_opts* opts, sfh_batch* out)
+{
+    if (!out || (!bodies && n)) return fail(SFH_ERR_INVALID, "null argument");
+    sfh_batch_t* b = new sfh_batch_t();
+    b->n = n;
+    hipGetDevice(&b->device);
+    b->h_static.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const sfh_body_desc& d = bodies[i];
+        uint32_t flags = 0;
+        if (d.hydro) flags |= BF_HYDRO;
+        if (d.buoyant) flags |= BF_BUOYANT;
+        BodyStatic s = make_static(d.mesh, flags);
+        gl_to_rows(d.T_CG2C, s.cg2c, s.cg2c + 9);
+        gl_to_rows(d.T_CG2O, s.cg2o, nullptr);
+        for (int k = 0; k < 3; ++k) {
+            s.aabb_c[k] = 0.5 * (d.aabb_max[k] + d.aabb_min[k]);
+            s.aabb_h[k] = 0.5 * (d.aabb_max[k] - d.aabb_min[k]);
+            s.Cd[k] = d.Cd[k];
+            s.Cf[k] = d.Cf[k];
+            s.P_CB[k] = d.P_CB[k];
+        }
+        s.volume = d.volume;
+        s.surface = d.surface;
+        b->h_static[i] = s;
+        b->nfaces += s.nf;
+    }
+    b->K = 0;
+    int rc = hipMalloc(&b->d_dyn, sizeof(BodyDyn) * std::max<size_t>(1, n)) == hipSuccess ? SFH_OK
+             : fail(SFH_ERR_HIP, "batch alloc failed");
+    if (rc == SFH_OK) rc = apply_opts(b, opts);
+    if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
+    *out = b;
+    return SFH_OK;
+}
+
+extern "C" int sfh_batch_destroy(sfh_batch b)
+{
+    if (!b) return fail(SFH_ERR_INVALID, "null batch");
+    hipDeviceSynchronize();
+    batch_free(b);
+    delete b;
+    return SFH_OK;
+}
+
+extern "C" int sfh_batch_set_opts(sfh_batch b, const sfh_batch_opts* opts)
+{
+    if (!b) return fail(SFH_ERR_INVALID, "null batch");
+    return apply_opts(b, opts);
+}
+
+extern "C" int sfh_batch_info(sfh_batch b, uint32_t* nbodies, uint32_t* nchunks, uint64_t* nfaces)
+{
+    if (!b) return fail(SFH_ERR_INVALID, "null batch");
+    if (nbodies) *nbodies = b->n;
+    if (nchunks) *nchunks = b->nchunks;
+    if (nfaces) *nfaces = b->nfaces;
+    return SFH_OK;
+}
+
+static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s)
+{
+    if (b->nchunks == 0) return SFH_OK;
+    uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
+    uint32_t blocks = (waves + 3) / 4;
+#define LAUNCH_K(KK)                                                                                     \
+    case KK:                                                                                             \
+        hipLaunchKernelGGL(k_faces<KK>, dim3(blocks), dim3(256), 0, s, b->d_chunks, b->nchunks,         \
+                           b->d_static, b->d_dyn, oc, b->d_partials);                                   \
+        break;
+    switch (b->K) {
+        LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16) LAUNCH_K(32)
+        default: return fail(SFH_ERR_INVALID, "bad K");
+    }
+#undef LAUNCH_K
+    return SFH_OK;
+}
+
+static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_states, int kind, double* d_out,
+                        hipStream_t s, bool timed)
+{
+    if (b->n == 0) return SFH_OK;
+    OceanK oc = make_oceank(o);
+    uint32_t blocks = (b->n + 127) / 128;
+    if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
+    hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, b->d_dyn);
+    if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
+    int rc = launch_faces(b, oc, s);
+    if (rc != SFH_OK) return rc;
+    if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
+    hipLaunchKernelGGL(k_epilogue, dim3(blocks), dim3(128), 0, s, b->d_static, b->d_dyn, b->n, oc, b->d_partials, d_out);
+    HIPCHK(hipGetLastError());
+    return SFH_OK;
+}
+
+static int check_ocean(const sfh_ocean_t* o)
+{
+    if (!o) return fail(SFH_ERR_INVALID, "null ocean");
+    if (o->currents_enabled)
+        for (const Current& c : o->currents)
+            if (c.enabled && c.type != 0) return fail(SFH_ERR_UNSUPPORTED, "non-uniform current");
+    return SFH_OK;
+}
+
+static void harvest_timing(sfh_batch_t* b)
+{
+    if (!b->ev_pending) return;
+    float t_face = 0.f, t_tot = 0.f;
+    if (hipEventSynchronize(b->ev[2]) == hipSuccess &&
+        hipEventElapsedTime(&t_face, b->ev[1], b->ev[2]) == hipSuccess &&
+        hipEventElapsedTime(&t_tot, b->ev[0], b->ev[2]) == hipSuccess) {
+        b->face_ms_sum += t_face;
+        b->total_ms_sum += t_tot;
+        b->timed++;
+    }
+    b->ev_pending = false;
+}
+
+extern "C" int sfh_batch_compute(sfh_batch b, sfh_ocean o, const double* d_states, double* d_out, void* stream)
+{
+    if (!b || !d_states || !d_out) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(o);
+    if (rc != SFH_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    bool timed = b->timing;
+    if (timed) harvest_timing(b);
+    rc = batch_launch(b, o, d_states, 0, d_out, s, timed);
+    if (rc == SFH_OK && timed) b->ev_pending = true;
+    return rc;
+}
+
+extern "C" int sfh_batch_compute_host(sfh_batch b, sfh_ocean o, const double* states, double* out)
+{
+    if (!b || !states || !out) return fail(SFH_ERR_INVALID, "null argument");
+    double *ds = nullptr, *dout = nullptr;
+    size_t ns = (size_t)b->n * SFH_STATE_DOUBLES, no = (size_t)b->n * SFH_OUT_DOUBLES;
+    HIPCHK(hipMalloc(&ds, sizeof(double) * std::max<size_t>(1, ns)));
+    hipError_t e = hipMalloc(&dout, sizeof(double) * std::max<size_t>(1, no));
+    if (e != hipSuccess) { hipFree(ds); return fail(SFH_ERR_HIP, "alloc: %s", hipGetErrorString(e)); }
+    int rc = SFH_OK;
+    e = hipMemcpy(ds, states, sizeof(double) * ns, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dout, out, sizeof(double) * no, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = sfh_batch_compute(b, o, ds, dout, nullptr);
+        if (rc == SFH_OK) e = hipMemcpy(out, dout, sizeof(double) * no, hipMemcpyDeviceToHost);
+    }
+    hipFree(ds);
+    hipFree(dout);
+    if (e != hipSuccess) return fail(SFH_ERR_HIP, "batch host compute: %s", hipGetErrorString(e));
+    return rc;
+}
+
+extern "C" int sfh_batch_capture(sfh_batch b, sfh_ocean o, const double* d_states, double* d_out, void* stream)
+{
+    if (!b || !d_states || !d_out) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(o);
+    if (rc != SFH_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return fail(SFH_ERR_INVALID, "graph capture needs a non-default stream");
+    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = batch_launch(b, o, d_states, 0, d_out, s, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != SFH_OK) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(SFH_ERR_HIP, "end capture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    return SFH_OK;
+}
+
+extern "C" int sfh_batch_replay(sfh_batch b, void* stream)
+{
+    if (!b || !b->graph_exec) return fail(SFH_ERR_INVALID, "no captured graph");
+    HIPCHK(hipGraphLaunch(b->graph_exec, (hipStream_t)stream));
+    return SFH_OK;
+}
+
+extern "C" int sfh_batch_timing(sfh_batch b, int enable)
+{
+    if (!b) return fail(SFH_ERR_INVALID, "null batch");
+    if (enable && !b->ev[0])
+        for (int i = 0; i < 3; ++i) HIPCHK(hipEventCreate(&b->ev[i]));
+    if (enable) harvest_timing(b);
+    b->timing = enable != 0;
+    b->face_ms_sum = b->total_ms_sum = 0.0;
+    b->timed = 0;
+    b->ev_pending = false;
+    return SFH_OK;
+}
+
+extern "C" int sfh_batch_kernel_ms(sfh_batch b, double* face_ms, double* total_ms, uint32_t* count)
+{
+    if (!b) return fail(SFH_ERR_INVALID, "null batch");
+    harvest_timing(b);
+    if (count) *count = b->timed;
+    if (face_ms) *face_ms = b->timed ? b->face_ms_sum / b->timed : 0.0;
+    if (total_ms) *total_ms = b->timed ? b->total_ms_sum / b->timed : 0.0;
+    return SFH_OK;
+}
+
+// ---- per-body reference-shaped calls ------------------------------------------
+
+// One reusable single-body workspace; the reference's statics are called from
+// pool threads, so calls are serialised here.
+static std::mutex g_single_mu;
+struct Single {
+    sfh_batch_t* b = nullptr;
+    const sfh_mesh_t* mesh = nullptr;
+    uint32_t flags = 0;
+    double* d_state = nullptr;
+    double* d_out = nullptr;
+    double* d_aux = nullptr;
+    int device = -1;
+};
+static Single g_single;
+
+static int single_alloc()
+{
+    Single& S = g_single;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (S.device != dev) {   // workspace belongs to another device: start afresh
+        S.b = nullptr;
+        S.d_state = S.d_out = S.d_aux = nullptr;
+        S.device = dev;
+    }
+    if (!S.d_state) {
+        HIPCHK(hipMalloc(&S.d_state, sizeof(double) * 64));
+        HIPCHK(hipMalloc(&S.d_out, sizeof(double) * SFH_OUT_DOUBLES));
+        HIPCHK(hipMalloc(&S.d_aux, sizeof(double) * 32));
+    }
+    return SFH_OK;
+}
+
+static int single_prepare(const sfh_mesh_t* m, uint32_t flags)
+{
+    Single& S = g_single;
+    int rc = single_alloc();
+    if (rc != SFH_OK) return rc;
+    int dev = S.device;
+    if (!S.b || S.mesh != m || S.flags != flags) {
+        if (S.b) { batch_free(S.b); delete S.b; S.b = nullptr; }
+        sfh_batch_t* b = new sfh_batch_t();
+        b->n = 1;
+        b->device = dev;
+        b->h_static.push_back(make_static(m, flags));
+        b->nfaces = b->h_static[0].nf;
+        b->K = 0;
+        int rc = hipMalloc(&b->d_dyn, sizeof(BodyDyn)) == hipSuccess ? SFH_OK : fail(SFH_ERR_HIP, "alloc failed");
+        sfh_batch_opts opts = {8, 0};
+        if (rc == SFH_OK) rc = apply_opts(b, &opts);
+        if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
+        S.b = b;
+        S.mesh = m;
+        S.flags = flags;
+    }
+    return SFH_OK;
+}
+
+static int single_run(const sfh_ocean_t* o, const double T_CG[16], const double T_C[16], const double v[3],
+                      const double w[3], double* out)
+{
+    Single& S = g_single;
+    double st[38];
+    memcpy(st, T_CG, 16 * sizeof(double));
+    memcpy(st + 16, T_C, 16 * sizeof(double));
+    memcpy(st + 32, v, 3 * sizeof(double));
+    memcpy(st + 35, w, 3 * sizeof(double));
+    HIPCHK(hipMemcpy(S.d_state, st, sizeof(st), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(S.d_out, 0, sizeof(double) * SFH_OUT_DOUBLES));
+    int rc = batch_launch(S.b, o, S.d_state, 1, S.d_out, nullptr, false);
+    if (rc != SFH_OK) return rc;
+    HIPCHK(hipMemcpy(out, S.d_out, sizeof(double) * SFH_OUT_DOUBLES, hipMemcpyDeviceToHost));
+    return SFH_OK;
+}
+
+extern "C" int sfh_forces_surface(const sfh_settings* settings, sfh_mesh mesh, sfh_ocean ocean,
+                                  const double T_CG[16], const double T_C[16], const double v[3], const double omega[3],
+                                  double Fb[3], double Tb[3], double Fdq[3], double Tdq[3], double Fdf[3],
+                                  double Tdf[3], double* Swet, double* Vsub)
+{
+    if (!settings || !T_CG || !T_C || !v || !omega || !Fb || !Tb || !Fdq || !Tdq || !Fdf || !Tdf || !Swet || !Vsub)
+        return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(ocean);
+    if (rc != SFH_OK) return rc;
+    if (!mesh) {   // SolidEntity.cpp:1293-1312
+        if (settings->realistic_buoyancy)
+            for (int k = 0; k < 3; ++k) Fb[k] = Tb[k] = 0.0;
+        if (settings->damping_forces)
+            for (int k = 0; k < 3; ++k) Fdq[k] = Tdq[k] = Fdf[k] = Tdf[k] = 0.0;
+        *Swet = 0.0;
+        *Vsub = 0.0;
+        return SFH_OK;
+    }
+    uint32_t flags = BF_HYDRO | BF_MODE_SURF | BF_EXPLICIT_TC;
+    if (settings->damping_forces) flags |= BF_DAMPING;
+    if (settings->realistic_buoyancy) flags |= BF_BUOYANT;
+    std::lock_guard<std::mutex> lk(g_single_mu);
+    rc = single_prepare(mesh, flags);
+    if (rc != SFH_OK) return rc;
+    double out[SFH_OUT_DOUBLES];
+    rc = single_run(ocean, T_CG, T_C, v, omega, out);
+    if (rc != SFH_OK) return rc;
+    uint32_t wm = (uint32_t)out[20];
+    if (wm & 1u) memcpy(Fb, out, 24);
+    if (wm & 2u) memcpy(Tb, out + 3, 24);
+    if (wm & 4u) memcpy(Fdq, out + 6, 24);
+    if (wm & 8u) memcpy(Tdq, out + 9, 24);
+    if (wm & 16u) memcpy(Fdf, out + 12, 24);
+    if (wm & 32u) memcpy(Tdf, out + 15, 24);
+    if (wm & 64u) *Swet = out[18];
+    if (wm & 128u) *Vsub = out[19];
+    return SFH_OK;
+}
+
+extern "C" int sfh_forces_submerged(sfh_mesh mesh, sfh_ocean ocean, const double T_CG[16], const double T_C[16],
+                                    const double v[3], const double omega[3], double Fdq[3], double Tdq[3],
+                                    double Fdf[3], double Tdf[3])
+{
+    if (!T_CG || !T_C || !v || !omega || !Fdq || !Tdq || !Fdf || !Tdf) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(ocean);
+    if (rc != SFH_OK) return rc;
+    if (!mesh) {
+        for (int k = 0; k < 3; ++k) Fdq[k] = Tdq[k] = Fdf[k] = Tdf[k] = 0.0;
+        return SFH_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_single_mu);
+    rc = single_prepare(mesh, BF_HYDRO | BF_MODE_SUBM | BF_EXPLICIT_TC | BF_DAMPING);
+    if (rc != SFH_OK) return rc;
+    double out[SFH_OUT_DOUBLES];
+    rc = single_run(ocean, T_CG, T_C, v, omega, out);
+    if (rc != SFH_OK) return rc;
+    memcpy(Fdq, out + 6, 24);
+    memcpy(Tdq, out + 9, 24);
+    memcpy(Fdf, out + 12, 24);
+    memcpy(Tdf, out + 15, 24);
+    return SFH_OK;
+}
+
+extern "C" int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],
+                           const double Cd[3], const double Cf[3], const double T_O[16])
+{
+    if (!ocean || !Fdq || !Tdq || !Fdf || !Tdf || !Cd || !Cf || !T_O) return fail(SFH_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(g_single_mu);
+    int rc = single_alloc();
+    if (rc != SFH_OK) return rc;
+    double in[15], io[12];
+    gl_to_rows(T_O, in, nullptr);
+    memcpy(in + 9, Cd, 24);
+    memcpy(in + 12, Cf, 24);
+    memcpy(io, Fdq, 24); memcpy(io + 3, Tdq, 24); memcpy(io + 6, Fdf, 24); memcpy(io + 9, Tdf, 24);
+    Single& S = g_single;
+    HIPCHK(hipMemcpy(S.d_aux, in, sizeof(in), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S.d_aux + 16, io, sizeof(io), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_correct, dim3(1), dim3(64), 0, 0, ocean->rho, S.d_aux, S.d_aux + 16);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(io, S.d_aux + 16, sizeof(io), hipMemcpyDeviceToHost));
+    memcpy(Fdq, io, 24); memcpy(Tdq, io + 3, 24); memcpy(Fdf, io + 6, 24); memcpy(Tdf, io + 9, 24);
+    return SFH_OK;
+}
+
+// ---- wave grid ----------------------------------------------------------------
+
+extern "C" int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_t n, float* d_grid,
+                                   void* stream)
+{
+    if ((!comps && ncomp) || !d_grid) return fail(SFH_ERR_INVALID, "null argument");
+    if (n < 2 || n > 16384 || ncomp > 4096) return fail(SFH_ERR_SIZE, "bad grid size or component count");
+    sfh_wave_component* d_c = nullptr;
+    HIPCHK(hipMalloc(&d_c, sizeof(sfh_wave_component) * std::max<uint32_t>(1, ncomp)));
+    hipStream_t s = (hipStream_t)stream;
+    if (ncomp) HIPCHK(hipMemcpyAsync(d_c, comps, sizeof(sfh_wave_component) * ncomp, hipMemcpyHostToDevice, s));
+    dim3 grid((n + 255) / 256, n);
+    hipLaunchKernelGGL(k_wave_grid, grid, dim3(256), 0, s, d_c, ncomp, n, d_grid);
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(d_c);
+    if (e != hipSuccess) return fail(SFH_ERR_HIP, "wave grid: %s", hipGetErrorString(e));
+    return SFH_OK;
+}

@@ -1,0 +1,280 @@
+"""Host-side mirror of the reference's hydrodynamics interface, over the C ABI.
+
+Names and argument meaning follow the reference:
+  HydrodynamicsSettings             Ocean.h:36-40
+  ComputeHydrodynamicForcesSurface  SolidEntity.h:135-137 / SolidEntity.cpp:1289-1721
+  ComputeHydrodynamicForcesSubmerged SolidEntity.h:152-153 / SolidEntity.cpp:1723-1797
+  CorrectHydrodynamicForces         SolidEntity.h:113-114 / SolidEntity.cpp:1263-1287
+  BodyBatch                         the per-body fan-out of SimulationManager.cpp:1619-1655
+                                    around SolidEntity::ComputeHydrodynamicForces (1799-1850)
+
+Output vectors are numpy float64 arrays of shape (3,), written in place; the
+reference's by-reference scalars (Swet, Vsub) live in a (2,) array.  Outputs the
+reference leaves untouched keep their values.  Every computation runs in
+libsfhydro.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .meshes import TriMesh
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(N.c_double_p)
+
+
+def _vec(x, n=3) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def init(device: int = 0) -> None:
+    N.check(N.load().sfh_init(device))
+
+
+@dataclass
+class HydrodynamicsSettings:
+    dampingForces: bool = True
+    reallisticBuoyancy: bool = True
+
+
+class Mesh:
+    """Device-resident triangle mesh (the reference's `const Mesh*`)."""
+
+    def __init__(self, mesh: TriMesh):
+        lib = N.load()
+        v = np.ascontiguousarray(mesh.verts, dtype=np.float32)
+        f = np.ascontiguousarray(mesh.faces, dtype=np.uint32)
+        h = C.c_void_p()
+        N.check(lib.sfh_mesh_create(v.ctypes.data_as(C.c_void_p), 12, v.shape[0],
+                                    f.ctypes.data_as(N.c_uint32_p), f.shape[0], C.byref(h)))
+        self.handle = h
+        self.nv, self.nf = int(v.shape[0]), int(f.shape[0])
+        self.aabb_min = v.min(axis=0).astype(np.float64) if v.size else np.zeros(3)
+        self.aabb_max = v.max(axis=0).astype(np.float64) if v.size else np.zeros(3)
+
+    def close(self) -> None:
+        if self.handle:
+            N.check(N.load().sfh_mesh_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ocean:
+    """Liquid, free surface and currents (the reference's `Ocean*`)."""
+
+    def __init__(self, density: float = 1000.0, gravity=(0.0, 0.0, 9.81)):
+        lib = N.load()
+        h = C.c_void_p()
+        g = _vec(gravity)
+        N.check(lib.sfh_ocean_create(float(density), _dp(g), C.byref(h)))
+        self.handle = h
+        self.density = float(density)
+        self.gravity = g
+
+    def set_waves(self, data, n: int, channels: int, lengths, device_ptr: int | None = None) -> None:
+        """Wave height texels (n x n x channels, float32); layer l = channel l,
+        physical size lengths[l]; None for a flat surface."""
+        lib = N.load()
+        if data is None and device_ptr is None:
+            N.check(lib.sfh_ocean_set_waves(self.handle, None, 1, 0, 0, 0, None))
+            return
+        L = np.ascontiguousarray(np.asarray(lengths, dtype=np.float32).reshape(-1))
+        if device_ptr is not None:
+            N.check(lib.sfh_ocean_set_waves(self.handle, C.c_void_p(device_ptr), 0, n, channels, L.size,
+                                            L.ctypes.data_as(N.c_float_p)))
+        else:
+            d = np.ascontiguousarray(data, dtype=np.float32)
+            assert d.size == n * n * channels
+            N.check(lib.sfh_ocean_set_waves(self.handle, d.ctypes.data_as(C.c_void_p), 1, n, channels, L.size,
+                                            L.ctypes.data_as(N.c_float_p)))
+
+    def add_uniform_current(self, v, enabled: bool = True) -> None:
+        p = np.zeros(12)
+        p[:3] = _vec(v)
+        N.check(N.load().sfh_ocean_add_current(self.handle, 0, int(enabled), _dp(p)))
+
+    def add_current(self, kind: int, params, enabled: bool = True) -> None:
+        p = np.zeros(12)
+        params = np.asarray(params, dtype=np.float64).reshape(-1)
+        p[:params.size] = params
+        N.check(N.load().sfh_ocean_add_current(self.handle, int(kind), int(enabled), _dp(p)))
+
+    def enable_currents(self, on: bool = True) -> None:
+        N.check(N.load().sfh_ocean_enable_currents(self.handle, int(on)))
+
+    def close(self) -> None:
+        if self.handle:
+            N.check(N.load().sfh_ocean_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ComputeHydrodynamicForcesSurface(settings: HydrodynamicsSettings, mesh: Mesh | None, ocn: Ocean,
+                                     T_CG, T_C, v, omega, Fb, Tb, Fdq, Tdq, Fdf, Tdf, SwetVsub) -> None:
+    """SolidEntity::ComputeHydrodynamicForcesSurface (SolidEntity.cpp:1289-1721)."""
+    lib = N.load()
+    s = N.sfh_settings(int(settings.dampingForces), int(settings.reallisticBuoyancy))
+    T_CG, T_C, v, omega = _vec(T_CG, 16), _vec(T_C, 16), _vec(v), _vec(omega)
+    sw = np.zeros(1)
+    vs = np.zeros(1)
+    sw[0], vs[0] = SwetVsub[0], SwetVsub[1]
+    N.check(lib.sfh_forces_surface(C.byref(s), mesh.handle if mesh is not None else None, ocn.handle,
+                                   _dp(T_CG), _dp(T_C), _dp(v), _dp(omega), _dp(Fb), _dp(Tb), _dp(Fdq), _dp(Tdq),
+                                   _dp(Fdf), _dp(Tdf), _dp(sw), _dp(vs)))
+    SwetVsub[0], SwetVsub[1] = sw[0], vs[0]
+
+
+def ComputeHydrodynamicForcesSubmerged(mesh: Mesh | None, ocn: Ocean, T_CG, T_C, v, omega,
+                                       Fdq, Tdq, Fdf, Tdf) -> None:
+    """SolidEntity::ComputeHydrodynamicForcesSubmerged (SolidEntity.cpp:1723-1797)."""
+    lib = N.load()
+    T_CG, T_C, v, omega = _vec(T_CG, 16), _vec(T_C, 16), _vec(v), _vec(omega)
+    N.check(lib.sfh_forces_submerged(mesh.handle if mesh is not None else None, ocn.handle, _dp(T_CG), _dp(T_C),
+                                     _dp(v), _dp(omega), _dp(Fdq), _dp(Tdq), _dp(Fdf), _dp(Tdf)))
+
+
+def CorrectHydrodynamicForces(ocn: Ocean, Fdq, Tdq, Fdf, Tdf, fdCd, fdCf, T_O) -> None:
+    """SolidEntity::CorrectHydrodynamicForces (SolidEntity.cpp:1263-1287)."""
+    lib = N.load()
+    Cd, Cf, T_O = _vec(fdCd), _vec(fdCf), _vec(T_O, 16)
+    N.check(lib.sfh_correct(ocn.handle, _dp(Fdq), _dp(Tdq), _dp(Fdf), _dp(Tdf), _dp(Cd), _dp(Cf), _dp(T_O)))
+
+
+@dataclass
+class BodyDesc:
+    """What SolidEntity::ComputeHydrodynamicForces reads from the entity besides
+    its state (SolidEntity.h:400-437)."""
+    mesh: Mesh
+    T_CG2C: np.ndarray = field(default_factory=lambda: _ident())
+    T_CG2O: np.ndarray = field(default_factory=lambda: _ident())
+    aabb_min: np.ndarray | None = None   # CG-frame collision AABB; default: mesh AABB through T_CG2C
+    aabb_max: np.ndarray | None = None
+    Cd: np.ndarray = field(default_factory=lambda: np.ones(3))
+    Cf: np.ndarray = field(default_factory=lambda: 0.1 * np.ones(3))
+    volume: float = 0.0
+    surface: float = 0.0
+    P_CB: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    buoyant: bool = True
+    hydro: bool = True
+
+
+def _ident():
+    m = np.zeros(16)
+    m[[0, 5, 10, 15]] = 1.0
+    return m
+
+
+def cg_frame_aabb(mesh_min, mesh_max, T_CG2C, margin: float = 0.0):
+    """btTransformAabb of the collision child's AABB by T_CG2C (btAabbUtil2.h:182-196):
+    the btCompoundShape local AABB of a single-solid body."""
+    T = np.asarray(T_CG2C, dtype=np.float64)
+    B = np.stack([T[0:3], T[4:7], T[8:11]], axis=1)
+    h = 0.5 * (np.asarray(mesh_max) - np.asarray(mesh_min)) + margin
+    c = 0.5 * (np.asarray(mesh_max) + np.asarray(mesh_min))
+    wc = B @ c + T[12:15]
+    e = np.abs(B) @ h
+    return wc - e, wc + e
+
+
+class BodyBatch:
+    """Many bodies evaluated in one pass (SolidEntity::ComputeHydrodynamicForces
+    for each, with Ocean::ApplyFluidForces' settings {damping, buoyancy} = {true, true})."""
+
+    def __init__(self, bodies: list[BodyDesc], faces_per_lane: int = 0, waves: int = 0):
+        lib = N.load()
+        arr = (N.sfh_body_desc * max(1, len(bodies)))()
+        for i, b in enumerate(bodies):
+            d = arr[i]
+            d.mesh = b.mesh.handle
+            d.T_CG2C[:] = list(_vec(b.T_CG2C, 16))
+            d.T_CG2O[:] = list(_vec(b.T_CG2O, 16))
+            if b.aabb_min is None:
+                mn, mx = cg_frame_aabb(b.mesh.aabb_min, b.mesh.aabb_max, b.T_CG2C)
+            else:
+                mn, mx = b.aabb_min, b.aabb_max
+            d.aabb_min[:] = list(_vec(mn))
+            d.aabb_max[:] = list(_vec(mx))
+            d.Cd[:] = list(_vec(b.Cd))
+            d.Cf[:] = list(_vec(b.Cf))
+            d.volume = float(b.volume)
+            d.surface = float(b.surface)
+            d.P_CB[:] = list(_vec(b.P_CB))
+            d.buoyant = int(b.buoyant)
+            d.hydro = int(b.hydro)
+        opts = N.sfh_batch_opts(faces_per_lane, waves)
+        h = C.c_void_p()
+        N.check(lib.sfh_batch_create(arr, len(bodies), C.byref(opts), C.byref(h)))
+        self.handle = h
+        self.n = len(bodies)
+        self.bodies = bodies   # keep meshes alive
+
+    def info(self):
+        nb, nc, nf = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        N.check(N.load().sfh_batch_info(self.handle, C.byref(nb), C.byref(nc), C.byref(nf)))
+        return nb.value, nc.value, nf.value
+
+    def set_opts(self, faces_per_lane: int = 0, waves: int = 0) -> None:
+        opts = N.sfh_batch_opts(faces_per_lane, waves)
+        N.check(N.load().sfh_batch_set_opts(self.handle, C.byref(opts)))
+
+    def compute_host(self, ocn: Ocean, states: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
+        """states: (n, 22) float64 = T_CG[16], v[3], omega[3]; returns (n, 24)."""
+        st = np.ascontiguousarray(states, dtype=np.float64).reshape(self.n, N.SFH_STATE_DOUBLES)
+        if out is None:
+            out = np.zeros((self.n, N.SFH_OUT_DOUBLES))
+        N.check(N.load().sfh_batch_compute_host(self.handle, ocn.handle, _dp(st), _dp(out)))
+        return out
+
+    def compute(self, ocn: Ocean, d_states: int, d_out: int, stream: int = 0) -> None:
+        """Device pointers (e.g. torch tensor data_ptr()); asynchronous on `stream`."""
+        N.check(N.load().sfh_batch_compute(self.handle, ocn.handle, C.c_void_p(d_states), C.c_void_p(d_out),
+                                           C.c_void_p(stream)))
+
+    def capture(self, ocn: Ocean, d_states: int, d_out: int, stream: int) -> None:
+        N.check(N.load().sfh_batch_capture(self.handle, ocn.handle, C.c_void_p(d_states), C.c_void_p(d_out),
+                                           C.c_void_p(stream)))
+
+    def replay(self, stream: int) -> None:
+        N.check(N.load().sfh_batch_replay(self.handle, C.c_void_p(stream)))
+
+    def timing(self, on: bool) -> None:
+        N.check(N.load().sfh_batch_timing(self.handle, int(on)))
+
+    def kernel_ms(self):
+        f, t, c = C.c_double(), C.c_double(), C.c_uint32()
+        N.check(N.load().sfh_batch_kernel_ms(self.handle, C.byref(f), C.byref(t), C.byref(c)))
+        return f.value, t.value, c.value
+
+    def close(self) -> None:
+        if self.handle:
+            N.check(N.load().sfh_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+OUT_FIELDS = {"Fb": slice(0, 3), "Tb": slice(3, 6), "Fdq": slice(6, 9), "Tdq": slice(9, 12),
+              "Fdf": slice(12, 15), "Tdf": slice(15, 18), "Swet": 18, "Vsub": 19, "written": 20, "position": 21}

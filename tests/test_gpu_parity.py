@@ -1,0 +1,219 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bound: norm-relative error <= 1e-6 per F/T vector (|x - x64| / |x64|, with a
+floor of 1e-6 of the workload's force/torque scale for vectors near zero),
+relative 1e-6 on Swet and Vsub; write masks and body/fluid positions exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from stonefish_amd import hydro as H
+from stonefish_amd import meshes as M
+from stonefish_amd import transforms as T
+from stonefish_amd import waves as W
+from stonefish_amd import workloads as WL
+from util import compare_out
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-6
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def scales(out64):
+    F = max(np.abs(out64[:, 0:18:6]).max(), 1e-12)
+    Fs = max(np.linalg.norm(out64[:, [0, 1, 2, 6, 7, 8, 12, 13, 14]].reshape(-1, 3), axis=1).max(), 1e-12)
+    Ts = max(np.linalg.norm(out64[:, [3, 4, 5, 9, 10, 11, 15, 16, 17]].reshape(-1, 3), axis=1).max(), 1e-12)
+    del F
+    return Fs, Ts
+
+
+def assert_close(gpu, ref, fields=None, what=""):
+    Fs, Ts = scales(np.atleast_2d(ref))
+    kw = {} if fields is None else {"fields": fields}
+    worst = compare_out(gpu, ref, Fs, Ts, **kw)
+    log(f"{what} worst norm-rel errors: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()))
+    for k, v in worst.items():
+        assert v <= RTOL, f"{what}: {k} error {v:.3e} > {RTOL}"
+
+
+def test_submerged_capsule(gpu):
+    log("case: submerged capsule, 32 random states")
+    w = WL.capsule_submerged()
+    tm = w.bodies[0].mesh
+    gm, om = H.Mesh(tm), O.Mesh.from_tri(tm)
+    go, oo = H.Ocean(), O.Ocean()
+    st = WL.random_states(w, 32)[:, 0, :]
+    g = np.zeros((32, 24))
+    r = np.zeros((32, 24))
+    for s in range(32):
+        Fdq, Tdq, Fdf, Tdf = (np.zeros(3) for _ in range(4))
+        H.ComputeHydrodynamicForcesSubmerged(gm, go, st[s, :16], st[s, :16], st[s, 16:19], st[s, 19:22],
+                                             Fdq, Tdq, Fdf, Tdf)
+        g[s, 6:9], g[s, 9:12], g[s, 12:15], g[s, 15:18] = Fdq, Tdq, Fdf, Tdf
+        r[s] = O.submerged(64, om, oo, st[s, :16], st[s, :16], st[s, 16:19], st[s, 19:22])
+    assert_close(g, r, {k: v for k, v in [("Fdq", slice(6, 9)), ("Tdq", slice(9, 12)), ("Fdf", slice(12, 15)),
+                                           ("Tdf", slice(15, 18))]}, "submerged capsule")
+
+
+@pytest.mark.parametrize("waves", [False, True])
+def test_surface_ellipsoid(gpu, waves):
+    log(f"case: surface ellipsoid ico5, waves={waves}")
+    w = WL.ellipsoid_surface()
+    tm = w.bodies[0].mesh
+    gm, om = H.Mesh(tm), O.Mesh.from_tri(tm)
+    go = H.Ocean()
+    if waves:
+        comps = W.pm_components(16, 6.0, 64.0, seed=3)
+        grid = W.grid_reference(comps, 64).astype(np.float32)
+        go.set_waves(grid, 64, 1, [64.0])
+        oo = O.Ocean(wave_data=grid, wave_n=64, channels=1, lengths=[64.0])
+    else:
+        oo = O.Ocean()
+    st = WL.random_states(w, 16)[:, 0, :]
+    g = np.zeros((16, 24))
+    r = np.zeros((16, 24))
+    settings = H.HydrodynamicsSettings(True, True)
+    for s in range(16):
+        outs = [np.zeros(3) for _ in range(6)]
+        sv = np.zeros(2)
+        H.ComputeHydrodynamicForcesSurface(settings, gm, go, st[s, :16], st[s, :16], st[s, 16:19], st[s, 19:22],
+                                           *outs, sv)
+        g[s, :18] = np.concatenate(outs)
+        g[s, 18:20] = sv
+        r[s] = O.surface(64, 1, 1, om, oo, st[s, :16], st[s, :16], st[s, 16:19], st[s, 19:22])
+        assert r[s, 20] == 255
+    assert_close(g, r, what=f"surface ellipsoid waves={waves}")
+    np.testing.assert_allclose(g[:, 18], r[:, 18], rtol=RTOL)
+    np.testing.assert_allclose(g[:, 19], r[:, 19], rtol=RTOL)
+
+
+def build_fleet(n, f_hi=4000, waves=True, seed=11):
+    w = WL.fleet(n, seed=seed, f_lo=200, f_hi=f_hi, waves=waves, wave_n=128, wave_len=128.0)
+    gms = [H.Mesh(b.mesh) for b in w.bodies]
+    oms = [O.Mesh.from_tri(b.mesh) for b in w.bodies]
+    go = H.Ocean()
+    grid = None
+    if waves:
+        grid = W.grid_reference(w.sea.wave_components, w.sea.wave_n).astype(np.float32)
+        go.set_waves(grid, w.sea.wave_n, 1, [w.sea.wave_len])
+        oo = O.Ocean(wave_data=grid, wave_n=w.sea.wave_n, channels=1, lengths=[w.sea.wave_len])
+    else:
+        oo = O.Ocean()
+    descs, obodies = [], []
+    for b, gm, om in zip(w.bodies, gms, oms):
+        descs.append(H.BodyDesc(mesh=gm, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface))
+        obodies.append(O.make_body(om, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface))
+    return w, go, oo, descs, obodies
+
+
+@pytest.mark.parametrize("waves", [False, True])
+def test_batch_fleet(gpu, waves):
+    log(f"case: batch fleet 48 bodies, waves={waves}")
+    w, go, oo, descs, obodies = build_fleet(48, waves=waves)
+    batch = H.BodyBatch(descs)
+    st = WL.random_states(w, 2)
+    for s in range(2):
+        g = batch.compute_host(go, st[s])
+        r = np.stack([O.body_forces(64, obodies[i], oo, st[s, i, :16], st[s, i, 16:19], st[s, i, 19:22])
+                      for i in range(len(obodies))])
+        np.testing.assert_array_equal(g[:, 21], r[:, 21])   # position
+        np.testing.assert_array_equal(g[:, 20], r[:, 20])   # written mask
+        log(f"positions: {np.bincount(r[:, 21].astype(int), minlength=3)}")
+        assert_close(g, r, what=f"fleet waves={waves} step {s}")
+        np.testing.assert_allclose(g[:, 18], r[:, 18], rtol=RTOL)
+        m = r[:, 20].astype(int) & 128 > 0
+        np.testing.assert_allclose(g[m, 19], r[m, 19], rtol=RTOL)
+
+
+def test_reference_wave_layout(gpu):
+    """Two-layer RGBA texel layout of the reference's fftData (channels 0 and 1,
+    sizes 893 m and 101 m, OpenGLOcean.cpp:70)."""
+    log("case: reference RGBA 2-layer wave layout")
+    rng = np.random.default_rng(5)
+    n = 64
+    data = (0.3 * rng.standard_normal((n, n, 4))).astype(np.float32)
+    lengths = [893.0, 101.0]
+    tm = M.ico_ellipsoid(1.0, 0.3, 0.25, 3)
+    gm, om = H.Mesh(tm), O.Mesh.from_tri(tm)
+    go = H.Ocean()
+    go.set_waves(data, n, 4, lengths)
+    oo = O.Ocean(wave_data=data, wave_n=n, channels=4, lengths=lengths)
+    g, r = np.zeros((8, 24)), np.zeros((8, 24))
+    for s in range(8):
+        q = T.random_unit_quaternions(rng, 1)[0]
+        Tcg = T.quat_to_gl(q, [rng.uniform(-900, 900), rng.uniform(-900, 900), rng.uniform(-0.2, 0.2)])
+        v, om_ = rng.uniform(-2, 2, 3), rng.uniform(-1, 1, 3)
+        outs = [np.zeros(3) for _ in range(6)]
+        sv = np.zeros(2)
+        H.ComputeHydrodynamicForcesSurface(H.HydrodynamicsSettings(), gm, go, Tcg, Tcg, v, om_, *outs, sv)
+        g[s, :18] = np.concatenate(outs)
+        g[s, 18:20] = sv
+        r[s] = O.surface(64, 1, 1, om, oo, Tcg, Tcg, v, om_)
+    assert_close(g, r, what="reference wave layout")
+
+
+def test_untouched_outputs(gpu):
+    """Buoyancy outputs stay untouched when the submerged volume is <= 1e-9
+    (SolidEntity.cpp:1690); drag outputs stay untouched without damping."""
+    log("case: untouched outputs")
+    tm = M.icosphere(2)
+    gm = H.Mesh(tm)
+    go = H.Ocean()
+    Tcg = T.translation_gl([0.0, 0.0, -5.0])   # fully dry: 5 m above the surface
+    sentinel = [np.full(3, 7.0) for _ in range(6)]
+    sv = np.array([3.0, 4.0])
+    H.ComputeHydrodynamicForcesSurface(H.HydrodynamicsSettings(False, True), gm, go, Tcg, Tcg, [1, 0, 0],
+                                       [0, 0, 0], *sentinel, sv)
+    for k in range(6):
+        assert np.all(sentinel[k] == 7.0), k
+    assert sv[0] == 0.0 and sv[1] == 4.0
+
+
+def test_batch_determinism_and_graph(gpu):
+    import torch
+    log("case: determinism + graph replay")
+    w, go, oo, descs, obodies = build_fleet(24, waves=True, seed=3)
+    batch = H.BodyBatch(descs)
+    st = WL.random_states(w, 3)
+    dev = torch.device("cuda:0")
+    ds = torch.tensor(st[0], dtype=torch.float64, device=dev).contiguous()
+    out1 = torch.zeros((24, 24), dtype=torch.float64, device=dev)
+    out2 = torch.zeros_like(out1)
+    stream = torch.cuda.current_stream().cuda_stream
+    batch.compute(go, ds.data_ptr(), out1.data_ptr(), stream)
+    batch.compute(go, ds.data_ptr(), out2.data_ptr(), stream)
+    torch.cuda.synchronize()
+    assert torch.equal(out1, out2)
+    s = torch.cuda.Stream()
+    gout = torch.zeros_like(out1)
+    with torch.cuda.stream(s):
+        batch.capture(go, ds.data_ptr(), gout.data_ptr(), s.cuda_stream)
+    for rep in range(3):
+        ds.copy_(torch.tensor(st[rep], dtype=torch.float64))
+        eager = torch.zeros_like(out1)
+        batch.compute(go, ds.data_ptr(), eager.data_ptr(), stream)
+        torch.cuda.synchronize()
+        gout.zero_()
+        torch.cuda.synchronize()
+        batch.replay(s.cuda_stream)
+        s.synchronize()
+        assert torch.equal(gout, eager), rep
+
+
+def test_wave_grid_builder(gpu):
+    import torch
+    comps = W.pm_components(64, 12.0, 1024.0, seed=8)
+    n = 256
+    g = torch.empty((n, n), dtype=torch.float32, device="cuda:0")
+    W.build_grid_device(comps, n, g.data_ptr())
+    ref = W.grid_reference(comps, n)
+    err = np.abs(g.cpu().numpy() - ref).max()
+    log(f"wave grid max abs error {err:.2e}, rms height {ref.std():.3f}")
+    assert err < 2e-5
