@@ -36,7 +36,12 @@ class orc_mesh(C.Structure):
 class orc_result(C.Structure):
     _fields_ = [("Fb", C.c_double * 3), ("Tb", C.c_double * 3), ("Fdq", C.c_double * 3), ("Tdq", C.c_double * 3),
                 ("Fdf", C.c_double * 3), ("Tdf", C.c_double * 3), ("Swet", C.c_double), ("Vsub", C.c_double),
-                ("written", C.c_uint32), ("position", C.c_int32)]
+                ("written", C.c_uint32), ("position", C.c_int32), ("S", C.c_double * 6), ("raw", C.c_double * 12),
+                ("k", C.c_double * 4)]
+
+    def extras(self) -> np.ndarray:
+        """22 doubles: condition scales S[6], raw drag[12], Correct factors k[4]."""
+        return np.concatenate([list(self.S), list(self.raw), list(self.k)])
 
     def vec(self) -> np.ndarray:
         """24 doubles in the sfhydro output layout."""
@@ -171,22 +176,24 @@ def result(init: np.ndarray | None = None) -> orc_result:
     return r
 
 
-def surface(prec: int, damping, buoyancy, mesh: Mesh | None, ocn: Ocean, T_CG, T_C, v, w, init=None) -> np.ndarray:
+def surface(prec: int, damping, buoyancy, mesh: Mesh | None, ocn: Ocean, T_CG, T_C, v, w, init=None,
+            full: bool = False):
+    """Returns the 24-double output (and with full=True also the 22 extras)."""
     lib = load()
     r = result(init)
     fn = lib.orc_surface_64 if prec == 64 else lib.orc_surface_32
     fn(int(damping), int(buoyancy), C.byref(mesh.s) if mesh is not None else None, C.byref(ocn.s), _d(T_CG),
        _d(T_C), _d(v), _d(w), C.byref(r))
-    return r.vec()
+    return (r.vec(), r.extras()) if full else r.vec()
 
 
-def submerged(prec: int, mesh: Mesh | None, ocn: Ocean, T_CG, T_C, v, w, init=None) -> np.ndarray:
+def submerged(prec: int, mesh: Mesh | None, ocn: Ocean, T_CG, T_C, v, w, init=None, full: bool = False):
     lib = load()
     r = result(init)
     fn = lib.orc_submerged_64 if prec == 64 else lib.orc_submerged_32
     fn(C.byref(mesh.s) if mesh is not None else None, C.byref(ocn.s), _d(T_CG), _d(T_C), _d(v), _d(w),
        C.byref(r))
-    return r.vec()
+    return (r.vec(), r.extras()) if full else r.vec()
 
 
 def make_body(mesh: Mesh, T_CG2C=None, T_CG2O=None, aabb_min=None, aabb_max=None, Cd=(1, 1, 1),
@@ -216,12 +223,12 @@ def make_body(mesh: Mesh, T_CG2C=None, T_CG2O=None, aabb_min=None, aabb_max=None
     return b
 
 
-def body_forces(prec: int, body: orc_body, ocn: Ocean, T_CG, v, w, init=None) -> np.ndarray:
+def body_forces(prec: int, body: orc_body, ocn: Ocean, T_CG, v, w, init=None, full: bool = False):
     lib = load()
     r = result(init)
     fn = lib.orc_body_forces_64 if prec == 64 else lib.orc_body_forces_32
     fn(C.byref(body), C.byref(ocn.s), _d(T_CG), _d(v), _d(w), C.byref(r))
-    return r.vec()
+    return (r.vec(), r.extras()) if full else r.vec()
 
 
 def batch_forces(prec: int, bodies: list, ocn: Ocean, states: np.ndarray, steps: int = 1,

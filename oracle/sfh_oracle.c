@@ -173,6 +173,12 @@ void orc_fluid_velocity(const orc_ocean* o, const double p[3], double out[3])
 void orc_correct(const orc_ocean* o, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],
                  const double Cd[3], const double Cf[3], const double T_O[16])
 {
+    orc_correct_k(o, Fdq, Tdq, Fdf, Tdf, Cd, Cf, T_O, NULL);
+}
+
+void orc_correct_k(const orc_ocean* o, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],
+                   const double Cd[3], const double Cf[3], const double T_O[16], double kout[4])
+{
     double M[9];
     orc_basis_inverse(T_O, M);
     double* vecs[4] = {Fdq, Tdq, Fdf, Tdf};
@@ -185,6 +191,7 @@ void orc_correct(const orc_ocean* o, double Fdq[3], double Tdq[3], double Fdf[3]
         double c = fabs(d[0]) * C[0] + fabs(d[1]) * C[1] + fabs(d[2]) * C[2];
         double s = (q < 2) ? 0.5 * o->density * c : o->density * c;
         for (int i = 0; i < 3; ++i) x[i] = s * x[i];
+        if (kout) kout[q] = s;
     }
 }
 

@@ -93,6 +93,16 @@ typedef struct {
     double Swet, Vsub;
     uint32_t written;
     int32_t position;   /* body-level entry points: 0 outside, 1 inside, 2 crossing */
+    /* Error-analysis extras (not outputs of the reference):
+     * S[6]  condition scale of Fb, Tb, Fdq, Tdq, Fdf, Tdf: the sum over faces of
+     *       the norms of the per-face contributions (for the flat-sea volume
+     *       buoyancy: rho |g| sum|6V|/6 and that times the largest lever arm);
+     * raw[12] Fdq, Tdq, Fdf, Tdf before CorrectHydrodynamicForces;
+     * k[4]  the scalars CorrectHydrodynamicForces multiplied them by (1 if not
+     *       applied). */
+    double S[6];
+    double raw[12];
+    double k[4];
 } orc_result;
 
 /* ---- LinearMath restatement (double) -------------------------------- */
@@ -130,6 +140,9 @@ void orc_submerged_32(const orc_mesh* mesh, const orc_ocean* o,
 
 void orc_correct(const orc_ocean* o, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],
                  const double Cd[3], const double Cf[3], const double T_O[16]);
+/* Same, also returning the four scalars applied (may be NULL). */
+void orc_correct_k(const orc_ocean* o, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],
+                   const double Cd[3], const double Cf[3], const double T_O[16], double k[4]);
 
 /* ---- body level (SolidEntity::ComputeHydrodynamicForces) ------------- */
 typedef struct {

@@ -106,113 +106,124 @@ SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
     face_drag(B, c, n1, A, acc);
 }
 
-// Tetra (p0; a, b, c) of the submerged-volume integral.  6V = (a-p0).((b-a)x(c-a)),
-// equal to the reference's p0a.(p0b x p0c) (SolidEntity.cpp:1373-1379) and free
-// of its cancellation; the centroid term is (p0a+p0b+p0c)/4 * 6V.
-template <typename T>
-SFH_HD void face_tetra(const FaceBody& B, f3 a, f3 b, f3 c, Acc<T>& acc)
-{
-    f3 ao = a - B.o;
-    float v6 = dot(ao, cross(b - a, c - a));
-    f3 g = (ao + (b - B.o) + (c - B.o)) * 0.25f;
-    add3(acc.CB, g * v6);
-    acc.V6 += (T)v6;
-}
-
-// Clip point on the edge from wet vertex `w` (depth dw >= 0) to dry vertex `d`
-// (depth dd < 0).
-SFH_HD f3 clip_pt(f3 w, f3 d, float dw, float dd)
-{
-    return w + (d - w) * (dw / (fabsf(dd) + dw));
-}
-
-// Result of clipping one face against the waterline: the wet polygon's
-// centroid, unit normal and area, and whether the face contributes.
+// Result of clipping one face against the waterline: the wet polygon as a
+// base vertex plus an offset to its centroid, its unit normal and area, and
+// whether the face contributes.
 struct Wet {
-    f3 c, n1;
+    f3 base, off;   // centroid = base + off
+    f3 n1;
     float A;
     bool ok;
 };
 
+// Tetra term of the submerged-volume integral for a triangle with normal
+// vector n (|n| = twice its area) through the point base + q:
+// 6V = (base - o + q) . n, and the centroid term (3(base - o) + s)/4 * 6V where
+// s is the sum of the triangle's offsets from base.
+template <typename T>
+SFH_HD void tetra_term(f3 bo, f3 q, f3 n, f3 s, Acc<T>& acc)
+{
+    float v6 = dot(bo + q, n);
+    f3 g = (bo * 3.0f + s) * 0.25f;
+    add3(acc.CB, g * v6);
+    acc.V6 += (T)v6;
+}
+
 // Surface face: waterline clip (7 cases, SolidEntity.cpp:1362-1647), tetra
 // volume, face properties.  d0..d2 are the vertex depths (>= 0 is wet).  The
 // tetra volume is accumulated before the degeneracy test, as in the reference.
+//
+// Every polygon is carried as a wet base vertex plus offsets built from mesh
+// edges: a clip point is base + (edge * t), so edges, normals and areas of thin
+// wet slivers keep FP32 precision relative to the sliver, not to the body.
 template <typename T>
 SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1, float d2, Acc<T>& acc)
 {
     Wet r;
     r.ok = false;
-    f3 p4 = p1, fv1, fv2, fv3 = p1, fv4 = p1;
+    f3 base, fv1, fv2, fv3, fv4, csum;
     bool quad = false;
     if (d0 < 0.f) {
-        if (d1 < 0.f) {                 // A: only p3 wet
-            p1 = clip_pt(p3, p1, d2, d0);
-            p2 = clip_pt(p3, p2, d2, d1);
-            face_tetra(B, p1, p2, p3, acc);
-            fv1 = p2 - p1; fv2 = p3 - p1;
-        } else if (d2 < 0.f) {          // B: only p2 wet
-            p1 = clip_pt(p2, p1, d1, d0);
-            p3 = clip_pt(p2, p3, d1, d2);
-            face_tetra(B, p1, p2, p3, acc);
-            fv1 = p2 - p1; fv2 = p3 - p1;
-        } else {                        // C: only p1 dry -> quad p1 p2 p3 p4
-            p4 = clip_pt(p3, p1, d2, d0);
-            p1 = clip_pt(p2, p1, d1, d0);
-            face_tetra(B, p1, p2, p3, acc);
-            face_tetra(B, p1, p3, p4, acc);
-            fv1 = p2 - p1; fv2 = p4 - p1; fv3 = p2 - p3; fv4 = p4 - p3;
+        if (d1 < 0.f) {                 // A: only p3 wet (1364-1398)
+            base = p3;
+            f3 q1 = (p1 - p3) * (d2 / (fabsf(d0) + d2));
+            f3 q2 = (p2 - p3) * (d2 / (fabsf(d1) + d2));
+            fv1 = q2 - q1; fv2 = q1 * -1.0f;
+            f3 n = cross(fv1, fv2);
+            tetra_term(base - B.o, q1, n, q1 + q2, acc);
+            csum = q1 + q2;
+        } else if (d2 < 0.f) {          // B: only p2 wet (1399-1433)
+            base = p2;
+            f3 q1 = (p1 - p2) * (d1 / (fabsf(d0) + d1));
+            f3 q3 = (p3 - p2) * (d1 / (fabsf(d2) + d1));
+            fv1 = q1 * -1.0f; fv2 = q3 - q1;
+            f3 n = cross(fv1, fv2);
+            tetra_term(base - B.o, q1, n, q1 + q3, acc);
+            csum = q1 + q3;
+        } else {                        // C: only p1 dry, quad p1' p2 p3 p4 (1434-1482)
+            base = p2;
+            f3 e23 = p3 - p2;
+            f3 q1 = (p1 - p2) * (d1 / (fabsf(d0) + d1));
+            f3 q4 = e23 + (p1 - p3) * (d2 / (fabsf(d0) + d2));
+            f3 bo = base - B.o;
+            tetra_term(bo, q1, cross(q1 * -1.0f, e23 - q1), q1 + e23, acc);      // (p1', p2, p3)
+            tetra_term(bo, q1, cross(e23 - q1, q4 - q1), q1 + e23 + q4, acc);   // (p1', p3, p4)
+            fv1 = q1 * -1.0f; fv2 = q4 - q1; fv3 = e23 * -1.0f; fv4 = q4 - e23;
+            csum = q1 + e23 + q4;
             quad = true;
         }
     } else if (d1 < 0.f) {
-        if (d2 < 0.f) {                 // D: only p1 wet
-            p2 = clip_pt(p1, p2, d0, d1);
-            p3 = clip_pt(p1, p3, d0, d2);
-            face_tetra(B, p1, p2, p3, acc);
-            fv1 = p2 - p1; fv2 = p3 - p1;
-        } else {                        // E: only p2 dry
-            p4 = clip_pt(p3, p2, d2, d1);
-            p2 = clip_pt(p1, p2, d0, d1);
-            face_tetra(B, p1, p2, p3, acc);
-            face_tetra(B, p2, p4, p3, acc);
-            fv1 = p2 - p1; fv2 = p3 - p1; fv3 = p2 - p3; fv4 = p4 - p3;
+        if (d2 < 0.f) {                 // D: only p1 wet (1486-1520)
+            base = p1;
+            f3 q2 = (p2 - p1) * (d0 / (fabsf(d1) + d0));
+            f3 q3 = (p3 - p1) * (d0 / (fabsf(d2) + d0));
+            fv1 = q2; fv2 = q3;
+            f3 n = cross(fv1, fv2);
+            tetra_term(base - B.o, f3{0.f, 0.f, 0.f}, n, q2 + q3, acc);
+            csum = q2 + q3;
+        } else {                        // E: only p2 dry, quad p1 p2' p4 p3 (1521-1568)
+            base = p1;
+            f3 q3 = p3 - p1;
+            f3 q2 = (p2 - p1) * (d0 / (fabsf(d1) + d0));
+            f3 q4 = q3 + (p2 - p3) * (d2 / (fabsf(d1) + d2));
+            f3 bo = base - B.o;
+            tetra_term(bo, f3{0.f, 0.f, 0.f}, cross(q2, q3), q2 + q3, acc);     // (p1, p2', p3)
+            tetra_term(bo, q2, cross(q4 - q2, q3 - q2), q2 + q4 + q3, acc);     // (p2', p4, p3)
+            fv1 = q2; fv2 = q3; fv3 = q2 - q3; fv4 = q4 - q3;
+            csum = q2 + q3 + q4;
             quad = true;
         }
-    } else if (d2 < 0.f) {              // F: only p3 dry
-        p4 = clip_pt(p1, p3, d0, d2);
-        p3 = clip_pt(p2, p3, d1, d2);
-        face_tetra(B, p1, p2, p3, acc);
-        face_tetra(B, p1, p3, p4, acc);
-        fv1 = p2 - p1; fv2 = p4 - p1; fv3 = p2 - p3; fv4 = p4 - p3;
+    } else if (d2 < 0.f) {              // F: only p3 dry, quad p1 p2 p3' p4 (1570-1617)
+        base = p1;
+        f3 q2 = p2 - p1;
+        f3 q3 = q2 + (p3 - p2) * (d1 / (fabsf(d2) + d1));
+        f3 q4 = (p3 - p1) * (d0 / (fabsf(d2) + d0));
+        f3 bo = base - B.o;
+        tetra_term(bo, f3{0.f, 0.f, 0.f}, cross(q2, q3), q2 + q3, acc);         // (p1, p2, p3')
+        tetra_term(bo, f3{0.f, 0.f, 0.f}, cross(q3, q4), q3 + q4, acc);         // (p1, p3', p4)
+        fv1 = q2; fv2 = q4; fv3 = q2 - q3; fv4 = q4 - q3;
+        csum = q2 + q3 + q4;
         quad = true;
-    } else {                            // G: all wet
+    } else {                            // G: all wet (1618-1647)
+        base = p1;
         fv1 = p2 - p1; fv2 = p3 - p1;
         f3 n = cross(fv1, fv2);
-        f3 ao = p1 - B.o;
-        float v6 = dot(ao, n);
-        f3 g = (ao + (p2 - B.o) + (p3 - B.o)) * 0.25f;
-        add3(acc.CB, g * v6);
-        acc.V6 += (T)v6;
-        float n2 = dot(n, n);
-        if (n2 < SFH_DEGENERATE_N2) return r;
-        float len = sqrtf(n2);
-        r.n1 = n * (1.0f / len);
-        r.A = 0.5f * len;
-        r.c = (p1 + p2 + p3) * (1.0f / 3.0f);
-        r.ok = true;
-        return r;
+        tetra_term(base - B.o, f3{0.f, 0.f, 0.f}, n, fv1 + fv2, acc);
+        csum = fv1 + fv2;
     }
     f3 n = cross(fv1, fv2);
     float n2 = dot(n, n);
     if (n2 < SFH_DEGENERATE_N2) return r;
     float len = sqrtf(n2);
     r.n1 = n * (1.0f / len);
+    r.base = base;
     if (quad) {
         f3 n34 = cross(fv3, fv4);
         r.A = 0.5f * (len + sqrtf(dot(n34, n34)));
-        r.c = (p1 + p2 + p3 + p4) * 0.25f;
+        r.off = csum * 0.25f;
     } else {
         r.A = 0.5f * len;
-        r.c = (p1 + p2 + p3) * (1.0f / 3.0f);
+        r.off = csum * (1.0f / 3.0f);
     }
     r.ok = true;
     return r;

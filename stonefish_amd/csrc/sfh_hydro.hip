@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -83,18 +84,21 @@ struct BodyStatic {
     double P_CB[3];
 };
 
+// Depth queries run in FP64: near the waterline the depth is a small
+// difference of body-scale quantities (z of the vertex, height of the wave),
+// and the clip points of thin wet slivers depend on it.
 struct LayerK {
-    float fu, fv;       // fractional texel coordinate of the mesh origin
+    double fu, fv;      // fractional texel coordinate of the mesh origin
+    double ru[3], rv[3];// rows 0/1 of R scaled by n / length: texels per mesh unit
     int ku, kv;         // integer texel coordinate of the mesh origin, mod n
-    f3 ru, rv;          // rows 0/1 of R scaled by n / length: texels per mesh unit
 };
 
 struct BodyDyn {
     int32_t pos;        // 0 outside, 1 inside (submerged loop), 2 crossing (surface loop)
     uint32_t flags;     // effective settings of the face loop
     FaceBody fb;
-    f3 rz;              // row 2 of R: world z of a mesh-frame vector
-    float tz;           // world z of the mesh origin
+    double rz[3];       // row 2 of R: world z of a mesh-frame vector
+    double tz;          // world z of the mesh origin
     LayerK layer[kMaxLayers];
     // FP64 epilogue constants
     double R[9];        // basis of T_C, row-major
@@ -127,33 +131,31 @@ __device__ __forceinline__ int wrap_n(int i, int n)
     return min(max(i, 0), n - 1);
 }
 
-// Sum over layers of the bilinear height at mesh-frame point p (FP32).  The
-// texel coordinate of p is u = ku + fu + ru.p (the integer part of the body
-// position is carried exactly in ku).  ComputeInterpolatedWaveData in exact
-// arithmetic: i0 = floor(u) mod n, i1 = i0 + 1 mod n, alpha = frac(u).
-__device__ __forceinline__ float wave_sum(const BodyDyn& D, const OceanK& oc, f3 p)
+// Depth of the mesh-frame point (x, y, z), FP64: z - H(x, y) with
+// H = -sum of layer heights (Ocean::GetDepth, OpenGLRealOcean.cpp:280-331).
+// The texel coordinate is u = ku + fu + ru.p, the integer part of the body
+// position carried exactly in ku; in exact arithmetic this is the reference's
+// i0 = floor(u) mod n, i1 = i0 + 1 mod n, alpha = frac(u).
+__device__ __forceinline__ double depth64(const BodyDyn& D, const OceanK& oc, double x, double y, double z)
 {
-    float h = 0.f;
+    double d = D.tz + D.rz[0] * x + D.rz[1] * y + D.rz[2] * z;
     for (int l = 0; l < oc.nlayers; ++l) {
         const LayerK& L = D.layer[l];
-        float u = L.fu + dot(L.ru, p);
-        float v = L.fv + dot(L.rv, p);
-        float fi = floorf(u), fj = floorf(v);
-        float a = u - fi, b = v - fj;
+        double u = L.fu + L.ru[0] * x + L.ru[1] * y + L.ru[2] * z;
+        double v = L.fv + L.rv[0] * x + L.rv[1] * y + L.rv[2] * z;
+        double fi = floor(u), fj = floor(v);
+        double a = u - fi, b = v - fj;
         int i = wrap_n(L.ku + (int)fi, oc.n);
         int j = wrap_n(L.kv + (int)fj, oc.n);
         float4 t = oc.quad[l][(size_t)j * oc.n + i];
-        h += (1.f - a) * (1.f - b) * t.x + a * (1.f - b) * t.y + (1.f - a) * b * t.z + a * b * t.w;
+        d += (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
     }
-    return h;
+    return d;
 }
 
-// depth = z - H(x, y) with H = -sum of layer heights (Ocean::GetDepth).
-__device__ __forceinline__ float depth_local(const BodyDyn& D, const OceanK& oc, f3 p)
+__device__ __forceinline__ float depth_vertex(const BodyDyn& D, const OceanK& oc, f3 p)
 {
-    float z = D.tz + dot(D.rz, p);
-    if (oc.nlayers) z += wave_sum(D, oc, p);
-    return z;
+    return (float)depth64(D, oc, p.x, p.y, p.z);
 }
 
 // FP64 wave height sum at world (x, y), for the classification corners.
@@ -283,24 +285,23 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     x[0] = tcg[0] - tc[0]; x[1] = tcg[1] - tc[1]; x[2] = -tc[2];
     mat_t_vec(R, x, y);
     D.fb.o = to_f3(y);
-    D.rz = mk3((float)R[6], (float)R[7], (float)R[8]);
-    D.tz = (float)tc[2];
+    D.rz[0] = R[6]; D.rz[1] = R[7]; D.rz[2] = R[8];
+    D.tz = tc[2];
     for (int l = 0; l < kMaxLayers; ++l) {
         LayerK L;
-        L.fu = L.fv = 0.f; L.ku = L.kv = 0;
-        L.ru = L.rv = mk3(0.f, 0.f, 0.f);
+        L.fu = L.fv = 0.0; L.ku = L.kv = 0;
+        for (int k = 0; k < 3; ++k) L.ru[k] = L.rv[k] = 0.0;
         if (l < oc.nlayers) {
             double s = (double)oc.n / oc.len[l];
             double bu = tc[0] * s - 0.5, bv = tc[1] * s - 0.5;
             double fu = floor(bu), fv = floor(bv);
-            L.fu = (float)(bu - fu);
-            L.fv = (float)(bv - fv);
+            L.fu = bu - fu;
+            L.fv = bv - fv;
             long long ku = (long long)fu % oc.n, kv = (long long)fv % oc.n;
             if (ku < 0) ku += oc.n;
             if (kv < 0) kv += oc.n;
             L.ku = (int)ku; L.kv = (int)kv;
-            L.ru = mk3((float)(R[0] * s), (float)(R[1] * s), (float)(R[2] * s));
-            L.rv = mk3((float)(R[3] * s), (float)(R[4] * s), (float)(R[5] * s));
+            for (int k = 0; k < 3; ++k) { L.ru[k] = R[k] * s; L.rv[k] = R[3 + k] * s; }
         }
         D.layer[l] = L;
     }
@@ -370,19 +371,22 @@ __global__ __launch_bounds__(256) void k_faces(const uint2* __restrict__ chunks,
                 if (f < fend) {
                     f3 p1, p2, p3;
                     load_face(S, f, p1, p2, p3);
-                    float d0 = depth_local(D, oc, p1);
-                    float d1 = depth_local(D, oc, p2);
-                    float d2 = depth_local(D, oc, p3);
+                    float d0 = depth_vertex(D, oc, p1);
+                    float d1 = depth_vertex(D, oc, p2);
+                    float d2 = depth_vertex(D, oc, p3);
                     if (!(d0 < 0.f && d1 < 0.f && d2 < 0.f)) {
                         Wet wt = face_clip(fb, p1, p2, p3, d0, d1, d2, acc);
                         if (wt.ok) {
-                            if (wave_buoy) {
-                                float dc = depth_local(D, oc, wt.c);
+                            f3 c = wt.base + wt.off;
+                            if (wave_buoy) {   // pressure at the wet centroid (1649-1658)
+                                float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x,
+                                                          (double)wt.base.y + (double)wt.off.y,
+                                                          (double)wt.base.z + (double)wt.off.z);
                                 f3 fbi = wt.n1 * (-wt.A * dc);
                                 add3(acc.Fb, fbi);
-                                add3(acc.Mb, cross(wt.c, fbi));
+                                add3(acc.Mb, cross(c, fbi));
                             }
-                            if (damping) face_drag(fb, wt.c, wt.n1, wt.A, acc);
+                            if (damping) face_drag(fb, c, wt.n1, wt.A, acc);
                             acc.Swet += (double)wt.A;
                         }
                     }
@@ -611,12 +615,28 @@ __global__ void k_wave_quad(const float* __restrict__ data, uint32_t n, uint32_t
 // host objects
 // ---------------------------------------------------------------------------
 
+// Meshes are reference counted: the creator holds one reference and every
+// batch (and the per-body workspace) that keeps the mesh's device pointers holds
+// another, so sfh_mesh_destroy never frees memory a batch still reads.
 struct sfh_mesh_t {
     float* d_verts = nullptr;
     uint32_t* d_faces = nullptr;
     uint32_t nv = 0, nf = 0;
     float aabb_min[3], aabb_max[3];
+    uint64_t id = 0;
+    std::atomic<int> refs{1};
 };
+
+static std::atomic<uint64_t> g_mesh_ids{1};
+
+static void mesh_release(sfh_mesh_t* m)
+{
+    if (m && m->refs.fetch_sub(1) == 1) {
+        hipFree(m->d_verts);
+        hipFree(m->d_faces);
+        delete m;
+    }
+}
 
 struct Current {
     int type, enabled;
@@ -635,6 +655,7 @@ struct sfh_ocean_t {
 };
 
 struct sfh_batch_t {
+    std::vector<sfh_mesh_t*> meshes;   // one reference per body
     uint32_t n = 0;
     uint32_t nchunks = 0;
     uint64_t nfaces = 0;
@@ -707,6 +728,7 @@ extern "C" int sfh_mesh_create(const void* vertex_pos, size_t vertex_stride, uin
     for (size_t i = 0; i < 3 * (size_t)nf; ++i)
         if (face_ids[i] >= nv) return fail(SFH_ERR_INVALID, "face %zu references vertex %u >= %u", i / 3, face_ids[i], nv);
     sfh_mesh_t* m = new sfh_mesh_t();
+    m->id = g_mesh_ids.fetch_add(1);
     m->nv = nv;
     m->nf = nf;
     for (int k = 0; k < 3; ++k) { m->aabb_min[k] = mn[k]; m->aabb_max[k] = mx[k]; }
@@ -727,9 +749,7 @@ extern "C" int sfh_mesh_create(const void* vertex_pos, size_t vertex_stride, uin
 extern "C" int sfh_mesh_destroy(sfh_mesh m)
 {
     if (!m) return fail(SFH_ERR_INVALID, "null mesh");
-    hipFree(m->d_verts);
-    hipFree(m->d_faces);
-    delete m;
+    mesh_release(m);
     return SFH_OK;
 }
 
@@ -847,6 +867,8 @@ static void batch_free(sfh_batch_t* b)
     hipFree(b->d_chunks);
     hipFree(b->d_dyn);
     hipFree(b->d_partials);
+    for (sfh_mesh_t* m : b->meshes) mesh_release(m);
+    b->meshes.clear();
 }
 
 static uint32_t default_waves()
@@ -925,6 +947,10 @@ extern "C" int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const s
         if (d.hydro) flags |= BF_HYDRO;
         if (d.buoyant) flags |= BF_BUOYANT;
         BodyStatic s = make_static(d.mesh, flags);
+        if (d.mesh) {
+            d.mesh->refs.fetch_add(1);
+            b->meshes.push_back(d.mesh);
+        }
         gl_to_rows(d.T_CG2C, s.cg2c, s.cg2c + 9);
         gl_to_rows(d.T_CG2O, s.cg2o, nullptr);
         for (int k = 0; k < 3; ++k) {
@@ -1121,7 +1147,7 @@ extern "C" int sfh_batch_kernel_ms(sfh_batch b, double* face_ms, double* total_m
 static std::mutex g_single_mu;
 struct Single {
     sfh_batch_t* b = nullptr;
-    const sfh_mesh_t* mesh = nullptr;
+    uint64_t mesh_id = 0;
     uint32_t flags = 0;
     double* d_state = nullptr;
     double* d_out = nullptr;
@@ -1148,17 +1174,19 @@ static int single_alloc()
     return SFH_OK;
 }
 
-static int single_prepare(const sfh_mesh_t* m, uint32_t flags)
+static int single_prepare(sfh_mesh_t* m, uint32_t flags)
 {
     Single& S = g_single;
     int rc = single_alloc();
     if (rc != SFH_OK) return rc;
     int dev = S.device;
-    if (!S.b || S.mesh != m || S.flags != flags) {
+    if (!S.b || S.mesh_id != m->id || S.flags != flags) {
         if (S.b) { batch_free(S.b); delete S.b; S.b = nullptr; }
         sfh_batch_t* b = new sfh_batch_t();
         b->n = 1;
         b->device = dev;
+        m->refs.fetch_add(1);
+        b->meshes.push_back(m);
         b->h_static.push_back(make_static(m, flags));
         b->nfaces = b->h_static[0].nf;
         b->K = 0;
@@ -1167,7 +1195,7 @@ static int single_prepare(const sfh_mesh_t* m, uint32_t flags)
         if (rc == SFH_OK) rc = apply_opts(b, &opts);
         if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
         S.b = b;
-        S.mesh = m;
+        S.mesh_id = m->id;
         S.flags = flags;
     }
     return SFH_OK;

@@ -40,8 +40,8 @@ extern "C" void harness_body(int mode, const float* verts, const uint32_t* faces
     mtv(R, x, dt); B.dt = mk3((float)dt[0], (float)dt[1], (float)dt[2]);
     x[0] = tcg[0] - tc[0]; x[1] = tcg[1] - tc[1]; x[2] = -tc[2];
     mtv(R, x, y); B.o = mk3((float)y[0], (float)y[1], (float)y[2]);
-    f3 rz = mk3((float)R[6], (float)R[7], (float)R[8]);
-    float tz = (float)tc[2];
+    double rz[3] = {R[6], R[7], R[8]}, tz = tc[2];
+    auto depth = [&](f3 p) { return (float)(tz + rz[0] * p.x + rz[1] * p.y + rz[2] * p.z); };
     Acc<double> A;
     acc_zero(A);
     for (uint32_t f = 0; f < nf; ++f) {
@@ -50,11 +50,11 @@ extern "C" void harness_body(int mode, const float* verts, const uint32_t* faces
         const float* c = verts + 3 * faces[3 * f + 2];
         f3 p1 = mk3(a[0], a[1], a[2]), p2 = mk3(b[0], b[1], b[2]), p3 = mk3(c[0], c[1], c[2]);
         if (mode == 1) { face_submerged(B, p1, p2, p3, A); continue; }
-        float d0 = tz + dot(rz, p1), d1 = tz + dot(rz, p2), d2 = tz + dot(rz, p3);
+        float d0 = depth(p1), d1 = depth(p2), d2 = depth(p3);
         if (d0 < 0.f && d1 < 0.f && d2 < 0.f) continue;
         Wet wt = face_clip(B, p1, p2, p3, d0, d1, d2, A);
         if (!wt.ok) continue;
-        face_drag(B, wt.c, wt.n1, wt.A, A);
+        face_drag(B, wt.base + wt.off, wt.n1, wt.A, A);
         A.Swet += wt.A;
     }
     memset(out, 0, 24 * sizeof(double));

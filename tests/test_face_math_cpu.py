@@ -13,7 +13,7 @@ import oracle as O
 from stonefish_amd import meshes as M
 from stonefish_amd import transforms as T
 from stonefish_amd import workloads as WL
-from util import compare_out
+from util import bound_ratio
 
 HERE = Path(__file__).resolve().parent
 
@@ -43,17 +43,23 @@ def run(lib, mode, tm, Tcg, Tc, v, w):
     return out
 
 
+def check(g, r, x, names):
+    worst, plain = bound_ratio(g, r, x, names=names)
+    assert max(worst.values()) <= 1.0, (worst, plain)
+    return worst, plain
+
+
 def test_submerged_capsule_host(harness):
     w = WL.capsule_submerged()
     tm = w.bodies[0].mesh
     om, oo = O.Mesh.from_tri(tm), O.Ocean()
     st = WL.random_states(w, 12)[:, 0, :]
     g = np.stack([run(harness, 1, tm, s[:16], s[:16], s[16:19], s[19:22]) for s in st])
-    r = np.stack([O.submerged(64, om, oo, s[:16], s[:16], s[16:19], s[19:22]) for s in st])
-    F = np.abs(r[:, 6:18]).max()
-    worst = compare_out(g, r, F, F, fields={"Fdq": slice(6, 9), "Tdq": slice(9, 12), "Fdf": slice(12, 15),
-                                            "Tdf": slice(15, 18)})
-    assert max(worst.values()) < 1e-6, worst
+    rr = [O.submerged(64, om, oo, s[:16], s[:16], s[16:19], s[19:22], full=True) for s in st]
+    r = np.stack([a for a, _ in rr])
+    x = np.stack([b for _, b in rr])
+    r[:, 20] = 60
+    check(g, r, x, ("Fdq", "Tdq", "Fdf", "Tdf"))
 
 
 @pytest.mark.parametrize("offset", [(0.0, 0.0), (37.0, -512.0)])
@@ -61,18 +67,38 @@ def test_surface_ellipsoid_host(harness, offset):
     tm = M.ico_ellipsoid(1.0, 0.3, 0.25, 4)
     om, oo = O.Mesh.from_tri(tm), O.Ocean()
     rng = np.random.default_rng(4)
-    g, r = [], []
+    g, r, x = [], [], []
     for _ in range(12):
         Tcg = T.quat_to_gl(T.random_unit_quaternions(rng, 1)[0], [offset[0], offset[1], rng.uniform(-0.2, 0.2)])
         # a C frame offset from the CG exercises the lever-arm transport
         Tc = T.gl_mul(Tcg, T.translation_gl([0.05, -0.02, 0.01]))
         v, om_ = rng.uniform(-2, 2, 3), rng.uniform(-1, 1, 3)
         g.append(run(harness, 2, tm, Tcg, Tc, v, om_))
-        r.append(O.surface(64, 1, 1, om, oo, Tcg, Tc, v, om_))
-    g, r = np.stack(g), np.stack(r)
-    Fs = np.abs(r[:, :18:6]).max()
-    worst = compare_out(g, r, np.abs(r[:, 0:3]).max(), np.abs(r[:, 3:6]).max())
-    assert max(worst.values()) < 1e-6, worst
+        a, b = O.surface(64, 1, 1, om, oo, Tcg, Tc, v, om_, full=True)
+        r.append(a)
+        x.append(b)
+    g, r, x = np.stack(g), np.stack(r), np.stack(x)
+    check(g, r, x, ("Fb", "Tb", "Fdq", "Tdq", "Fdf", "Tdf"))
     np.testing.assert_allclose(g[:, 18], r[:, 18], rtol=1e-6)
     np.testing.assert_allclose(g[:, 19], r[:, 19], rtol=1e-6)
-    del Fs
+
+
+def test_fleet_crossing_host(harness):
+    """Crossing bodies of a fleet (flat sea), including bodies that are fully
+    under water but classified crossing by their loose AABB (zero buoyancy
+    torque) and bodies with thin wet slivers."""
+    w = WL.fleet(48, seed=11, f_lo=200, f_hi=4000, waves=False)
+    st = WL.random_states(w, 2)[0]
+    oo = O.Ocean()
+    g, r, x = [], [], []
+    for i, b in enumerate(w.bodies):
+        om = O.Mesh.from_tri(b.mesh)
+        ob = O.make_body(om, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface)
+        if O.load().orc_classify_64(C.byref(ob), C.byref(oo.s), O._d(st[i, :16])) != 2:
+            continue
+        g.append(run(harness, 2, b.mesh, st[i, :16], st[i, :16], st[i, 16:19], st[i, 19:22]))
+        a, e = O.surface(64, 1, 1, om, oo, st[i, :16], st[i, :16], st[i, 16:19], st[i, 19:22], full=True)
+        r.append(a)
+        x.append(e)
+    assert len(g) > 10
+    check(np.stack(g), np.stack(r), np.stack(x), ("Fb", "Tb", "Fdq", "Tdq", "Fdf", "Tdf"))
