@@ -90,8 +90,9 @@ void orc_safe_normalize(double v[3])
 {
     double l2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     if (l2 >= DBL_EPSILON * DBL_EPSILON) {
-        double l = sqrt(l2);
-        v[0] /= l; v[1] /= l; v[2] /= l;
+        /* btVector3::operator/= multiplies by the reciprocal (btVector3.h) */
+        double inv = 1.0 / sqrt(l2);
+        v[0] *= inv; v[1] *= inv; v[2] *= inv;
     } else {
         v[0] = 1.0; v[1] = 0.0; v[2] = 0.0;
     }

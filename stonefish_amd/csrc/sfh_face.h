@@ -21,6 +21,18 @@
 
 namespace sfh {
 
+// Square root and reciprocal: the hardware's single instructions on the
+// device (about 1 ulp), the libm functions on the host.
+#ifdef __HIP_DEVICE_COMPILE__
+SFH_HD float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+SFH_HD float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+SFH_HD float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+#else
+SFH_HD float fsqrt(float x) { return sqrtf(x); }
+SFH_HD float frcp(float x) { return 1.0f / x; }
+SFH_HD float frsq(float x) { return 1.0f / sqrtf(x); }
+#endif
+
 struct f3 { float x, y, z; };
 
 SFH_HD f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
@@ -36,11 +48,11 @@ SFH_HD f3 cross(f3 a, f3 b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x 
 #define SFH_SKIN_VT2      1e-9f    // skin friction when |vt|^2 above this (1784)
 
 // Per-body, per-step constants of the face loop, all in the mesh frame.
+// The velocity of the fluid relative to a face centroid c (mesh frame) is
+// vc = u - (v + w x (c + dt)) = -(vb + w x c) with vb = v + w x dt - u.
 struct FaceBody {
-    f3 v;     // R^T * linear velocity of the body
+    f3 vb;    // R^T * (v + w x (t_C - t_CG) - u)
     f3 w;     // R^T * angular velocity
-    f3 u;     // R^T * fluid velocity (uniform part)
-    f3 dt;    // R^T * (t_C - t_CG): mesh origin relative to the CoG
     f3 o;     // R^T * (p0 - t_C), p0 = (t_CG.x, t_CG.y, 0): volume apex
 };
 
@@ -71,24 +83,26 @@ SFH_HD void add3(T* acc, f3 x)
 }
 
 // Drag integrand of one (possibly clipped) face: centroid c (mesh frame), unit
-// normal n1, area A (SolidEntity.cpp:1660-1683, 1767-1790).
+// normal n1, area A (SolidEntity.cpp:1660-1683, 1767-1790).  With vm = -vc:
+// quadratic q = vc |vc| (-vc.n1) A = vm (-|vm| (vm.n1) A), applied when
+// vc.n1 < -1e-12; tangential vt = vc - n1 (vc.n1) = n1 (vm.n1) - vm, skin
+// s = vt A when |vt|^2 > 1e-9.
 template <typename T>
 SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, Acc<T>& acc)
 {
-    f3 r = c + B.dt;
-    f3 vc = B.u - (B.v + cross(B.w, r));
-    float vcn = dot(vc, n1);
-    if (vcn < SFH_APPROACH_VN) {
-        float m = sqrtf(dot(vc, vc));
-        f3 q = vc * (m * (-vcn) * A);
+    f3 vm = B.vb + cross(B.w, c);
+    float vn = dot(vm, n1);
+    if (vn > -SFH_APPROACH_VN) {
+        float m = fsqrt(dot(vm, vm));
+        f3 q = vm * (-(m * vn * A));
         add3(acc.Fdq, q);
         add3(acc.Mdq, cross(c, q));
     }
-    f3 vt = vc - n1 * vcn;
+    f3 vt = n1 * vn - vm;
     if (dot(vt, vt) > SFH_SKIN_VT2) {
-        f3 s = vt * A;
-        add3(acc.Fdf, s);
-        add3(acc.Mdf, cross(c, s));
+        f3 sk = vt * A;
+        add3(acc.Fdf, sk);
+        add3(acc.Mdf, cross(c, sk));
     }
 }
 
@@ -96,14 +110,13 @@ SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, Acc<T>& acc)
 template <typename T>
 SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
 {
-    f3 n = cross(p2 - p1, p3 - p1);
+    f3 e1 = p2 - p1, e2 = p3 - p1;
+    f3 n = cross(e1, e2);
     float n2 = dot(n, n);
     if (n2 < SFH_DEGENERATE_N2) return;
-    float len = sqrtf(n2);
-    f3 n1 = n * (1.0f / len);
-    float A = 0.5f * len;
-    f3 c = (p1 + p2 + p3) * (1.0f / 3.0f);
-    face_drag(B, c, n1, A, acc);
+    float inv = frsq(n2);
+    f3 c = p1 + (e1 + e2) * (1.0f / 3.0f);
+    face_drag(B, c, n * inv, 0.5f * (n2 * inv), acc);
 }
 
 // Result of clipping one face against the waterline: the wet polygon as a
@@ -214,12 +227,13 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
     f3 n = cross(fv1, fv2);
     float n2 = dot(n, n);
     if (n2 < SFH_DEGENERATE_N2) return r;
-    float len = sqrtf(n2);
-    r.n1 = n * (1.0f / len);
+    float inv = frsq(n2);
+    float len = n2 * inv;
+    r.n1 = n * inv;
     r.base = base;
     if (quad) {
         f3 n34 = cross(fv3, fv4);
-        r.A = 0.5f * (len + sqrtf(dot(n34, n34)));
+        r.A = 0.5f * (len + fsqrt(dot(n34, n34)));
         r.off = csum * 0.25f;
     } else {
         r.A = 0.5f * len;

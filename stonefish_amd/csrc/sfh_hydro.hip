@@ -2,15 +2,16 @@
 // hydrodynamics (SolidEntity.cpp:1239-1850, Ocean.cpp:149-201,
 // OpenGLRealOcean.cpp:280-331, Compound-free body path).
 //
-// One evaluation pass over a batch of bodies is three launches on one stream:
+// One evaluation pass over a batch of bodies is four launches on one stream:
 //   k_prologue  one lane per body, FP64: C transform, body/fluid classification
 //               (8 AABB corners against the surface), per-body FP32 constants of
 //               the face loop in the mesh frame, FP64 constants of the epilogue;
-//   k_faces     one wavefront per chunk of 64*K faces of one body, FP32 face
-//               arithmetic, FP64 per-lane accumulation and FP64 wavefront
+//   k_depths    FP64 depth of every vertex of the bodies crossing the surface;
+//   k_faces     one wavefront per chunk of up to 64*K faces of one body: FP32
+//               face arithmetic with FP32 per-lane partials, FP64 wavefront
 //               reduction; one 24-double partial per chunk;
-//   k_epilogue  one lane per body, FP64: sums the body's chunk partials in chunk
-//               order (deterministic), rotates to world, moment about the CoG,
+//   k_epilogue  one wavefront per body, FP64: sums the body's chunk partials
+//               (deterministic order), rotates to world, moment about the CoG,
 //               buoyancy, CorrectHydrodynamicForces, output write mask.
 #include <hip/hip_runtime.h>
 
@@ -75,6 +76,9 @@ struct BodyStatic {
     uint32_t flags;
     uint32_t chunk0;
     uint32_t nchunks;
+    uint32_t nv;
+    uint32_t pad_;
+    uint64_t dep_off;   // offset of this body's vertex depths in the depth buffer
     double cg2c[12];   // basis row-major, origin
     double cg2o[9];    // basis row-major
     double aabb_c[3];  // CG-frame AABB centre and half extents
@@ -274,14 +278,17 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     D.flags = flags;
 
     // mesh-frame FP32 constants
-    double x[3], y[3];
-    mat_t_vec(R, v, y);   D.fb.v = to_f3(y);
-    mat_t_vec(R, w, y);   D.fb.w = to_f3(y);
-    mat_t_vec(R, oc.u, y); D.fb.u = to_f3(y);
+    double x[3], y[3], vl[3], wl[3], ul[3], wd[3];
+    mat_t_vec(R, v, vl);
+    mat_t_vec(R, w, wl);
+    mat_t_vec(R, oc.u, ul);
+    D.fb.w = to_f3(wl);
     for (int i = 0; i < 3; ++i) x[i] = tc[i] - tcg[i];
     mat_t_vec(R, x, y);
-    D.fb.dt = to_f3(y);
     D.dt[0] = y[0]; D.dt[1] = y[1]; D.dt[2] = y[2];
+    cross64(wl, D.dt, wd);
+    for (int i = 0; i < 3; ++i) y[i] = vl[i] + wd[i] - ul[i];
+    D.fb.vb = to_f3(y);
     x[0] = tcg[0] - tc[0]; x[1] = tcg[1] - tc[1]; x[2] = -tc[2];
     mat_t_vec(R, x, y);
     D.fb.o = to_f3(y);
@@ -311,7 +318,44 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
 }
 
 // ---------------------------------------------------------------------------
-// k_faces: one wavefront per chunk of 64*K faces
+// k_depths: FP64 depth of every vertex of the bodies crossing the surface
+// ---------------------------------------------------------------------------
+
+// A work item of a pass: a run of consecutive faces (k_faces) or vertices
+// (k_depths) of one body.
+struct Chunk {
+    uint32_t body, first, count, pad;
+};
+
+// Near the waterline a depth is a small difference of body-scale values, and
+// the clip points of thin wet slivers are built from it, so it is evaluated in
+// FP64 once per vertex (a mesh vertex is shared by about six faces) and stored
+// as float (relative precision is kept for small depths).
+__global__ __launch_bounds__(256) void k_depths(const Chunk* __restrict__ vchunks, uint32_t nvchunks,
+                                                const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
+                                                OceanK oc, float* __restrict__ depth)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t c = wave; c < nvchunks; c += nwaves) {
+        const Chunk ch = vchunks[c];
+        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
+        const int pos = __builtin_amdgcn_readfirstlane(dyn[b].pos);
+        if (pos != 2) continue;
+        const BodyDyn& D = dyn[b];
+        const BodyStatic& S = st[b];
+        const uint32_t v0 = __builtin_amdgcn_readfirstlane(ch.first);
+        const uint32_t n = __builtin_amdgcn_readfirstlane(ch.count);
+        for (uint32_t v = lane; v < n; v += 64) {
+            const float* p = S.verts + 3 * (size_t)(v0 + v);
+            depth[S.dep_off + v0 + v] = (float)depth64(D, oc, p[0], p[1], p[2]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_faces: one wavefront per chunk of up to 64*K faces of one body
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum_f64(double x)
@@ -321,82 +365,140 @@ __device__ __forceinline__ double wave_sum_f64(double x)
     return x;
 }
 
-__device__ __forceinline__ void load_face(const BodyStatic& S, uint32_t f, f3& p1, f3& p2, f3& p3)
+// Transposed FP64 reduction of 24 per-lane values across a wavefront.  Each
+// step exchanges half of the remaining values with the partner lane, so the
+// wave moves 12+6+3+2+1+1 doubles instead of 6 per value.  On return v[0] of
+// lane l holds the wave total of quantity q = 12 b5 + 6 b4 + 3 b3 + 2 b2 + b1
+// (b = bits of l); q == 23 and (2 b2 + b1) == 3 are padding.
+__device__ __forceinline__ void xreduce24(double (&v)[24], int lane)
 {
-    const uint32_t* fi = S.faces + 3 * (size_t)f;
-    uint32_t i0 = fi[0], i1 = fi[1], i2 = fi[2];
-    const float* v0 = S.verts + 3 * (size_t)i0;
-    const float* v1 = S.verts + 3 * (size_t)i1;
-    const float* v2 = S.verts + 3 * (size_t)i2;
-    p1 = mk3(v0[0], v0[1], v0[2]);
-    p2 = mk3(v1[0], v1[1], v1[2]);
-    p3 = mk3(v2[0], v2[1], v2[2]);
+    bool hi = lane & 32;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        double send = hi ? v[i] : v[i + 12], keep = hi ? v[i + 12] : v[i];
+        v[i] = keep + __shfl_xor(send, 32, 64);
+    }
+    hi = lane & 16;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double send = hi ? v[i] : v[i + 6], keep = hi ? v[i + 6] : v[i];
+        v[i] = keep + __shfl_xor(send, 16, 64);
+    }
+    hi = lane & 8;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double send = hi ? v[i] : v[i + 3], keep = hi ? v[i + 3] : v[i];
+        v[i] = keep + __shfl_xor(send, 8, 64);
+    }
+    v[3] = 0.0;
+    hi = lane & 4;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        double send = hi ? v[i] : v[i + 2], keep = hi ? v[i + 2] : v[i];
+        v[i] = keep + __shfl_xor(send, 4, 64);
+    }
+    hi = lane & 2;
+    {
+        double send = hi ? v[0] : v[1], keep = hi ? v[1] : v[0];
+        v[0] = keep + __shfl_xor(send, 2, 64);
+    }
+    v[0] += __shfl_xor(v[0], 1, 64);
 }
 
+__device__ __forceinline__ int xreduce24_slot(int lane)
+{
+    int g = ((lane >> 1) & 1) + 2 * ((lane >> 2) & 1);
+    if (g == 3) return -1;
+    int q = 12 * ((lane >> 5) & 1) + 6 * ((lane >> 4) & 1) + 3 * ((lane >> 3) & 1) + g;
+    return (q < 23 && !(lane & 1)) ? q : -1;
+}
+
+__device__ __forceinline__ f3 vert(const float* __restrict__ verts, uint32_t i)
+{
+    const float* v = verts + 3 * (size_t)i;
+    return mk3(v[0], v[1], v[2]);
+}
+
+// Per-face arithmetic in FP32 with FP32 per-lane partial sums over the
+// chunk's K faces per lane; the partials are reduced across the wavefront and
+// across chunks in FP64.
 template <int K>
-__global__ __launch_bounds__(256) void k_faces(const uint2* __restrict__ chunks, uint32_t nchunks,
+__global__ __launch_bounds__(256) void k_faces(const Chunk* __restrict__ chunks, uint32_t nchunks,
                                                const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
-                                               OceanK oc, double* __restrict__ partials)
+                                               OceanK oc, const float* __restrict__ depth,
+                                               double* __restrict__ partials)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t c = wave; c < nchunks; c += nwaves) {
-        const uint2 ch = chunks[c];
-        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.x);
-        const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.y);
+        const Chunk ch = chunks[c];
+        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
+        const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
+        const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
         const BodyDyn& D = dyn[b];
         const int pos = __builtin_amdgcn_readfirstlane(D.pos);
         if (pos <= 0) continue;
         const BodyStatic& S = st[b];
-        const uint32_t fend = min(f0 + 64u * K, S.nf);
+        const float* __restrict__ verts = S.verts;
+        const uint32_t* __restrict__ faces = S.faces + 3 * (size_t)f0;
         const FaceBody fb = D.fb;
-        const uint32_t flags = D.flags;
-        Acc<double> acc;
+        Acc<float> acc;
         acc_zero(acc);
+        // Two-stage software pipeline over the lane's K faces: the indices of
+        // face j+2 and the vertices (and depths) of face j+1 are in flight
+        // while face j is computed.  Face numbers past the chunk are clamped
+        // to its last face so every load stays in bounds; their results are
+        // not used.
+        const uint32_t flast = nfc - 1;
+        uint32_t fc0 = min((uint32_t)lane, flast), fc1 = min(64u + lane, flast);
+        uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
+        uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
+        f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
         if (pos == 1) {
             for (int j = 0; j < K; ++j) {
-                uint32_t f = f0 + 64u * j + lane;
-                if (f < fend) {
-                    f3 p1, p2, p3;
-                    load_face(S, f, p1, p2, p3);
-                    face_submerged(fb, p1, p2, p3, acc);
-                }
+                const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
+                f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
+                uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
+                if (64u * j + lane < nfc) face_submerged(fb, p1, p2, p3, acc);
+                p1 = q1; p2 = q2; p3 = q3;
+                b0 = c0; b1 = c1; b2 = c2;
             }
         } else {
+            const uint32_t flags = D.flags;
             const bool damping = flags & BF_DAMPING;
             const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
+            const float* __restrict__ dep = depth + S.dep_off;
+            float d0 = dep[a0], d1 = dep[a1], d2 = dep[a2];
             for (int j = 0; j < K; ++j) {
-                uint32_t f = f0 + 64u * j + lane;
-                if (f < fend) {
-                    f3 p1, p2, p3;
-                    load_face(S, f, p1, p2, p3);
-                    float d0 = depth_vertex(D, oc, p1);
-                    float d1 = depth_vertex(D, oc, p2);
-                    float d2 = depth_vertex(D, oc, p3);
-                    if (!(d0 < 0.f && d1 < 0.f && d2 < 0.f)) {
-                        Wet wt = face_clip(fb, p1, p2, p3, d0, d1, d2, acc);
-                        if (wt.ok) {
-                            f3 c = wt.base + wt.off;
-                            if (wave_buoy) {   // pressure at the wet centroid (1649-1658)
-                                float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x,
-                                                          (double)wt.base.y + (double)wt.off.y,
-                                                          (double)wt.base.z + (double)wt.off.z);
-                                f3 fbi = wt.n1 * (-wt.A * dc);
-                                add3(acc.Fb, fbi);
-                                add3(acc.Mb, cross(c, fbi));
-                            }
-                            if (damping) face_drag(fb, c, wt.n1, wt.A, acc);
-                            acc.Swet += (double)wt.A;
+                const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
+                f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
+                float e0 = dep[b0], e1 = dep[b1], e2 = dep[b2];
+                uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
+                if (64u * j + lane < nfc && !(d0 < 0.f && d1 < 0.f && d2 < 0.f)) {
+                    Wet wt = face_clip(fb, p1, p2, p3, d0, d1, d2, acc);
+                    if (wt.ok) {
+                        f3 cpt = wt.base + wt.off;
+                        if (wave_buoy) {   // pressure at the wet centroid (1649-1658)
+                            float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x,
+                                                      (double)wt.base.y + (double)wt.off.y,
+                                                      (double)wt.base.z + (double)wt.off.z);
+                            f3 fbi = wt.n1 * (-wt.A * dc);
+                            add3(acc.Fb, fbi);
+                            add3(acc.Mb, cross(cpt, fbi));
                         }
+                        if (damping) face_drag(fb, cpt, wt.n1, wt.A, acc);
+                        acc.Swet += wt.A;
                     }
                 }
+                p1 = q1; p2 = q2; p3 = q3;
+                d0 = e0; d1 = e1; d2 = e2;
+                b0 = c0; b1 = c1; b2 = c2;
             }
         }
-        // FP64 wavefront reduction; chunk partial layout:
+        // FP64 wavefront reduction into the chunk partial:
         // Fdq Mdq Fdf Mdf Fb Mb CB (3 each), V6, Swet
-        double* out = partials + (size_t)c * kChunkDoubles;
-        double vals[kChunkDoubles - 1];
+        double vals[24];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             vals[k] = acc.Fdq[k]; vals[3 + k] = acc.Mdq[k];
@@ -406,22 +508,15 @@ __global__ __launch_bounds__(256) void k_faces(const uint2* __restrict__ chunks,
         }
         vals[21] = acc.V6;
         vals[22] = acc.Swet;
-        if (pos == 1) {
-#pragma unroll
-            for (int k = 0; k < 12; ++k) vals[k] = wave_sum_f64(vals[k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < kChunkDoubles - 1; ++k) vals[k] = wave_sum_f64(vals[k]);
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < kChunkDoubles - 1; ++k) out[k] = vals[k];
-        }
+        vals[23] = 0.0;
+        xreduce24(vals, lane);
+        const int q = xreduce24_slot(lane);
+        if (q >= 0) partials[(size_t)c * kChunkDoubles + q] = vals[0];
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_epilogue: one lane per body
+// k_epilogue: one wavefront per body
 // ---------------------------------------------------------------------------
 
 // CorrectHydrodynamicForces (SolidEntity.cpp:1263-1287): M = inverse(basis of
@@ -444,8 +539,8 @@ __device__ void correct64(double rho, const double* BO, const double* Cd, const 
         mat_vec(M, x, d);
         double l2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (l2 >= 2.220446049250313e-16 * 2.220446049250313e-16) {
-            double l = sqrt(l2);
-            d[0] /= l; d[1] /= l; d[2] /= l;
+            double inv = 1.0 / sqrt(l2);   // btVector3::operator/= multiplies by the reciprocal
+            d[0] *= inv; d[1] *= inv; d[2] *= inv;
         } else {
             d[0] = 1.0; d[1] = 0.0; d[2] = 0.0;
         }
@@ -456,10 +551,12 @@ __device__ void correct64(double rho, const double* BO, const double* Cd, const 
     }
 }
 
-__global__ void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t n,
-                           OceanK oc, const double* __restrict__ partials, double* __restrict__ out)
+__global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
+                                                 uint32_t n, OceanK oc, const double* __restrict__ partials,
+                                                 double* __restrict__ out)
 {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int lane = threadIdx.x;
     if (b >= n) return;
     const BodyStatic& S = st[b];
     const BodyDyn& D = dyn[b];
@@ -467,24 +564,27 @@ __global__ void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __r
     const int pos = D.pos;
     const uint32_t flags = D.flags;
     if (pos < 0) {   // not a hydrodynamic body: nothing assigned
-        o[20] = 0.0;
-        o[21] = -1.0;
+        if (lane == 0) { o[20] = 0.0; o[21] = -1.0; }
         return;
     }
     if (pos == 0) {  // OUTSIDE: everything zero (SolidEntity.cpp:1810-1821)
-        for (int k = 0; k < 20; ++k) o[k] = 0.0;
-        o[20] = 255.0;
-        o[21] = 0.0;
+        if (lane < 20) o[lane] = 0.0;
+        if (lane == 0) { o[20] = 255.0; o[21] = 0.0; }
         return;
     }
+    // sum the body's chunk partials: lanes stride over chunks, then a wave
+    // reduction (fixed order, so the result is deterministic)
     double P[23];
 #pragma unroll
     for (int k = 0; k < 23; ++k) P[k] = 0.0;
-    for (uint32_t c = 0; c < S.nchunks; ++c) {
+    for (uint32_t c = lane; c < S.nchunks; c += 64) {
         const double* p = partials + (size_t)(S.chunk0 + c) * kChunkDoubles;
 #pragma unroll
         for (int k = 0; k < 23; ++k) P[k] += p[k];
     }
+#pragma unroll
+    for (int k = 0; k < 23; ++k) P[k] = wave_sum_f64(P[k]);
+    if (lane != 0) return;
     const bool body_level = !(flags & (BF_MODE_SURF | BF_MODE_SUBM));
     const bool damping = flags & BF_DAMPING;
     const bool buoyant = flags & BF_BUOYANT;
@@ -655,21 +755,26 @@ struct sfh_ocean_t {
 };
 
 struct sfh_batch_t {
-    std::vector<sfh_mesh_t*> meshes;   // one reference per body
+    std::vector<sfh_mesh_t*> meshes;   // one reference per body with a mesh
+    std::vector<sfh_mesh_t*> meshes_by_body;
     uint32_t n = 0;
     uint32_t nchunks = 0;
     uint64_t nfaces = 0;
     uint32_t K = 8;
     uint32_t waves = 0;
+    uint32_t nvchunks = 0;
+    uint64_t nverts = 0;
     std::vector<BodyStatic> h_static;
     BodyStatic* d_static = nullptr;
-    uint2* d_chunks = nullptr;
+    Chunk* d_chunks = nullptr;
+    Chunk* d_vchunks = nullptr;
+    float* d_depth = nullptr;
     BodyDyn* d_dyn = nullptr;
     double* d_partials = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool timing = false;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_pending = false;
     double face_ms_sum = 0.0, total_ms_sum = 0.0;
     uint32_t timed = 0;
@@ -713,33 +818,53 @@ extern "C" int sfh_mesh_create(const void* vertex_pos, size_t vertex_stride, uin
     if (!out || (!vertex_pos && nv) || (!face_ids && nf)) return fail(SFH_ERR_INVALID, "null argument");
     if (vertex_stride < 12) return fail(SFH_ERR_INVALID, "vertex stride %zu < 12", vertex_stride);
     if (nf > (1u << 30)) return fail(SFH_ERR_SIZE, "too many faces");
-    std::vector<float> v(3 * (size_t)nv);
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i)
+        if (face_ids[i] >= nv) return fail(SFH_ERR_INVALID, "face %zu references vertex %u >= %u", i / 3, face_ids[i], nv);
+    // Renumber vertices in order of first use by the faces, so each run of
+    // consecutive faces references a compact vertex window.  Positions are
+    // unchanged, so every per-face result is unchanged.
+    std::vector<uint32_t> remap(nv, UINT32_MAX);
+    std::vector<uint32_t> order;
+    order.reserve(nv);
+    std::vector<uint32_t> faces(3 * (size_t)nf);
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i) {
+        uint32_t v = face_ids[i];
+        if (remap[v] == UINT32_MAX) {
+            remap[v] = (uint32_t)order.size();
+            order.push_back(v);
+        }
+        faces[i] = remap[v];
+    }
+    for (uint32_t v = 0; v < nv; ++v)
+        if (remap[v] == UINT32_MAX) {
+            remap[v] = (uint32_t)order.size();
+            order.push_back(v);
+        }
+    std::vector<float> verts(3 * (size_t)nv);
     const char* src = (const char*)vertex_pos;
     float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
     for (uint32_t i = 0; i < nv; ++i) {
         float p[3];
-        memcpy(p, src + (size_t)i * vertex_stride, 12);
+        memcpy(p, src + (size_t)order[i] * vertex_stride, 12);
         for (int k = 0; k < 3; ++k) {
-            v[3 * (size_t)i + k] = p[k];
+            verts[3 * (size_t)i + k] = p[k];
             if (i == 0 || p[k] < mn[k]) mn[k] = p[k];
             if (i == 0 || p[k] > mx[k]) mx[k] = p[k];
         }
     }
-    for (size_t i = 0; i < 3 * (size_t)nf; ++i)
-        if (face_ids[i] >= nv) return fail(SFH_ERR_INVALID, "face %zu references vertex %u >= %u", i / 3, face_ids[i], nv);
     sfh_mesh_t* m = new sfh_mesh_t();
     m->id = g_mesh_ids.fetch_add(1);
     m->nv = nv;
     m->nf = nf;
     for (int k = 0; k < 3; ++k) { m->aabb_min[k] = mn[k]; m->aabb_max[k] = mx[k]; }
     if (nv) {
-        hipError_t e = hipMalloc(&m->d_verts, v.size() * sizeof(float));
-        if (e == hipSuccess) e = hipMemcpy(m->d_verts, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice);
+        hipError_t e = hipMalloc(&m->d_verts, verts.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(m->d_verts, verts.data(), verts.size() * sizeof(float), hipMemcpyHostToDevice);
         if (e != hipSuccess) { hipFree(m->d_verts); delete m; return fail(SFH_ERR_HIP, "mesh upload: %s", hipGetErrorString(e)); }
     }
     if (nf) {
-        hipError_t e = hipMalloc(&m->d_faces, 3 * (size_t)nf * sizeof(uint32_t));
-        if (e == hipSuccess) e = hipMemcpy(m->d_faces, face_ids, 3 * (size_t)nf * sizeof(uint32_t), hipMemcpyHostToDevice);
+        hipError_t e = hipMalloc(&m->d_faces, faces.size() * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemcpy(m->d_faces, faces.data(), faces.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
         if (e != hipSuccess) { hipFree(m->d_verts); hipFree(m->d_faces); delete m; return fail(SFH_ERR_HIP, "mesh upload: %s", hipGetErrorString(e)); }
     }
     *out = m;
@@ -861,10 +986,12 @@ static void batch_free(sfh_batch_t* b)
 {
     if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
     if (b->graph) hipGraphDestroy(b->graph);
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 4; ++i)
         if (b->ev[i]) hipEventDestroy(b->ev[i]);
     hipFree(b->d_static);
     hipFree(b->d_chunks);
+    hipFree(b->d_vchunks);
+    hipFree(b->d_depth);
     hipFree(b->d_dyn);
     hipFree(b->d_partials);
     for (sfh_mesh_t* m : b->meshes) mesh_release(m);
@@ -881,28 +1008,57 @@ static uint32_t default_waves()
     return (uint32_t)cus * 32u;
 }
 
-// (Re)builds the face-prefix chunk list for chunk size 64*K.
+// (Re)builds the work lists: every body's faces cut into chunks of 64*K
+// consecutive faces (the face-prefix work list of the face pass) and its
+// vertices into runs of 512 (the depth pass), plus each body's offset in the
+// vertex-depth buffer.
 static int batch_build_chunks(sfh_batch_t* b)
 {
-    std::vector<uint2> chunks;
-    const uint32_t csize = 64u * b->K;
+    std::vector<Chunk> chunks, vchunks;
+    const uint32_t csize = 64u * b->K, vsize = 512u;
+    uint64_t dep = 0;
     for (uint32_t i = 0; i < b->n; ++i) {
         BodyStatic& s = b->h_static[i];
         s.chunk0 = (uint32_t)chunks.size();
-        uint32_t nc = (s.nf + csize - 1) / csize;
-        s.nchunks = nc;
-        for (uint32_t c = 0; c < nc; ++c) chunks.push_back(make_uint2(i, c * csize));
+        s.nchunks = (s.nf + csize - 1) / csize;
+        for (uint32_t c = 0; c < s.nchunks; ++c) {
+            Chunk ch;
+            ch.body = i;
+            ch.first = c * csize;
+            ch.count = std::min(csize, s.nf - c * csize);
+            ch.pad = 0;
+            chunks.push_back(ch);
+        }
+        s.dep_off = dep;
+        dep += s.nv;
+        for (uint32_t v = 0; v < s.nv; v += vsize) {
+            Chunk ch;
+            ch.body = i;
+            ch.first = v;
+            ch.count = std::min(vsize, s.nv - v);
+            ch.pad = 0;
+            vchunks.push_back(ch);
+        }
     }
     hipFree(b->d_static);
     hipFree(b->d_chunks);
+    hipFree(b->d_vchunks);
+    hipFree(b->d_depth);
     hipFree(b->d_partials);
-    b->d_static = nullptr; b->d_chunks = nullptr; b->d_partials = nullptr;
+    b->d_static = nullptr; b->d_chunks = nullptr; b->d_vchunks = nullptr; b->d_depth = nullptr;
+    b->d_partials = nullptr;
     b->nchunks = (uint32_t)chunks.size();
+    b->nvchunks = (uint32_t)vchunks.size();
+    b->nverts = dep;
     HIPCHK(hipMalloc(&b->d_static, sizeof(BodyStatic) * std::max<size_t>(1, b->n)));
     HIPCHK(hipMemcpy(b->d_static, b->h_static.data(), sizeof(BodyStatic) * b->n, hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&b->d_chunks, sizeof(uint2) * std::max<size_t>(1, chunks.size())));
+    HIPCHK(hipMalloc(&b->d_chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size())));
     if (!chunks.empty())
-        HIPCHK(hipMemcpy(b->d_chunks, chunks.data(), sizeof(uint2) * chunks.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->d_chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b->d_vchunks, sizeof(Chunk) * std::max<size_t>(1, vchunks.size())));
+    if (!vchunks.empty())
+        HIPCHK(hipMemcpy(b->d_vchunks, vchunks.data(), sizeof(Chunk) * vchunks.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b->d_depth, sizeof(float) * std::max<uint64_t>(1, dep)));
     HIPCHK(hipMalloc(&b->d_partials, sizeof(double) * kChunkDoubles * std::max<size_t>(1, chunks.size())));
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -912,8 +1068,8 @@ static int batch_build_chunks(sfh_batch_t* b)
 static int apply_opts(sfh_batch_t* b, const sfh_batch_opts* opts)
 {
     uint32_t K = opts && opts->faces_per_lane ? opts->faces_per_lane : 8u;
-    if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16 && K != 32)
-        return fail(SFH_ERR_INVALID, "faces_per_lane must be 1, 2, 4, 8, 16 or 32");
+    if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16)
+        return fail(SFH_ERR_INVALID, "faces_per_lane must be 1, 2, 4, 8 or 16");
     b->waves = opts && opts->waves ? opts->waves : default_waves();
     if (K != b->K || !b->d_chunks) {
         b->K = K;
@@ -929,6 +1085,7 @@ static BodyStatic make_static(const sfh_mesh_t* m, uint32_t flags)
     s.verts = m ? m->d_verts : nullptr;
     s.faces = m ? m->d_faces : nullptr;
     s.nf = m ? m->nf : 0;
+    s.nv = m ? m->nv : 0;
     s.flags = flags;
     for (int i = 0; i < 3; ++i) s.cg2c[4 * i] = s.cg2o[4 * i] = 1.0;
     return s;
@@ -951,6 +1108,7 @@ extern "C" int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const s
             d.mesh->refs.fetch_add(1);
             b->meshes.push_back(d.mesh);
         }
+        b->meshes_by_body.push_back(d.mesh);
         gl_to_rows(d.T_CG2C, s.cg2c, s.cg2c + 9);
         gl_to_rows(d.T_CG2O, s.cg2o, nullptr);
         for (int k = 0; k < 3; ++k) {
@@ -998,18 +1156,24 @@ extern "C" int sfh_batch_info(sfh_batch b, uint32_t* nbodies, uint32_t* nchunks,
     return SFH_OK;
 }
 
-static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s)
+static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool timed)
 {
     if (b->nchunks == 0) return SFH_OK;
+    if (b->nvchunks) {
+        uint32_t wv = std::min<uint32_t>(b->waves, b->nvchunks);
+        hipLaunchKernelGGL(k_depths, dim3((wv + 3) / 4), dim3(256), 0, s, b->d_vchunks, b->nvchunks, b->d_static,
+                           b->d_dyn, oc, b->d_depth);
+    }
+    if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
     uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
     uint32_t blocks = (waves + 3) / 4;
 #define LAUNCH_K(KK)                                                                                     \
     case KK:                                                                                             \
         hipLaunchKernelGGL(k_faces<KK>, dim3(blocks), dim3(256), 0, s, b->d_chunks, b->nchunks,         \
-                           b->d_static, b->d_dyn, oc, b->d_partials);                                   \
+                           b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);                       \
         break;
     switch (b->K) {
-        LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16) LAUNCH_K(32)
+        LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16)
         default: return fail(SFH_ERR_INVALID, "bad K");
     }
 #undef LAUNCH_K
@@ -1025,10 +1189,10 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
     hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, b->d_dyn);
     if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
-    int rc = launch_faces(b, oc, s);
+    int rc = launch_faces(b, oc, s, timed);
     if (rc != SFH_OK) return rc;
-    if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
-    hipLaunchKernelGGL(k_epilogue, dim3(blocks), dim3(128), 0, s, b->d_static, b->d_dyn, b->n, oc, b->d_partials, d_out);
+    if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
+    hipLaunchKernelGGL(k_epilogue, dim3(b->n), dim3(64), 0, s, b->d_static, b->d_dyn, b->n, oc, b->d_partials, d_out);
     HIPCHK(hipGetLastError());
     return SFH_OK;
 }
@@ -1046,9 +1210,9 @@ static void harvest_timing(sfh_batch_t* b)
 {
     if (!b->ev_pending) return;
     float t_face = 0.f, t_tot = 0.f;
-    if (hipEventSynchronize(b->ev[2]) == hipSuccess &&
-        hipEventElapsedTime(&t_face, b->ev[1], b->ev[2]) == hipSuccess &&
-        hipEventElapsedTime(&t_tot, b->ev[0], b->ev[2]) == hipSuccess) {
+    if (hipEventSynchronize(b->ev[3]) == hipSuccess &&
+        hipEventElapsedTime(&t_face, b->ev[2], b->ev[3]) == hipSuccess &&
+        hipEventElapsedTime(&t_tot, b->ev[0], b->ev[3]) == hipSuccess) {
         b->face_ms_sum += t_face;
         b->total_ms_sum += t_tot;
         b->timed++;
@@ -1121,7 +1285,7 @@ extern "C" int sfh_batch_timing(sfh_batch b, int enable)
 {
     if (!b) return fail(SFH_ERR_INVALID, "null batch");
     if (enable && !b->ev[0])
-        for (int i = 0; i < 3; ++i) HIPCHK(hipEventCreate(&b->ev[i]));
+        for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&b->ev[i]));
     if (enable) harvest_timing(b);
     b->timing = enable != 0;
     b->face_ms_sum = b->total_ms_sum = 0.0;
@@ -1187,6 +1351,7 @@ static int single_prepare(sfh_mesh_t* m, uint32_t flags)
         b->device = dev;
         m->refs.fetch_add(1);
         b->meshes.push_back(m);
+        b->meshes_by_body.push_back(m);
         b->h_static.push_back(make_static(m, flags));
         b->nfaces = b->h_static[0].nf;
         b->K = 0;

@@ -32,12 +32,14 @@ extern "C" void harness_body(int mode, const float* verts, const uint32_t* faces
     double R[9], tc[3] = {T_C[12], T_C[13], T_C[14]}, tcg[3] = {T_CG[12], T_CG[13], T_CG[14]}, y[3], x[3];
     rows(T_C, R);
     FaceBody B;
-    mtv(R, v, y); B.v = mk3((float)y[0], (float)y[1], (float)y[2]);
-    mtv(R, w, y); B.w = mk3((float)y[0], (float)y[1], (float)y[2]);
-    B.u = mk3(0, 0, 0);
+    double vl[3], wl[3], wd[3];
+    mtv(R, v, vl);
+    mtv(R, w, wl); B.w = mk3((float)wl[0], (float)wl[1], (float)wl[2]);
     double dt[3];
     for (int i = 0; i < 3; ++i) x[i] = tc[i] - tcg[i];
-    mtv(R, x, dt); B.dt = mk3((float)dt[0], (float)dt[1], (float)dt[2]);
+    mtv(R, x, dt);
+    cr(wl, dt, wd);
+    B.vb = mk3((float)(vl[0] + wd[0]), (float)(vl[1] + wd[1]), (float)(vl[2] + wd[2]));
     x[0] = tcg[0] - tc[0]; x[1] = tcg[1] - tc[1]; x[2] = -tc[2];
     mtv(R, x, y); B.o = mk3((float)y[0], (float)y[1], (float)y[2]);
     double rz[3] = {R[6], R[7], R[8]}, tz = tc[2];

@@ -242,3 +242,27 @@ def test_mesh_lifetime(gpu):
     ob = O.make_body(O.Mesh.from_tri(tm), volume=tm.volume(), surface=tm.area())
     r = O.body_forces(64, ob, oo, Tcg, [0.3, 0, 0], [0, 0, 0.2])
     assert abs(g[0, 19] / r[19] - 1) < 1e-6
+
+
+def test_unstaged_chunks(gpu):
+    """Faces in random order: vertex windows exceed the LDS stage, so the
+    chunks take the global-gather path; results must not change."""
+    log("case: scrambled face order (unstaged chunks)")
+    rng = np.random.default_rng(9)
+    tm = M.ico_ellipsoid(1.0, 0.4, 0.3, 5)
+    perm = rng.permutation(tm.nf)
+    tms = M.TriMesh(tm.verts, tm.faces[perm])
+    oo, go = O.Ocean(), H.Ocean()
+    g, r, x = np.zeros((6, 24)), np.zeros((6, 24)), np.zeros((6, 22))
+    for s in range(6):
+        gm = H.Mesh(tms if s % 2 else tm)
+        om = O.Mesh.from_tri(tms if s % 2 else tm)
+        Tcg = T.quat_to_gl(T.random_unit_quaternions(rng, 1)[0], [rng.uniform(-5, 5), 2.0, rng.uniform(-0.2, 0.2)])
+        v, w_ = rng.uniform(-2, 2, 3), rng.uniform(-1, 1, 3)
+        outs = [np.zeros(3) for _ in range(6)]
+        sv = np.zeros(2)
+        H.ComputeHydrodynamicForcesSurface(H.HydrodynamicsSettings(), gm, go, Tcg, Tcg, v, w_, *outs, sv)
+        g[s, :18] = np.concatenate(outs)
+        g[s, 18:20] = sv
+        r[s], x[s] = O.surface(64, 1, 1, om, oo, Tcg, Tcg, v, w_, full=True)
+    assert_bound(g, r, x, "unstaged chunks")
