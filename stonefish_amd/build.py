@@ -67,6 +67,17 @@ def build_hip(verbose: bool = False) -> list[Path]:
     return out
 
 
+def build_cpp_check(verbose: bool = False) -> Path:
+    """Builds the C++ interface check program (tests/cpp) against libsfhydro.so."""
+    target = LIBDIR / "sfh_cpp_check"
+    cmd = ["g++", "-O2", "-std=c++17", "-o", str(target), str(ROOT / "tests" / "cpp" / "sfh_cpp_check.cpp"),
+           f"-L{LIBDIR}", "-lsfhydro", f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    return target
+
+
 def build_oracle(verbose: bool = False) -> None:
     """Builds the CPU oracle (test infrastructure) and, where the reference
     checkout exists, oracle/_ref."""
@@ -78,6 +89,7 @@ def build_oracle(verbose: bool = False) -> None:
 
 def build_all(verbose: bool = False) -> None:
     build_hip(verbose)
+    build_cpp_check(verbose)
     build_oracle(verbose)
 
 

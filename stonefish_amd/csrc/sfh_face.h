@@ -76,6 +76,18 @@ SFH_HD void acc_zero(Acc<T>& a)
     a.Swet = T(0);
 }
 
+// Accumulators of the submerged loop (drag only).
+template <typename T>
+struct AccDrag {
+    T Fdq[3], Mdq[3], Fdf[3], Mdf[3];
+};
+
+template <typename T>
+SFH_HD void acc_zero(AccDrag<T>& a)
+{
+    for (int k = 0; k < 3; ++k) a.Fdq[k] = a.Mdq[k] = a.Fdf[k] = a.Mdf[k] = T(0);
+}
+
 template <typename T>
 SFH_HD void add3(T* acc, f3 x)
 {
@@ -87,8 +99,8 @@ SFH_HD void add3(T* acc, f3 x)
 // quadratic q = vc |vc| (-vc.n1) A = vm (-|vm| (vm.n1) A), applied when
 // vc.n1 < -1e-12; tangential vt = vc - n1 (vc.n1) = n1 (vm.n1) - vm, skin
 // s = vt A when |vt|^2 > 1e-9.
-template <typename T>
-SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, Acc<T>& acc)
+template <class AccT>
+SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, AccT& acc)
 {
     f3 vm = B.vb + cross(B.w, c);
     float vn = dot(vm, n1);
@@ -107,8 +119,8 @@ SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, Acc<T>& acc)
 }
 
 // Submerged face (no clipping, no buoyancy): SolidEntity.cpp:1749-1791.
-template <typename T>
-SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
+template <class AccT>
+SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, AccT& acc)
 {
     f3 e1 = p2 - p1, e2 = p3 - p1;
     f3 n = cross(e1, e2);

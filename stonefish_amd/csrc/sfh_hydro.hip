@@ -2,12 +2,16 @@
 // hydrodynamics (SolidEntity.cpp:1239-1850, Ocean.cpp:149-201,
 // OpenGLRealOcean.cpp:280-331, Compound-free body path).
 //
-// One evaluation pass over a batch of bodies is four launches on one stream:
+// One evaluation pass over a batch of bodies is six launches on one stream:
 //   k_prologue  one lane per body, FP64: C transform, body/fluid classification
 //               (8 AABB corners against the surface), per-body FP32 constants of
 //               the face loop in the mesh frame, FP64 constants of the epilogue;
+//   k_classify  one workgroup: this step's work lists (chunks of submerged and
+//               of crossing bodies, vertex chunks of crossing bodies), in body
+//               order;
 //   k_depths    FP64 depth of every vertex of the bodies crossing the surface;
-//   k_faces     one wavefront per chunk of up to 64*K faces of one body: FP32
+//   k_faces_sub / k_faces_cross
+//               one wavefront per chunk of up to 64*K faces of one body: FP32
 //               face arithmetic with FP32 per-lane partials, FP64 wavefront
 //               reduction; one 24-double partial per chunk;
 //   k_epilogue  one wavefront per body, FP64: sums the body's chunk partials
@@ -77,6 +81,8 @@ struct BodyStatic {
     uint32_t chunk0;
     uint32_t nchunks;
     uint32_t nv;
+    uint32_t vchunk0;   // first vertex chunk of the body (depth pass)
+    uint32_t nvchunks;
     uint32_t pad_;
     uint64_t dep_off;   // offset of this body's vertex depths in the depth buffer
     double cg2c[12];   // basis row-major, origin
@@ -331,18 +337,18 @@ struct Chunk {
 // the clip points of thin wet slivers are built from it, so it is evaluated in
 // FP64 once per vertex (a mesh vertex is shared by about six faces) and stored
 // as float (relative precision is kept for small depths).
-__global__ __launch_bounds__(256) void k_depths(const Chunk* __restrict__ vchunks, uint32_t nvchunks,
-                                                const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
-                                                OceanK oc, float* __restrict__ depth)
+__global__ __launch_bounds__(256) void k_depths(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                                                const Chunk* __restrict__ vchunks, const BodyStatic* __restrict__ st,
+                                                const BodyDyn* __restrict__ dyn, OceanK oc,
+                                                float* __restrict__ depth)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t c = wave; c < nvchunks; c += nwaves) {
-        const Chunk ch = vchunks[c];
+    const uint32_t nl = count[0];
+    for (uint32_t li = wave; li < nl; li += nwaves) {
+        const Chunk ch = vchunks[__builtin_amdgcn_readfirstlane(list[li])];
         const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
-        const int pos = __builtin_amdgcn_readfirstlane(dyn[b].pos);
-        if (pos != 2) continue;
         const BodyDyn& D = dyn[b];
         const BodyStatic& S = st[b];
         const uint32_t v0 = __builtin_amdgcn_readfirstlane(ch.first);
@@ -419,85 +425,165 @@ __device__ __forceinline__ f3 vert(const float* __restrict__ verts, uint32_t i)
     return mk3(v[0], v[1], v[2]);
 }
 
-// Per-face arithmetic in FP32 with FP32 per-lane partial sums over the
-// chunk's K faces per lane; the partials are reduced across the wavefront and
-// across chunks in FP64.
+// Transposed FP64 reduction of 12 per-lane values: on return v[0] of lane l
+// holds the total of quantity q = 6 b5 + 3 b4 + 2 b3 + b2 ((2 b3 + b2) == 3 is
+// padding); lanes with b1 = b0 = 0 hold it.
+__device__ __forceinline__ void xreduce12(double (&v)[12], int lane)
+{
+    bool hi = lane & 32;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double send = hi ? v[i] : v[i + 6], keep = hi ? v[i + 6] : v[i];
+        v[i] = keep + __shfl_xor(send, 32, 64);
+    }
+    hi = lane & 16;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double send = hi ? v[i] : v[i + 3], keep = hi ? v[i + 3] : v[i];
+        v[i] = keep + __shfl_xor(send, 16, 64);
+    }
+    v[3] = 0.0;
+    hi = lane & 8;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        double send = hi ? v[i] : v[i + 2], keep = hi ? v[i + 2] : v[i];
+        v[i] = keep + __shfl_xor(send, 8, 64);
+    }
+    hi = lane & 4;
+    {
+        double send = hi ? v[0] : v[1], keep = hi ? v[1] : v[0];
+        v[0] = keep + __shfl_xor(send, 4, 64);
+    }
+    v[0] += __shfl_xor(v[0], 2, 64);
+    v[0] += __shfl_xor(v[0], 1, 64);
+}
+
+__device__ __forceinline__ int xreduce12_slot(int lane)
+{
+    int g = ((lane >> 2) & 1) + 2 * ((lane >> 3) & 1);
+    if (g == 3 || (lane & 3)) return -1;
+    return 6 * ((lane >> 5) & 1) + 3 * ((lane >> 4) & 1) + g;
+}
+
+// Submerged loop (bodies fully below the surface): per-face arithmetic in FP32
+// with FP32 per-lane partial sums over the chunk's K faces per lane, reduced
+// across the wavefront and across chunks in FP64.  `list` holds the ids of the
+// chunks of this step's submerged bodies (built by k_classify).
 template <int K>
-__global__ __launch_bounds__(256) void k_faces(const Chunk* __restrict__ chunks, uint32_t nchunks,
-                                               const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
-                                               OceanK oc, const float* __restrict__ depth,
-                                               double* __restrict__ partials)
+__global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ count,
+                                                   const Chunk* __restrict__ chunks,
+                                                   const BodyStatic* __restrict__ st,
+                                                   const BodyDyn* __restrict__ dyn, double* __restrict__ partials)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t c = wave; c < nchunks; c += nwaves) {
+    const uint32_t n = count[0];
+    for (uint32_t li = wave; li < n; li += nwaves) {
+        const uint32_t c = __builtin_amdgcn_readfirstlane(list[li]);
         const Chunk ch = chunks[c];
         const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
         const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
         const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
-        const BodyDyn& D = dyn[b];
-        const int pos = __builtin_amdgcn_readfirstlane(D.pos);
-        if (pos <= 0) continue;
         const BodyStatic& S = st[b];
         const float* __restrict__ verts = S.verts;
         const uint32_t* __restrict__ faces = S.faces + 3 * (size_t)f0;
-        const FaceBody fb = D.fb;
-        Acc<float> acc;
+        const FaceBody fb = dyn[b].fb;
+        AccDrag<float> acc;
         acc_zero(acc);
         // Two-stage software pipeline over the lane's K faces: the indices of
-        // face j+2 and the vertices (and depths) of face j+1 are in flight
-        // while face j is computed.  Face numbers past the chunk are clamped
-        // to its last face so every load stays in bounds; their results are
-        // not used.
+        // face j+2 and the vertices of face j+1 are in flight while face j is
+        // computed.  Face numbers past the chunk are clamped to its last face
+        // so every load stays in bounds; their results are not used.
         const uint32_t flast = nfc - 1;
         uint32_t fc0 = min((uint32_t)lane, flast), fc1 = min(64u + lane, flast);
         uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
         uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
         f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
-        if (pos == 1) {
-            for (int j = 0; j < K; ++j) {
-                const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
-                f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
-                uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
-                if (64u * j + lane < nfc) face_submerged(fb, p1, p2, p3, acc);
-                p1 = q1; p2 = q2; p3 = q3;
-                b0 = c0; b1 = c1; b2 = c2;
-            }
-        } else {
-            const uint32_t flags = D.flags;
-            const bool damping = flags & BF_DAMPING;
-            const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
-            const float* __restrict__ dep = depth + S.dep_off;
-            float d0 = dep[a0], d1 = dep[a1], d2 = dep[a2];
-            for (int j = 0; j < K; ++j) {
-                const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
-                f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
-                float e0 = dep[b0], e1 = dep[b1], e2 = dep[b2];
-                uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
-                if (64u * j + lane < nfc && !(d0 < 0.f && d1 < 0.f && d2 < 0.f)) {
-                    Wet wt = face_clip(fb, p1, p2, p3, d0, d1, d2, acc);
-                    if (wt.ok) {
-                        f3 cpt = wt.base + wt.off;
-                        if (wave_buoy) {   // pressure at the wet centroid (1649-1658)
-                            float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x,
-                                                      (double)wt.base.y + (double)wt.off.y,
-                                                      (double)wt.base.z + (double)wt.off.z);
-                            f3 fbi = wt.n1 * (-wt.A * dc);
-                            add3(acc.Fb, fbi);
-                            add3(acc.Mb, cross(cpt, fbi));
-                        }
-                        if (damping) face_drag(fb, cpt, wt.n1, wt.A, acc);
-                        acc.Swet += wt.A;
-                    }
-                }
-                p1 = q1; p2 = q2; p3 = q3;
-                d0 = e0; d1 = e1; d2 = e2;
-                b0 = c0; b1 = c1; b2 = c2;
-            }
+        for (int j = 0; j < K; ++j) {
+            const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
+            f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
+            uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
+            if (64u * j + lane < nfc) face_submerged(fb, p1, p2, p3, acc);
+            p1 = q1; p2 = q2; p3 = q3;
+            b0 = c0; b1 = c1; b2 = c2;
         }
-        // FP64 wavefront reduction into the chunk partial:
-        // Fdq Mdq Fdf Mdf Fb Mb CB (3 each), V6, Swet
+        double vals[12];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            vals[k] = acc.Fdq[k]; vals[3 + k] = acc.Mdq[k];
+            vals[6 + k] = acc.Fdf[k]; vals[9 + k] = acc.Mdf[k];
+        }
+        xreduce12(vals, lane);
+        const int q = xreduce12_slot(lane);
+        if (q >= 0) partials[(size_t)c * kChunkDoubles + q] = vals[0];
+    }
+}
+
+// Surface loop (bodies crossing the surface): waterline clip with the vertex
+// depths of k_depths, tetra volumes, pressure buoyancy with waves, drag.
+template <int K>
+__global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ count,
+                                                     const Chunk* __restrict__ chunks,
+                                                     const BodyStatic* __restrict__ st,
+                                                     const BodyDyn* __restrict__ dyn, OceanK oc,
+                                                     const float* __restrict__ depth, double* __restrict__ partials)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t n = count[0];
+    for (uint32_t li = wave; li < n; li += nwaves) {
+        const uint32_t c = __builtin_amdgcn_readfirstlane(list[li]);
+        const Chunk ch = chunks[c];
+        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
+        const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
+        const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
+        const BodyDyn& D = dyn[b];
+        const BodyStatic& S = st[b];
+        const float* __restrict__ verts = S.verts;
+        const uint32_t* __restrict__ faces = S.faces + 3 * (size_t)f0;
+        const FaceBody fb = D.fb;
+        const uint32_t flags = D.flags;
+        const bool damping = flags & BF_DAMPING;
+        const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
+        const float* __restrict__ dep = depth + S.dep_off;
+        Acc<float> acc;
+        acc_zero(acc);
+        const uint32_t flast = nfc - 1;
+        uint32_t fc0 = min((uint32_t)lane, flast), fc1 = min(64u + lane, flast);
+        uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
+        uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
+        f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
+        float d0 = dep[a0], d1 = dep[a1], d2 = dep[a2];
+        for (int j = 0; j < K; ++j) {
+            const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
+            f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
+            float e0 = dep[b0], e1 = dep[b1], e2 = dep[b2];
+            uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
+            if (64u * j + lane < nfc && !(d0 < 0.f && d1 < 0.f && d2 < 0.f)) {
+                Wet wt = face_clip(fb, p1, p2, p3, d0, d1, d2, acc);
+                if (wt.ok) {
+                    f3 cpt = wt.base + wt.off;
+                    if (wave_buoy) {   // pressure at the wet centroid (1649-1658)
+                        float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x,
+                                                  (double)wt.base.y + (double)wt.off.y,
+                                                  (double)wt.base.z + (double)wt.off.z);
+                        f3 fbi = wt.n1 * (-wt.A * dc);
+                        add3(acc.Fb, fbi);
+                        add3(acc.Mb, cross(cpt, fbi));
+                    }
+                    if (damping) face_drag(fb, cpt, wt.n1, wt.A, acc);
+                    acc.Swet += wt.A;
+                }
+            }
+            p1 = q1; p2 = q2; p3 = q3;
+            d0 = e0; d1 = e1; d2 = e2;
+            b0 = c0; b1 = c1; b2 = c2;
+        }
+        // chunk partial layout: Fdq Mdq Fdf Mdf Fb Mb CB (3 each), V6, Swet
         double vals[24];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -512,6 +598,68 @@ __global__ __launch_bounds__(256) void k_faces(const Chunk* __restrict__ chunks,
         xreduce24(vals, lane);
         const int q = xreduce24_slot(lane);
         if (q >= 0) partials[(size_t)c * kChunkDoubles + q] = vals[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_classify: deterministic per-step work lists (one workgroup)
+// ---------------------------------------------------------------------------
+
+// After k_prologue has set each body's position, writes the ids of the face
+// chunks of submerged bodies, of crossing bodies, and of the vertex chunks of
+// crossing bodies, in body order, with their counts in counts[0..2].
+static constexpr int kClassifyThreads = 1024;
+
+__global__ __launch_bounds__(kClassifyThreads) void k_classify(const BodyStatic* __restrict__ st,
+                                                               const BodyDyn* __restrict__ dyn, uint32_t n,
+                                                               uint32_t* __restrict__ list_sub,
+                                                               uint32_t* __restrict__ list_cross,
+                                                               uint32_t* __restrict__ list_vert,
+                                                               uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t wsum[3][kClassifyThreads / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t per = (n + kClassifyThreads - 1) / kClassifyThreads;
+    const uint32_t b0 = min(t * per, n), b1 = min(b0 + per, n);
+    uint32_t c[3] = {0, 0, 0};
+    for (uint32_t b = b0; b < b1; ++b) {
+        const int pos = dyn[b].pos;
+        const uint32_t nc = st[b].nchunks, nvc = st[b].nvchunks;
+        if (pos == 1) c[0] += nc;
+        if (pos == 2) { c[1] += nc; c[2] += nvc; }
+    }
+    // exclusive scan over threads: wave scan, then over wave totals
+    uint32_t incl[3] = {c[0], c[1], c[2]};
+    for (int off = 1; off < 64; off <<= 1)
+        for (int k = 0; k < 3; ++k) {
+            uint32_t y = __shfl_up(incl[k], off, 64);
+            if ((int)lane >= off) incl[k] += y;
+        }
+    if (lane == 63)
+        for (int k = 0; k < 3; ++k) wsum[k][w] = incl[k];
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run[3] = {0, 0, 0};
+        for (int i = 0; i < kClassifyThreads / 64; ++i)
+            for (int k = 0; k < 3; ++k) { uint32_t x = wsum[k][i]; wsum[k][i] = run[k]; run[k] += x; }
+        for (int k = 0; k < 3; ++k) counts[k] = run[k];
+    }
+    __syncthreads();
+    uint32_t o[3];
+    for (int k = 0; k < 3; ++k) o[k] = wsum[k][w] + incl[k] - c[k];
+    for (uint32_t b = b0; b < b1; ++b) {
+        const int pos = dyn[b].pos;
+        const uint32_t c0 = st[b].chunk0, nc = st[b].nchunks, v0 = st[b].vchunk0, nvc = st[b].nvchunks;
+        if (pos == 1) {
+            for (uint32_t i = 0; i < nc; ++i) list_sub[o[0] + i] = c0 + i;
+            o[0] += nc;
+        }
+        if (pos == 2) {
+            for (uint32_t i = 0; i < nc; ++i) list_cross[o[1] + i] = c0 + i;
+            for (uint32_t i = 0; i < nvc; ++i) list_vert[o[2] + i] = v0 + i;
+            o[1] += nc;
+            o[2] += nvc;
+        }
     }
 }
 
@@ -769,6 +917,7 @@ struct sfh_batch_t {
     Chunk* d_chunks = nullptr;
     Chunk* d_vchunks = nullptr;
     float* d_depth = nullptr;
+    uint32_t* d_lists = nullptr;    // list_sub [nchunks], list_cross [nchunks], list_vert [nvchunks], counts [4]
     BodyDyn* d_dyn = nullptr;
     double* d_partials = nullptr;
     hipGraph_t graph = nullptr;
@@ -992,6 +1141,7 @@ static void batch_free(sfh_batch_t* b)
     hipFree(b->d_chunks);
     hipFree(b->d_vchunks);
     hipFree(b->d_depth);
+    hipFree(b->d_lists);
     hipFree(b->d_dyn);
     hipFree(b->d_partials);
     for (sfh_mesh_t* m : b->meshes) mesh_release(m);
@@ -1031,6 +1181,8 @@ static int batch_build_chunks(sfh_batch_t* b)
         }
         s.dep_off = dep;
         dep += s.nv;
+        s.vchunk0 = (uint32_t)vchunks.size();
+        s.nvchunks = (s.nv + vsize - 1) / vsize;
         for (uint32_t v = 0; v < s.nv; v += vsize) {
             Chunk ch;
             ch.body = i;
@@ -1045,8 +1197,9 @@ static int batch_build_chunks(sfh_batch_t* b)
     hipFree(b->d_vchunks);
     hipFree(b->d_depth);
     hipFree(b->d_partials);
+    hipFree(b->d_lists);
     b->d_static = nullptr; b->d_chunks = nullptr; b->d_vchunks = nullptr; b->d_depth = nullptr;
-    b->d_partials = nullptr;
+    b->d_partials = nullptr; b->d_lists = nullptr;
     b->nchunks = (uint32_t)chunks.size();
     b->nvchunks = (uint32_t)vchunks.size();
     b->nverts = dep;
@@ -1060,6 +1213,7 @@ static int batch_build_chunks(sfh_batch_t* b)
         HIPCHK(hipMemcpy(b->d_vchunks, vchunks.data(), sizeof(Chunk) * vchunks.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&b->d_depth, sizeof(float) * std::max<uint64_t>(1, dep)));
     HIPCHK(hipMalloc(&b->d_partials, sizeof(double) * kChunkDoubles * std::max<size_t>(1, chunks.size())));
+    HIPCHK(hipMalloc(&b->d_lists, sizeof(uint32_t) * (2 * (size_t)b->nchunks + b->nvchunks + 8)));
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
     return SFH_OK;
@@ -1156,21 +1310,33 @@ extern "C" int sfh_batch_info(sfh_batch b, uint32_t* nbodies, uint32_t* nchunks,
     return SFH_OK;
 }
 
+// Launches the per-step work-list build, the depth pass and the two face
+// passes.
 static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool timed)
 {
-    if (b->nchunks == 0) return SFH_OK;
-    if (b->nvchunks) {
-        uint32_t wv = std::min<uint32_t>(b->waves, b->nvchunks);
-        hipLaunchKernelGGL(k_depths, dim3((wv + 3) / 4), dim3(256), 0, s, b->d_vchunks, b->nvchunks, b->d_static,
-                           b->d_dyn, oc, b->d_depth);
+    if (b->nchunks == 0) {
+        if (timed) { HIPCHK(hipEventRecord(b->ev[1], s)); HIPCHK(hipEventRecord(b->ev[2], s)); }
+        return SFH_OK;
     }
+    uint32_t* list_sub = b->d_lists;   // list_sub [nchunks], list_cross [nchunks], list_vert [nvchunks], counts [8]
+    uint32_t* list_cross = list_sub + b->nchunks;
+    uint32_t* list_vert = list_cross + b->nchunks;
+    uint32_t* counts = list_vert + b->nvchunks;   // list sizes
+    hipLaunchKernelGGL(k_classify, dim3(1), dim3(kClassifyThreads), 0, s, b->d_static, b->d_dyn, b->n, list_sub,
+                       list_cross, list_vert, counts);
+    if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
+    uint32_t wv = std::min<uint32_t>(b->waves, std::max<uint32_t>(1, b->nvchunks));
+    hipLaunchKernelGGL(k_depths, dim3((wv + 3) / 4), dim3(256), 0, s, list_vert, counts + 2, b->d_vchunks,
+                       b->d_static, b->d_dyn, oc, b->d_depth);
     if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
     uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
     uint32_t blocks = (waves + 3) / 4;
 #define LAUNCH_K(KK)                                                                                     \
     case KK:                                                                                             \
-        hipLaunchKernelGGL(k_faces<KK>, dim3(blocks), dim3(256), 0, s, b->d_chunks, b->nchunks,         \
-                           b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);                       \
+        hipLaunchKernelGGL(k_faces_sub<KK>, dim3(blocks), dim3(256), 0, s, list_sub, counts, b->d_chunks, \
+                           b->d_static, b->d_dyn, b->d_partials);                                       \
+        hipLaunchKernelGGL(k_faces_cross<KK>, dim3(blocks), dim3(256), 0, s, list_cross, counts + 1,     \
+                           b->d_chunks, b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);          \
         break;
     switch (b->K) {
         LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16)
@@ -1188,7 +1354,6 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
     uint32_t blocks = (b->n + 127) / 128;
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
     hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, b->d_dyn);
-    if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
     int rc = launch_faces(b, oc, s, timed);
     if (rc != SFH_OK) return rc;
     if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
