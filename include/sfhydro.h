@@ -165,6 +165,14 @@ int sfh_batch_compute(sfh_batch batch, sfh_ocean ocean, const double* d_states, 
 /* Same with host buffers; synchronous. */
 int sfh_batch_compute_host(sfh_batch batch, sfh_ocean ocean, const double* states, double* out);
 
+/* sfh_batch_compute restricted to the bodies with d_active[i] != 0 (device,
+ * n bytes; NULL = all).  The work lists of the step hold only the active
+ * bodies' faces; the output rows of inactive bodies are not written.  This is
+ * the exact fallback of the surrogate (the bodies outside its training range)
+ * and its audit. */
+int sfh_batch_compute_masked(sfh_batch batch, sfh_ocean ocean, const double* d_states,
+                             const uint8_t* d_active, double* d_out, void* stream);
+
 /* Captures sfh_batch_compute(batch, ocean, d_states, d_out) on `stream` into a
  * HIP graph held by the batch, then sfh_batch_replay launches it. */
 int sfh_batch_capture(sfh_batch batch, sfh_ocean ocean, const double* d_states, double* d_out, void* stream);
@@ -193,6 +201,56 @@ typedef struct {
  * fftData. */
 int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_t n,
                         float* d_grid, void* stream);
+
+/* ---- surrogate glue (no counterpart in the reference) -------------------- */
+
+/* The learned surrogate (sfmlp.h) sees 13 features of a body's state:
+ *   0-2   linear velocity relative to the current at the CG, CG frame (R^T (v - u))
+ *   3-5   angular velocity, CG frame (R^T omega)
+ *   6-9   orientation of the CG frame, unit quaternion (x, y, z, w), w >= 0
+ *   10    depth of the centre of buoyancy below the local surface (Ocean::GetDepth)
+ *   11-12 slope of the depth offset along world x and y (central difference
+ *         over one texel of the first wave layer; 0 on a flat sea)
+ * and predicts the total force and torque about the CG, CG frame:
+ *   R^T (Fb + Fdq + Fdf), R^T (Tb + Tdq + Tdf) after CorrectHydrodynamicForces. */
+#define SFH_SUR_FEATURES 13
+#define SFH_SUR_ROW 32        /* network input row: 13 features, zero-padded */
+
+typedef struct {
+    float in_shift[SFH_SUR_FEATURES];   /* network input = (feature - shift) * scale */
+    float in_scale[SFH_SUR_FEATURES];
+    float in_lo[SFH_SUR_FEATURES];      /* training range: a feature outside it makes */
+    float in_hi[SFH_SUR_FEATURES];      /* the body out-of-distribution               */
+    float out_shift[6];                 /* CG-frame F, T = network output * scale + shift */
+    float out_scale[6];
+} sfh_surrogate_norm;
+
+/* Features of bodies first .. first+count-1 of the batch (states: the whole
+ * batch's state array, device).  Every output is optional (NULL) and indexed
+ * from 0:
+ *   d_raw      count x 13 FP32 features
+ *   d_x        rows x 32 BF16 network inputs (rows >= count, a multiple of 32;
+ *              rows past count are zero); needs norm
+ *   d_ood      count bytes, 1 where a feature is outside [in_lo, in_hi] or not
+ *              finite; needs norm
+ * and, from the exact outputs d_exact (the batch's n x SFH_OUT_DOUBLES array;
+ * fields whose written bit is clear count as zero):
+ *   d_raw_target  count x 6 FP32 CG-frame total F, T
+ *   d_target      count x 6 FP32 normalised targets; needs norm */
+int sfh_surrogate_features(sfh_batch batch, sfh_ocean ocean, const double* d_states, uint32_t first,
+                           uint32_t count, uint32_t rows, const sfh_surrogate_norm* norm,
+                           float* d_raw, uint16_t* d_x, uint8_t* d_ood,
+                           const double* d_exact, float* d_raw_target, float* d_target, void* stream);
+
+/* World-frame total force and torque (count x 6 doubles, d_ft) of bodies
+ * first .. first+count-1 from the network outputs d_y (count x 6 FP32): bodies
+ * flagged in d_ood (optional) take the exact totals from d_exact instead.
+ * With d_exact and d_err (count x 2 FP32, optional), also the audit errors of
+ * the network against the exact result: |F - F_exact| / |F_exact| and
+ * |T - T_exact| / |T_exact| (0 when both are 0). */
+int sfh_surrogate_apply(sfh_batch batch, const double* d_states, uint32_t first, uint32_t count,
+                        const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
+                        const double* d_exact, double* d_ft, float* d_err, void* stream);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,21 @@ class sfh_wave_component(C.Structure):
     _fields_ = [("amplitude", C.c_float), ("phase", C.c_float), ("mx", C.c_int32), ("my", C.c_int32)]
 
 
+SFH_SUR_FEATURES = 13
+SFH_SUR_ROW = 32
+
+
+class sfh_surrogate_norm(C.Structure):
+    _fields_ = [
+        ("in_shift", C.c_float * 13),
+        ("in_scale", C.c_float * 13),
+        ("in_lo", C.c_float * 13),
+        ("in_hi", C.c_float * 13),
+        ("out_shift", C.c_float * 6),
+        ("out_scale", C.c_float * 6),
+    ]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "sfh_last_error": (C.c_char_p, []),
@@ -80,12 +95,17 @@ _PROTOS = {
     "sfh_batch_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint32_p, C.POINTER(C.c_uint64)]),
     "sfh_batch_compute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfh_batch_compute_host": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
+    "sfh_batch_compute_masked": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfh_batch_capture": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfh_batch_replay": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sfh_batch_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sfh_batch_kernel_ms": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_uint32_p]),
     "sfh_wave_grid_build": (C.c_int, [C.POINTER(sfh_wave_component), C.c_uint32, C.c_uint32, C.c_void_p,
                                       C.c_void_p]),
+    "sfh_surrogate_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 7),
+    "sfh_surrogate_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                      C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 6),
 }
 
 EXPORTED = tuple(_PROTOS)
@@ -126,3 +146,70 @@ def check(rc: int) -> None:
     if rc != SFH_OK:
         msg = load().sfh_last_error()
         raise SfhError(rc, msg.decode() if msg else "")
+
+
+# ---- libsfmlp.so (include/sfmlp.h) ------------------------------------------
+
+SFM_OK = 0
+SFM_IN = 13
+SFM_IN_PAD = 32
+SFM_HIDDEN = 256
+SFM_LAYERS = 5
+SFM_OUT = 6
+SFM_OUT_PAD = 32
+SFM_DIMS = (SFM_IN_PAD, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_OUT_PAD)
+
+_MLP_PROTOS = {
+    "sfm_last_error": (C.c_char_p, []),
+    "sfm_create": (C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sfm_destroy": (C.c_int, [C.c_void_p]),
+    "sfm_param_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "sfm_layer_io": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_int]),
+    "sfm_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "sfm_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                 C.c_float, C.c_void_p, C.c_void_p]),
+    "sfm_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfm_to_bf16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "sfm_probe_mfma": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfm_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "sfm_kernel_ms": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_uint32_p]),
+}
+
+MLP_EXPORTED = tuple(_MLP_PROTOS)
+
+_mlp = None
+
+
+def mlp_lib_path() -> Path:
+    return LIBDIR / "libsfmlp.so"
+
+
+def load_mlp() -> C.CDLL:
+    """Loads libsfmlp.so (raises if it has not been built)."""
+    global _mlp
+    if _mlp is not None:
+        return _mlp
+    p = mlp_lib_path()
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: the HIP library has not been built "
+            "(run `python -m stonefish_amd.build` or __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in _MLP_PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _mlp = lib
+    return lib
+
+
+class SfmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sfmlp error {code}: {msg}")
+        self.code = code
+
+
+def check_mlp(rc: int) -> None:
+    if rc != SFM_OK:
+        msg = load_mlp().sfm_last_error()
+        raise SfmError(rc, msg.decode() if msg else "")

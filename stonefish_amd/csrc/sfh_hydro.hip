@@ -217,11 +217,17 @@ __device__ __forceinline__ f3 to_f3(const double* x) { return mk3((float)x[0], (
 
 // states: kind 0 = body state (T_CG[16], v, w; T_C = T_CG * T_CG2C),
 //         kind 1 = explicit (T_CG[16], T_C[16], v, w).
+// active (optional): bodies with active[b] == 0 get position -2 and are left
+// out of the step entirely.
 __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
-                           uint32_t n, OceanK oc, BodyDyn* __restrict__ dyn)
+                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active, BodyDyn* __restrict__ dyn)
 {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
+    if (active && !active[b]) {
+        dyn[b].pos = -2;
+        return;
+    }
     const BodyStatic& S = st[b];
     BodyDyn D;
     const double* s = states + (size_t)b * (state_kind ? 38 : 22);
@@ -711,6 +717,7 @@ __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ 
     double* o = out + (size_t)b * SFH_OUT_DOUBLES;
     const int pos = D.pos;
     const uint32_t flags = D.flags;
+    if (pos == -2) return;   // inactive in a masked step: row not written
     if (pos < 0) {   // not a hydrodynamic body: nothing assigned
         if (lane == 0) { o[20] = 0.0; o[21] = -1.0; }
         return;
@@ -1347,13 +1354,14 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
 }
 
 static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_states, int kind, double* d_out,
-                        hipStream_t s, bool timed)
+                        hipStream_t s, bool timed, const uint8_t* active = nullptr)
 {
     if (b->n == 0) return SFH_OK;
     OceanK oc = make_oceank(o);
     uint32_t blocks = (b->n + 127) / 128;
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
-    hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, b->d_dyn);
+    hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, active,
+                       b->d_dyn);
     int rc = launch_faces(b, oc, s, timed);
     if (rc != SFH_OK) return rc;
     if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
@@ -1394,6 +1402,20 @@ extern "C" int sfh_batch_compute(sfh_batch b, sfh_ocean o, const double* d_state
     bool timed = b->timing;
     if (timed) harvest_timing(b);
     rc = batch_launch(b, o, d_states, 0, d_out, s, timed);
+    if (rc == SFH_OK && timed) b->ev_pending = true;
+    return rc;
+}
+
+extern "C" int sfh_batch_compute_masked(sfh_batch b, sfh_ocean o, const double* d_states, const uint8_t* d_active,
+                                        double* d_out, void* stream)
+{
+    if (!b || !d_states || !d_out) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(o);
+    if (rc != SFH_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    bool timed = b->timing;
+    if (timed) harvest_timing(b);
+    rc = batch_launch(b, o, d_states, 0, d_out, s, timed, d_active);
     if (rc == SFH_OK && timed) b->ev_pending = true;
     return rc;
 }
@@ -1649,5 +1671,212 @@ extern "C" int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t nco
     hipError_t e = hipStreamSynchronize(s);
     hipFree(d_c);
     if (e != hipSuccess) return fail(SFH_ERR_HIP, "wave grid: %s", hipGetErrorString(e));
+    return SFH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// surrogate glue: feature build, out-of-distribution mask, decode and audit
+// ---------------------------------------------------------------------------
+
+struct SurNorm {
+    float in_shift[SFH_SUR_FEATURES], in_scale[SFH_SUR_FEATURES];
+    float in_lo[SFH_SUR_FEATURES], in_hi[SFH_SUR_FEATURES];
+    float out_shift[6], out_scale[6];
+};
+
+__device__ __forceinline__ uint16_t bf16_bits(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+// Total F and T (world frame) of one output row, fields with a clear written
+// bit counted as zero.
+__device__ __forceinline__ void exact_totals(const double* e, double* F, double* T)
+{
+    const uint32_t wm = (uint32_t)e[20];
+    for (int k = 0; k < 3; ++k) { F[k] = 0.0; T[k] = 0.0; }
+    for (int q = 0; q < 3; ++q) {
+        if (wm & (1u << (2 * q)))
+            for (int k = 0; k < 3; ++k) F[k] += e[6 * q + k];
+        if (wm & (1u << (2 * q + 1)))
+            for (int k = 0; k < 3; ++k) T[k] += e[6 * q + 3 + k];
+    }
+}
+
+// Unit quaternion (x, y, z, w) of a rotation matrix (row-major), w >= 0.
+__device__ void rot_to_quat(const double* R, double* q)
+{
+    const double tr = R[0] + R[4] + R[8];
+    double x, y, z, w;
+    if (tr > 0.0) {
+        double s = 2.0 * sqrt(tr + 1.0);
+        w = 0.25 * s; x = (R[7] - R[5]) / s; y = (R[2] - R[6]) / s; z = (R[3] - R[1]) / s;
+    } else if (R[0] > R[4] && R[0] > R[8]) {
+        double s = 2.0 * sqrt(1.0 + R[0] - R[4] - R[8]);
+        w = (R[7] - R[5]) / s; x = 0.25 * s; y = (R[1] + R[3]) / s; z = (R[2] + R[6]) / s;
+    } else if (R[4] > R[8]) {
+        double s = 2.0 * sqrt(1.0 + R[4] - R[0] - R[8]);
+        w = (R[2] - R[6]) / s; x = (R[1] + R[3]) / s; y = 0.25 * s; z = (R[5] + R[7]) / s;
+    } else {
+        double s = 2.0 * sqrt(1.0 + R[8] - R[0] - R[4]);
+        w = (R[3] - R[1]) / s; x = (R[2] + R[6]) / s; y = (R[5] + R[7]) / s; z = 0.25 * s;
+    }
+    const double sg = (w < 0.0) ? -1.0 : 1.0;
+    const double inv = sg / sqrt(x * x + y * y + z * z + w * w);
+    q[0] = x * inv; q[1] = y * inv; q[2] = z * inv; q[3] = w * inv;
+}
+
+__global__ void k_sur_features(const BodyStatic* __restrict__ st, const double* __restrict__ states, uint32_t first,
+                               uint32_t count, uint32_t rows, OceanK oc, SurNorm nm, float* __restrict__ raw,
+                               uint16_t* __restrict__ x, uint8_t* __restrict__ ood,
+                               const double* __restrict__ exact, float* __restrict__ raw_target,
+                               float* __restrict__ target)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    if (i >= count) {
+        if (x) {
+            uint4* row = reinterpret_cast<uint4*>(x + (size_t)i * SFH_SUR_ROW);
+            for (int k = 0; k < SFH_SUR_ROW / 8; ++k) row[k] = make_uint4(0, 0, 0, 0);
+        }
+        return;
+    }
+    const uint32_t b = first + i;
+    const double* s = states + (size_t)b * SFH_STATE_DOUBLES;
+    double R[9], t[3] = {s[12], s[13], s[14]}, vr[3], f[SFH_SUR_FEATURES];
+    gl_basis(s, R);
+    for (int k = 0; k < 3; ++k) vr[k] = s[16 + k] - oc.u[k];
+    mat_t_vec(R, vr, f);
+    mat_t_vec(R, s + 19, f + 3);
+    rot_to_quat(R, f + 6);
+    double cb[3];
+    mat_vec(R, st[b].P_CB, cb);
+    for (int k = 0; k < 3; ++k) cb[k] += t[k];
+    f[10] = cb[2];
+    f[11] = f[12] = 0.0;
+    if (oc.nlayers) {
+        const double d = oc.len[0] / (double)oc.n;
+        f[10] += wave_sum64(oc, cb[0], cb[1]);
+        f[11] = (wave_sum64(oc, cb[0] + d, cb[1]) - wave_sum64(oc, cb[0] - d, cb[1])) / (2.0 * d);
+        f[12] = (wave_sum64(oc, cb[0], cb[1] + d) - wave_sum64(oc, cb[0], cb[1] - d)) / (2.0 * d);
+    }
+    if (raw)
+        for (int k = 0; k < SFH_SUR_FEATURES; ++k) raw[(size_t)i * SFH_SUR_FEATURES + k] = (float)f[k];
+    if (ood) {
+        uint8_t o = 0;
+        for (int k = 0; k < SFH_SUR_FEATURES; ++k) {
+            const float v = (float)f[k];
+            if (!(v >= nm.in_lo[k] && v <= nm.in_hi[k])) o = 1;   // also catches NaN
+        }
+        ood[i] = o;
+    }
+    if (x) {
+        uint32_t w[SFH_SUR_ROW / 2];
+        for (int k = 0; k < SFH_SUR_ROW / 2; ++k) w[k] = 0;
+        for (int k = 0; k < SFH_SUR_FEATURES; ++k) {
+            const uint32_t h = bf16_bits(((float)f[k] - nm.in_shift[k]) * nm.in_scale[k]);
+            w[k >> 1] |= h << (16 * (k & 1));
+        }
+        uint4* row = reinterpret_cast<uint4*>(x + (size_t)i * SFH_SUR_ROW);
+        for (int k = 0; k < SFH_SUR_ROW / 8; ++k) row[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    }
+    if (exact && (raw_target || target)) {
+        double F[3], T[3], y[6];
+        exact_totals(exact + (size_t)b * SFH_OUT_DOUBLES, F, T);
+        mat_t_vec(R, F, y);
+        mat_t_vec(R, T, y + 3);
+        for (int k = 0; k < 6; ++k) {
+            if (raw_target) raw_target[(size_t)i * 6 + k] = (float)y[k];
+            if (target) target[(size_t)i * 6 + k] = (float)(((float)y[k] - nm.out_shift[k]) / nm.out_scale[k]);
+        }
+    }
+}
+
+__global__ void k_sur_apply(const double* __restrict__ states, uint32_t first, uint32_t count, SurNorm nm,
+                            const float* __restrict__ y, const uint8_t* __restrict__ ood,
+                            const double* __restrict__ exact, double* __restrict__ ft, float* __restrict__ err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t b = first + i;
+    double R[9], yb[6], Fs[3], Ts[3];
+    gl_basis(states + (size_t)b * SFH_STATE_DOUBLES, R);
+    for (int k = 0; k < 6; ++k) yb[k] = (double)(y[(size_t)i * 6 + k] * nm.out_scale[k] + nm.out_shift[k]);
+    mat_vec(R, yb, Fs);
+    mat_vec(R, yb + 3, Ts);
+    double Fe[3], Te[3];
+    if (exact) exact_totals(exact + (size_t)b * SFH_OUT_DOUBLES, Fe, Te);
+    const bool fallback = exact && ood && ood[i];
+    double* o = ft + (size_t)i * 6;
+    for (int k = 0; k < 3; ++k) {
+        o[k] = fallback ? Fe[k] : Fs[k];
+        o[3 + k] = fallback ? Te[k] : Ts[k];
+    }
+    if (err && exact) {
+        double dF = 0, nF = 0, dT = 0, nT = 0;
+        for (int k = 0; k < 3; ++k) {
+            dF += (Fs[k] - Fe[k]) * (Fs[k] - Fe[k]);
+            nF += Fe[k] * Fe[k];
+            dT += (Ts[k] - Te[k]) * (Ts[k] - Te[k]);
+            nT += Te[k] * Te[k];
+        }
+        err[2 * (size_t)i] = (float)(dF == 0.0 ? 0.0 : sqrt(dF / nF));
+        err[2 * (size_t)i + 1] = (float)(dT == 0.0 ? 0.0 : sqrt(dT / nT));
+    }
+}
+
+static SurNorm sur_norm(const sfh_surrogate_norm* n)
+{
+    SurNorm k;
+    memset(&k, 0, sizeof(k));
+    if (!n) {
+        for (int i = 0; i < SFH_SUR_FEATURES; ++i) k.in_scale[i] = 1.f;
+        for (int i = 0; i < 6; ++i) k.out_scale[i] = 1.f;
+        return k;
+    }
+    memcpy(k.in_shift, n->in_shift, sizeof(k.in_shift));
+    memcpy(k.in_scale, n->in_scale, sizeof(k.in_scale));
+    memcpy(k.in_lo, n->in_lo, sizeof(k.in_lo));
+    memcpy(k.in_hi, n->in_hi, sizeof(k.in_hi));
+    memcpy(k.out_shift, n->out_shift, sizeof(k.out_shift));
+    memcpy(k.out_scale, n->out_scale, sizeof(k.out_scale));
+    return k;
+}
+
+extern "C" int sfh_surrogate_features(sfh_batch b, sfh_ocean o, const double* d_states, uint32_t first,
+                                      uint32_t count, uint32_t rows, const sfh_surrogate_norm* norm, float* d_raw,
+                                      uint16_t* d_x, uint8_t* d_ood, const double* d_exact, float* d_raw_target,
+                                      float* d_target, void* stream)
+{
+    if (!b || !d_states) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(o);
+    if (rc != SFH_OK) return rc;
+    if ((uint64_t)first + count > b->n) return fail(SFH_ERR_SIZE, "body range out of the batch");
+    if ((d_x || d_ood || d_target) && !norm) return fail(SFH_ERR_INVALID, "normalised outputs need a norm");
+    if ((d_raw_target || d_target) && !d_exact) return fail(SFH_ERR_INVALID, "targets need the exact outputs");
+    if (d_x && (rows < count || rows % 32)) return fail(SFH_ERR_SIZE, "rows must be a multiple of 32 >= count");
+    if (d_x && ((uintptr_t)d_x % 16)) return fail(SFH_ERR_INVALID, "d_x must be 16-byte aligned");
+    const uint32_t r = d_x ? rows : count;
+    if (r == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_sur_features, dim3((r + 127) / 128), dim3(128), 0, (hipStream_t)stream, b->d_static,
+                       d_states, first, count, r, make_oceank(o), sur_norm(norm), d_raw, d_x, d_ood, d_exact,
+                       d_raw_target, d_target);
+    HIPCHK(hipGetLastError());
+    return SFH_OK;
+}
+
+extern "C" int sfh_surrogate_apply(sfh_batch b, const double* d_states, uint32_t first, uint32_t count,
+                                   const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
+                                   const double* d_exact, double* d_ft, float* d_err, void* stream)
+{
+    if (!b || !d_states || !norm || !d_y || !d_ft) return fail(SFH_ERR_INVALID, "null argument");
+    if ((uint64_t)first + count > b->n) return fail(SFH_ERR_SIZE, "body range out of the batch");
+    if (count == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_sur_apply, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_states, first,
+                       count, sur_norm(norm), d_y, d_ood, d_exact, d_ft, d_err);
+    HIPCHK(hipGetLastError());
     return SFH_OK;
 }

@@ -1,0 +1,592 @@
+// sfmlp: hand-written gfx950 BF16 MFMA kernels of the hydrodynamics surrogate
+// (forward, backward, Adam).  No counterpart in the reference.
+//
+// Every matrix product is C = A * Bt^T with A [rows x K] and Bt [cols x K] both
+// row-major and contiguous in K, so each lane's MFMA operands are one 16-byte
+// load: the forward pass keeps transposed copies of its activations (H^T) and
+// of the weights (W^T, refreshed by Adam) for the backward products.
+//
+//   forward   Z = H W^T + b,  H' = gelu(Z)                 (A = H,    Bt = W)
+//   dH        dH = dZ W                                     (A = dZ,   Bt = W^T)
+//   dW        dW = dZ^T H, split over the batch             (A = dZ^T, Bt = H^T)
+//
+// One wavefront computes one 32x32 tile with v_mfma_f32_32x32x16_bf16; the
+// epilogues fuse bias, GELU (exact erf) and its derivative, the loss and its
+// gradient, BF16 rounding and the transposed stores.  Weight-gradient partials
+// of the batch splits are summed in a fixed order by the Adam kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/sfmlp.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SFM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+extern "C" const char* sfm_last_error(void) { return g_err; }
+
+static constexpr int kDims[SFM_LAYERS + 1] = {SFM_IN_PAD, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_OUT_PAD};
+static constexpr int kReal[SFM_LAYERS + 1] = {SFM_IN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_OUT};
+static constexpr int kSplitK = 256;   // batch rows per weight-gradient split
+
+// ---------------------------------------------------------------------------
+// numerics
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                             // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float z)
+{
+    return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tile kernel
+// ---------------------------------------------------------------------------
+
+enum { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_BWD = 2, EPI_WGRAD = 3 };
+
+struct GemmArgs {
+    const uint16_t* A;      // [rows x K], lda
+    const uint16_t* Bt;     // [cols x K], ldb
+    int lda, ldb;
+    int rows, cols, K, ksplit;
+    const float* bias;      // [cols]                      (HIDDEN, OUT)
+    float* Z;               // [rows x cols] FP32          (HIDDEN: written; BWD: read)
+    uint16_t* H;            // [rows x cols] BF16          (HIDDEN, OUT-train, BWD)
+    uint16_t* Ht;           // [cols x rows] BF16          (optional)
+    const float* target;    // [rows x SFM_OUT]            (OUT-train)
+    float* y;               // [rows x SFM_OUT]            (OUT)
+    float* loss;            //                              (OUT-train)
+    float scale;            // 1 / (rows * SFM_OUT)         (OUT-train)
+    float* part;            // [ksplit][rows x cols] FP32  (WGRAD)
+};
+
+// Accumulator register r of lane l of a 32x32 MFMA tile holds row
+// 8 (r / 4) + 4 (l / 32) + r % 4 and column l % 32 of the tile.
+__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tiles_n = g.cols >> 5;
+    const int tm = wave / tiles_n, tn = wave - tm * tiles_n;
+    if (tm * 32 >= g.rows) return;
+    const int split = blockIdx.y;
+    const int kper = g.K / g.ksplit;
+    const int k0 = split * kper;
+    const int kk = (lane >> 5) * 8;
+    const uint16_t* ap = g.A + (size_t)(tm * 32 + (lane & 31)) * g.lda + k0 + kk;
+    const uint16_t* bp = g.Bt + (size_t)(tn * 32 + (lane & 31)) * g.ldb + k0 + kk;
+    f32x16 acc = {};
+    for (int k = 0; k < kper; k += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(ap + k);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(bp + k);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int n = tn * 32 + (lane & 31);
+    const int mb = tm * 32;
+    if (EPI == EPI_WGRAD) {
+        float* out = g.part + (size_t)split * g.rows * g.cols;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[(size_t)(mb + acc_row(r, lane)) * g.cols + n] = acc[r];
+        return;
+    }
+    float sq = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint16_t hv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 4 * q + i;
+            const int m = mb + acc_row(r, lane);
+            const size_t idx = (size_t)m * g.cols + n;
+            float out;
+            if (EPI == EPI_HIDDEN) {
+                const float z = acc[r] + g.bias[n];
+                if (g.Z) g.Z[idx] = z;
+                out = gelu(z);
+            } else if (EPI == EPI_OUT) {
+                const float z = acc[r] + g.bias[n];
+                out = 0.f;
+                if (n < SFM_OUT) {
+                    if (g.y) g.y[(size_t)m * SFM_OUT + n] = z;
+                    if (g.target) {
+                        const float d = z - g.target[(size_t)m * SFM_OUT + n];
+                        sq += d * d;
+                        out = 2.0f * d * g.scale;   // dLoss/dz
+                    }
+                }
+            } else {   // EPI_BWD: dZ = dH * gelu'(Z)
+                out = acc[r] * gelu_grad(g.Z[idx]);
+            }
+            hv[i] = f2bf(out);
+            if (g.H) g.H[idx] = hv[i];
+        }
+        if (g.Ht) {   // four consecutive rows -> one 8-byte store of the transposed tile
+            const int m0 = mb + 8 * q + 4 * (lane >> 5);
+            uint2 pk;
+            pk.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
+            pk.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
+            *reinterpret_cast<uint2*>(g.Ht + (size_t)n * g.rows + m0) = pk;
+        }
+    }
+    if (EPI == EPI_OUT && g.target) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off, 64);
+        if (lane == 0) atomicAdd(g.loss, sq * g.scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+
+// Transposes x [rows x cols] (BF16) into xt [cols x rows].
+__global__ void k_transpose(const uint16_t* __restrict__ x, int rows, int cols, uint16_t* __restrict__ xt)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols) return;
+    int r = (int)(i / cols), c = (int)(i - (size_t)r * cols);
+    xt[(size_t)c * rows + r] = x[i];
+}
+
+// Row sums of xt [n x m] (BF16): one wavefront per row, fixed summation order.
+__global__ __launch_bounds__(64) void k_rowsum(const uint16_t* __restrict__ xt, int m, float* __restrict__ out)
+{
+    const int row = blockIdx.x, lane = threadIdx.x;
+    float s = 0.f;
+    for (int i = lane; i < m; i += 64) s += __uint_as_float((uint32_t)xt[(size_t)row * m + i] << 16);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) out[row] = s;
+}
+
+// Adam on n parameters of a [rows x cols] matrix (cols = 1 for biases): the
+// gradient is the sum of nsplit partials (fixed order); keeps the FP32 master
+// copy and refreshes the BF16 copy and its transpose.
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ mo, float* __restrict__ ve,
+                       const float* __restrict__ part, int nsplit, int rows, int cols, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2, float* __restrict__ gsave, uint16_t* __restrict__ wb,
+                       uint16_t* __restrict__ wbt)
+{
+    const int n = rows * cols;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gr = 0.f;
+    for (int s = 0; s < nsplit; ++s) gr += part[(size_t)s * n + i];
+    gsave[i] = gr;
+    const float m = b1 * mo[i] + (1.f - b1) * gr;
+    const float v = b2 * ve[i] + (1.f - b2) * gr * gr;
+    mo[i] = m;
+    ve[i] = v;
+    const float wn = w[i] - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+    w[i] = wn;
+    if (wb) {
+        const uint16_t h = f2bf(wn);
+        wb[i] = h;
+        const int r = i / cols, c = i - r * cols;
+        wbt[(size_t)c * rows + r] = h;
+    }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// One 32x32x16 product with the operand mapping of k_gemm: a [32 x 16],
+// b [16 x 32] row-major; c [32 x 32] row-major.
+__global__ void k_probe(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b, float* __restrict__ c)
+{
+    const int lane = threadIdx.x;
+    const int kk = (lane >> 5) * 8;
+    bf16x8 av, bv;
+    for (int i = 0; i < 8; ++i) {
+        uint16_t x = a[(lane & 31) * 16 + kk + i];
+        uint16_t y = b[(kk + i) * 32 + (lane & 31)];
+        av[i] = *reinterpret_cast<const __bf16*>(&x);
+        bv[i] = *reinterpret_cast<const __bf16*>(&y);
+    }
+    f32x16 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    for (int r = 0; r < 16; ++r) c[acc_row(r, lane) * 32 + (lane & 31)] = acc[r];
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+
+struct Layer {
+    float *W = nullptr, *b = nullptr, *mW = nullptr, *vW = nullptr, *mb = nullptr, *vb = nullptr;
+    float *gW = nullptr, *gb = nullptr;
+    uint16_t *Wb = nullptr, *Wbt = nullptr;
+};
+
+struct sfm_net_t {
+    uint32_t maxb = 0;
+    Layer L[SFM_LAYERS];
+    uint16_t* H[SFM_LAYERS] = {};    // H[l] = input of layer l (H[0] unused: the caller's x)
+    uint16_t* Ht[SFM_LAYERS] = {};   // transposed inputs
+    float* Z[SFM_LAYERS] = {};       // pre-activations of layers 0..3 outputs (Z[l] = output of layer l)
+    uint16_t *dZ[2] = {}, *dZt[2] = {};
+    float* part = nullptr;
+    float* dbias = nullptr;
+    uint64_t step = 0;
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending = false;
+    double pending_flops = 0, ms_sum = 0, flops_sum = 0;
+    uint32_t timed = 0;
+};
+
+static uint64_t splitmix(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static void net_free(sfm_net_t* n)
+{
+    for (Layer& l : n->L) {
+        for (float* p : {l.W, l.b, l.mW, l.vW, l.mb, l.vb, l.gW, l.gb}) hipFree(p);
+        hipFree(l.Wb);
+        hipFree(l.Wbt);
+    }
+    for (int i = 0; i < SFM_LAYERS; ++i) { hipFree(n->H[i]); hipFree(n->Ht[i]); hipFree(n->Z[i]); }
+    for (int i = 0; i < 2; ++i) { hipFree(n->dZ[i]); hipFree(n->dZt[i]); }
+    hipFree(n->part);
+    hipFree(n->dbias);
+    for (int i = 0; i < 2; ++i)
+        if (n->ev[i]) hipEventDestroy(n->ev[i]);
+}
+
+static int upload_layer(sfm_net_t* n, int l, const float* w, const float* b)
+{
+    const int rows = kDims[l + 1], cols = kDims[l];
+    std::vector<uint16_t> wb((size_t)rows * cols), wbt((size_t)rows * cols);
+    for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) {
+            float f = w[(size_t)r * cols + c];
+            uint32_t u;
+            memcpy(&u, &f, 4);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            uint16_t h = (uint16_t)(u >> 16);
+            wb[(size_t)r * cols + c] = h;
+            wbt[(size_t)c * rows + r] = h;
+        }
+    Layer& L = n->L[l];
+    HIPCHK(hipMemcpy(L.W, w, sizeof(float) * rows * cols, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.b, b, sizeof(float) * rows, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.Wb, wb.data(), 2 * wb.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.Wbt, wbt.data(), 2 * wbt.size(), hipMemcpyHostToDevice));
+    return SFM_OK;
+}
+
+extern "C" int sfm_create(uint32_t max_batch, uint64_t seed, sfm_net* out)
+{
+    if (!out) return fail(SFM_ERR_INVALID, "null argument");
+    if (max_batch == 0 || max_batch % kSplitK || max_batch > (1u << 22))
+        return fail(SFM_ERR_SIZE, "max_batch must be a positive multiple of %d", kSplitK);
+    sfm_net_t* n = new sfm_net_t();
+    n->maxb = max_batch;
+    auto alloc = [&](void** p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes) != hipSuccess) return false;
+        return hipMemset(*p, 0, bytes) == hipSuccess;
+    };
+    bool ok = true;
+    for (int l = 0; l < SFM_LAYERS && ok; ++l) {
+        const size_t nw = (size_t)kDims[l + 1] * kDims[l], nb = kDims[l + 1];
+        Layer& L = n->L[l];
+        ok = alloc((void**)&L.W, 4 * nw) && alloc((void**)&L.b, 4 * nb) && alloc((void**)&L.mW, 4 * nw) &&
+             alloc((void**)&L.vW, 4 * nw) && alloc((void**)&L.mb, 4 * nb) && alloc((void**)&L.vb, 4 * nb) &&
+             alloc((void**)&L.gW, 4 * nw) && alloc((void**)&L.gb, 4 * nb) && alloc((void**)&L.Wb, 2 * nw) &&
+             alloc((void**)&L.Wbt, 2 * nw);
+        if (ok && l > 0) ok = alloc((void**)&n->H[l], 2 * (size_t)max_batch * kDims[l]) &&
+                               alloc((void**)&n->Z[l], 4 * (size_t)max_batch * kDims[l]);
+        if (ok) ok = alloc((void**)&n->Ht[l], 2 * (size_t)max_batch * kDims[l]);
+    }
+    for (int i = 0; i < 2 && ok; ++i)
+        ok = alloc((void**)&n->dZ[i], 2 * (size_t)max_batch * SFM_HIDDEN) &&
+             alloc((void**)&n->dZt[i], 2 * (size_t)max_batch * SFM_HIDDEN);
+    if (ok) ok = alloc((void**)&n->part, 4 * (size_t)(max_batch / kSplitK) * SFM_HIDDEN * SFM_HIDDEN) &&
+                 alloc((void**)&n->dbias, 4 * SFM_HIDDEN);
+    if (!ok) { net_free(n); delete n; return fail(SFM_ERR_HIP, "allocation failed"); }
+    // Kaiming-uniform weights on the real fan-in, zero biases, padding zero
+    uint64_t s = seed;
+    for (int l = 0; l < SFM_LAYERS; ++l) {
+        const int rows = kDims[l + 1], cols = kDims[l];
+        std::vector<float> w((size_t)rows * cols, 0.f), b(rows, 0.f);
+        const float bound = 1.0f / std::sqrt((float)kReal[l]);
+        for (int r = 0; r < kReal[l + 1]; ++r)
+            for (int c = 0; c < kReal[l]; ++c) {
+                double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+                w[(size_t)r * cols + c] = (float)((2.0 * u - 1.0) * bound);
+            }
+        int rc = upload_layer(n, l, w.data(), b.data());
+        if (rc != SFM_OK) { net_free(n); delete n; return rc; }
+    }
+    *out = n;
+    return SFM_OK;
+}
+
+extern "C" int sfm_destroy(sfm_net n)
+{
+    if (!n) return fail(SFM_ERR_INVALID, "null network");
+    hipDeviceSynchronize();
+    net_free(n);
+    delete n;
+    return SFM_OK;
+}
+
+extern "C" int sfm_param_count(sfm_net n, uint64_t* real_params)
+{
+    if (!n || !real_params) return fail(SFM_ERR_INVALID, "null argument");
+    uint64_t c = 0;
+    for (int l = 0; l < SFM_LAYERS; ++l) c += (uint64_t)kReal[l + 1] * kReal[l] + kReal[l + 1];
+    *real_params = c;
+    return SFM_OK;
+}
+
+extern "C" int sfm_layer_io(sfm_net n, int l, float* w, float* b, int set)
+{
+    if (!n || l < 0 || l >= SFM_LAYERS || !w || !b) return fail(SFM_ERR_INVALID, "bad argument");
+    HIPCHK(hipDeviceSynchronize());
+    if (set) return upload_layer(n, l, w, b);
+    const int rows = kDims[l + 1], cols = kDims[l];
+    HIPCHK(hipMemcpy(w, n->L[l].W, sizeof(float) * rows * cols, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(b, n->L[l].b, sizeof(float) * rows, hipMemcpyDeviceToHost));
+    return SFM_OK;
+}
+
+template <int EPI>
+static void launch_gemm(const GemmArgs& g, hipStream_t s)
+{
+    const int tiles = (g.rows / 32) * (g.cols / 32);
+    hipLaunchKernelGGL(k_gemm<EPI>, dim3((tiles + 3) / 4, g.ksplit), dim3(256), 0, s, g);
+}
+
+static GemmArgs gemm(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, int rows, int cols, int K)
+{
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.A = A; g.lda = lda; g.Bt = Bt; g.ldb = ldb;
+    g.rows = rows; g.cols = cols; g.K = K; g.ksplit = 1;
+    return g;
+}
+
+static double forward_flops(uint32_t batch)
+{
+    double f = 0;
+    for (int l = 0; l < SFM_LAYERS; ++l) f += 2.0 * kReal[l] * kReal[l + 1];
+    return f * batch;
+}
+
+static void time_begin(sfm_net_t* n, hipStream_t s)
+{
+    if (!n->timing) return;
+    if (n->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(n->ev[1]) == hipSuccess && hipEventElapsedTime(&ms, n->ev[0], n->ev[1]) == hipSuccess) {
+            n->ms_sum += ms;
+            n->flops_sum += n->pending_flops;
+            n->timed++;
+        }
+        n->pending = false;
+    }
+    hipEventRecord(n->ev[0], s);
+}
+
+static void time_end(sfm_net_t* n, hipStream_t s, double flops)
+{
+    if (!n->timing) return;
+    hipEventRecord(n->ev[1], s);
+    n->pending = true;
+    n->pending_flops = flops;
+}
+
+// Forward through the hidden layers: x -> H[1..4] (and Z, H^T when training).
+static void forward_hidden(sfm_net_t* n, const uint16_t* x, uint32_t M, bool train, hipStream_t s)
+{
+    for (int l = 0; l < SFM_LAYERS - 1; ++l) {
+        const uint16_t* A = (l == 0) ? x : n->H[l];
+        GemmArgs g = gemm(A, kDims[l], n->L[l].Wb, kDims[l], (int)M, kDims[l + 1], kDims[l]);
+        g.bias = n->L[l].b;
+        g.H = n->H[l + 1];
+        if (train) { g.Z = n->Z[l + 1]; g.Ht = n->Ht[l + 1]; }
+        launch_gemm<EPI_HIDDEN>(g, s);
+    }
+}
+
+extern "C" int sfm_forward(sfm_net n, const uint16_t* x, uint32_t batch, float* y, void* stream)
+{
+    if (!n || !x || !y) return fail(SFM_ERR_INVALID, "null argument");
+    if (batch == 0 || batch % 32 || batch > n->maxb) return fail(SFM_ERR_SIZE, "batch must be a multiple of 32 <= %u", n->maxb);
+    if ((uintptr_t)x % 16) return fail(SFM_ERR_INVALID, "x must be 16-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    time_begin(n, s);
+    forward_hidden(n, x, batch, false, s);
+    GemmArgs g = gemm(n->H[SFM_LAYERS - 1], SFM_HIDDEN, n->L[SFM_LAYERS - 1].Wb, SFM_HIDDEN, (int)batch, SFM_OUT_PAD,
+                      SFM_HIDDEN);
+    g.bias = n->L[SFM_LAYERS - 1].b;
+    g.y = y;
+    launch_gemm<EPI_OUT>(g, s);
+    time_end(n, s, forward_flops(batch));
+    HIPCHK(hipGetLastError());
+    return SFM_OK;
+}
+
+extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target, uint32_t batch, float lr, float beta1,
+                              float beta2, float eps, float* loss, void* stream)
+{
+    if (!n || !x || !target || !loss) return fail(SFM_ERR_INVALID, "null argument");
+    if (batch == 0 || batch % kSplitK || batch > n->maxb)
+        return fail(SFM_ERR_SIZE, "batch must be a multiple of %d <= %u", kSplitK, n->maxb);
+    if ((uintptr_t)x % 16) return fail(SFM_ERR_INVALID, "x must be 16-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    const int M = (int)batch;
+    time_begin(n, s);
+    HIPCHK(hipMemsetAsync(loss, 0, sizeof(float), s));
+    // x^T for the first layer's weight gradient
+    hipLaunchKernelGGL(k_transpose, dim3((M * SFM_IN_PAD + 255) / 256), dim3(256), 0, s, x, M, SFM_IN_PAD, n->Ht[0]);
+    forward_hidden(n, x, batch, true, s);
+    // output layer: loss and dLoss/dZ (row-major and transposed)
+    {
+        GemmArgs g = gemm(n->H[SFM_LAYERS - 1], SFM_HIDDEN, n->L[SFM_LAYERS - 1].Wb, SFM_HIDDEN, M, SFM_OUT_PAD,
+                          SFM_HIDDEN);
+        g.bias = n->L[SFM_LAYERS - 1].b;
+        g.target = target;
+        g.loss = loss;
+        g.scale = 1.0f / ((float)M * SFM_OUT);
+        g.H = n->dZ[0];
+        g.Ht = n->dZt[0];
+        launch_gemm<EPI_OUT>(g, s);
+    }
+    n->step++;
+    const float bc1 = 1.0f - std::pow(beta1, (float)n->step), bc2 = 1.0f - std::pow(beta2, (float)n->step);
+    int cur = 0;
+    for (int l = SFM_LAYERS - 1; l >= 0; --l) {
+        const int dout = kDims[l + 1], din = kDims[l];
+        Layer& L = n->L[l];
+        // dW = dZ^T H, split over the batch
+        {
+            const int splits = M / kSplitK;
+            GemmArgs g = gemm(n->dZt[cur], M, n->Ht[l], M, dout, din, M);
+            g.ksplit = splits;
+            g.part = n->part;
+            launch_gemm<EPI_WGRAD>(g, s);
+            hipLaunchKernelGGL(k_rowsum, dim3(dout), dim3(64), 0, s, n->dZt[cur], M, n->dbias);
+            // dH = dZ W  (with the pre-update weights), then dZ of the layer below
+            if (l > 0) {
+                GemmArgs h = gemm(n->dZ[cur], dout, L.Wbt, dout, M, din, dout);
+                h.Z = n->Z[l];
+                h.H = n->dZ[cur ^ 1];
+                h.Ht = n->dZt[cur ^ 1];
+                launch_gemm<EPI_BWD>(h, s);
+            }
+            const int nw = dout * din;
+            hipLaunchKernelGGL(k_adam, dim3((nw + 255) / 256), dim3(256), 0, s, L.W, L.mW, L.vW, n->part, splits, dout,
+                               din, lr, beta1, beta2, eps, bc1, bc2, L.gW, L.Wb, L.Wbt);
+            hipLaunchKernelGGL(k_adam, dim3((dout + 255) / 256), dim3(256), 0, s, L.b, L.mb, L.vb, n->dbias, 1, dout, 1,
+                               lr, beta1, beta2, eps, bc1, bc2, L.gb, (uint16_t*)nullptr, (uint16_t*)nullptr);
+        }
+        cur ^= 1;
+    }
+    time_end(n, s, 3.0 * forward_flops(batch));
+    HIPCHK(hipGetLastError());
+    return SFM_OK;
+}
+
+extern "C" int sfm_gradients(sfm_net n, int l, float* dw, float* db, void* stream)
+{
+    if (!n || l < 0 || l >= SFM_LAYERS || !dw || !db) return fail(SFM_ERR_INVALID, "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipMemcpyAsync(dw, n->L[l].gW, sizeof(float) * kDims[l + 1] * kDims[l], hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(db, n->L[l].gb, sizeof(float) * kDims[l + 1], hipMemcpyDeviceToDevice, s));
+    return SFM_OK;
+}
+
+extern "C" int sfm_to_bf16(const float* x, uint16_t* y, uint64_t n, void* stream)
+{
+    if ((!x || !y) && n) return fail(SFM_ERR_INVALID, "null argument");
+    if (n == 0) return SFM_OK;
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x, y, n);
+    HIPCHK(hipGetLastError());
+    return SFM_OK;
+}
+
+extern "C" int sfm_probe_mfma(const uint16_t* a, const uint16_t* b, float* c, void* stream)
+{
+    if (!a || !b || !c) return fail(SFM_ERR_INVALID, "null argument");
+    hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, c);
+    HIPCHK(hipGetLastError());
+    return SFM_OK;
+}
+
+extern "C" int sfm_timing(sfm_net n, int enable)
+{
+    if (!n) return fail(SFM_ERR_INVALID, "null network");
+    if (enable && !n->ev[0])
+        for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&n->ev[i]));
+    n->timing = enable != 0;
+    n->pending = false;
+    n->ms_sum = n->flops_sum = 0;
+    n->timed = 0;
+    return SFM_OK;
+}
+
+extern "C" int sfm_kernel_ms(sfm_net n, double* ms, double* flops, uint32_t* count)
+{
+    if (!n) return fail(SFM_ERR_INVALID, "null network");
+    if (n->pending) {
+        float t = 0.f;
+        if (hipEventSynchronize(n->ev[1]) == hipSuccess && hipEventElapsedTime(&t, n->ev[0], n->ev[1]) == hipSuccess) {
+            n->ms_sum += t;
+            n->flops_sum += n->pending_flops;
+            n->timed++;
+        }
+        n->pending = false;
+    }
+    if (count) *count = n->timed;
+    if (ms) *ms = n->timed ? n->ms_sum / n->timed : 0.0;
+    if (flops) *flops = n->timed ? n->flops_sum / n->timed : 0.0;
+    return SFM_OK;
+}

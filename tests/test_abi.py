@@ -1,5 +1,6 @@
-"""The C-ABI library loads without a GPU and exports every entry point that
-include/sfhydro.h declares (and the ctypes binding covers them all)."""
+"""The C-ABI libraries load without a GPU and export every entry point that
+include/sfhydro.h and include/sfmlp.h declare (and the ctypes bindings cover
+them all)."""
 import ctypes as C
 import re
 from pathlib import Path
@@ -9,10 +10,10 @@ from stonefish_amd import _native as N
 ROOT = Path(__file__).resolve().parents[1]
 
 
-def declared(header: str):
+def declared(header: str, prefix: str = "sfh"):
     text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(sfh_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -34,3 +35,20 @@ def test_last_error_without_device():
     rc = lib.sfh_mesh_create(None, 12, 3, None, 1, C.byref(C.c_void_p()))
     assert rc == N.SFH_ERR_INVALID
     assert b"null" in lib.sfh_last_error()
+
+
+def test_mlp_library_exports_every_declared_symbol():
+    lib = C.CDLL(str(N.mlp_lib_path()))
+    names = declared("sfmlp.h", "sfm")
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(N.MLP_EXPORTED)
+
+
+def test_mlp_argument_checks_without_device():
+    lib = N.load_mlp()
+    rc = lib.sfm_create(100, 1, C.byref(C.c_void_p()))   # not a multiple of 256
+    assert rc == -2
+    assert b"multiple" in lib.sfm_last_error()
+    assert lib.sfm_destroy(None) == -1

@@ -1,0 +1,310 @@
+"""GPU tests of the surrogate: the MFMA layout, the forward / backward / Adam
+kernels against the FP32 reference (tests/mlp_ref.py), the feature and decode
+kernels against the oracle, the masked exact fallback, and a training run
+that must beat the untrained network on held-out samples."""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(max_batch=1024, seed=0):
+    from stonefish_amd import surrogate as S
+    return S.Net(max_batch, seed)
+
+
+def _load(net, Ws, bs):
+    for l in range(5):
+        net.set_layer(l, Ws[l].numpy(), bs[l].numpy())
+
+
+def test_mfma_operand_layout(gpu):
+    from stonefish_amd import surrogate as S
+    g = torch.Generator().manual_seed(1)
+    a = torch.randn(32, 16, generator=g).to(torch.bfloat16)
+    b = torch.randn(16, 32, generator=g).to(torch.bfloat16)
+    c = torch.zeros(32, 32, dtype=torch.float32, device="cuda")
+    da, db = a.cuda(), b.cuda()
+    S.probe_mfma(da, db, c)
+    ref = a.float() @ b.float()
+    assert torch.allclose(c.cpu(), ref, rtol=1e-5, atol=1e-5)
+
+
+def test_forward_matches_reference(gpu):
+    net = _net(1024)
+    Ws, bs = R.init_weights(11)
+    _load(net, Ws, bs)
+    x, _ = R.random_inputs(12, 992)   # a multiple of 32, not of 256
+    y = torch.zeros(992, 6, dtype=torch.float32, device="cuda")
+    net.forward(x.to(torch.bfloat16).cuda(), y)
+    torch.cuda.synchronize()
+    yr, _ = R.forward(Ws, bs, x, rounded=True)
+    y0, _ = R.forward(Ws, bs, x, rounded=False)
+    yc = y.cpu()
+    scale = yr[:, :6].abs().max()
+    # same BF16 rounding points: only FP32 summation order differs
+    assert (yc - yr[:, :6]).abs().max() <= 3e-3 * scale
+    # and the rounding itself stays small against the unrounded network
+    assert (yc - y0[:, :6]).norm() <= 2e-2 * y0[:, :6].norm()
+    net.close()
+
+
+def test_gradients_match_reference(gpu):
+    net = _net(1024)
+    Ws, bs = R.init_weights(21)
+    _load(net, Ws, bs)
+    x, t = R.random_inputs(22, 512)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    net.train_step(x.to(torch.bfloat16).cuda(), t.cuda().contiguous(), loss, lr=0.0)
+    torch.cuda.synchronize()
+    y, cache = R.forward(Ws, bs, x, rounded=True)
+    lref, dy = R.loss_and_grad_out(y, t)
+    dWs, dbs = R.backward(Ws, cache, dy, rounded=True)
+    y0, cache0 = R.forward(Ws, bs, x, rounded=False)
+    _, dy0 = R.loss_and_grad_out(y0, t)
+    dW0, db0 = R.backward(Ws, cache0, dy0, rounded=False)
+    assert abs(float(loss.item()) - float(lref)) <= 1e-3 * float(lref)
+    for l in range(5):
+        gw, gb = net.gradients(l)
+        gw, gb = gw.cpu(), gb.cpu()
+        rw = float((gw - dWs[l]).norm() / dWs[l].norm())
+        rb = float((gb - dbs[l]).norm() / dbs[l].norm())
+        assert rw < 1e-2 and rb < 1e-2, (l, rw, rb)
+        rw0 = float((gw - dW0[l]).norm() / dW0[l].norm())
+        assert rw0 < 6e-2, (l, rw0)
+    # padding stays out of the gradients
+    gw0, _ = net.gradients(0)
+    assert torch.all(gw0[:, 13:] == 0)
+    gw4, gb4 = net.gradients(4)
+    assert torch.all(gw4[6:] == 0) and torch.all(gb4[6:] == 0)
+    net.close()
+
+
+def test_adam_matches_torch(gpu):
+    net = _net(512)
+    Ws, bs = R.init_weights(31)
+    _load(net, Ws, bs)
+    params = [p.clone().requires_grad_() for pair in zip(Ws, bs) for p in pair]
+    opt = torch.optim.Adam(params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    for step in range(3):
+        x, t = R.random_inputs(40 + step, 256)
+        net.train_step(x.to(torch.bfloat16).cuda(), t.cuda().contiguous(), loss, lr=1e-3)
+        torch.cuda.synchronize()
+        for l in range(5):
+            gw, gb = net.gradients(l)
+            params[2 * l].grad = gw.cpu()
+            params[2 * l + 1].grad = gb.cpu()
+        opt.step()
+        for l in range(5):
+            w, b = net.layer(l)
+            assert np.allclose(w, params[2 * l].detach().numpy(), rtol=1e-5, atol=2e-7), (step, l)
+            assert np.allclose(b, params[2 * l + 1].detach().numpy(), rtol=1e-5, atol=2e-7), (step, l)
+    w0, _ = net.layer(0)
+    w4, b4 = net.layer(4)
+    assert np.all(w0[:, 13:] == 0) and np.all(w4[6:] == 0) and np.all(b4[6:] == 0)
+    net.close()
+
+
+def test_training_fits_a_smooth_map(gpu):
+    from stonefish_amd import surrogate as S
+    net = _net(4096, seed=5)
+    g = torch.Generator().manual_seed(9)
+    n = 16384
+    xin = torch.rand(n, 13, generator=g) * 2 - 1
+    tgt = torch.stack([torch.sin(2 * xin[:, 0]) + xin[:, 1] * xin[:, 2], xin[:, 3] ** 2 - 0.3,
+                       torch.cos(xin[:, 4] + xin[:, 5]), xin[:, 6] * xin[:, 7] * xin[:, 8],
+                       torch.tanh(3 * xin[:, 9]), xin[:, 10] - xin[:, 11] + xin[:, 12]], 1)
+    x = torch.zeros(n, 32)
+    x[:, :13] = xin
+    dx = x.to(torch.bfloat16).cuda()
+    dt = tgt.cuda().contiguous()
+    y = S.predict(net, dx)
+    before = float(((y - dt) ** 2).mean())
+    S.train(net, dx, dt, S.TrainConfig(steps=600, batch=1024, lr=2e-3, warmup=50))
+    y = S.predict(net, dx)
+    after = float(((y - dt) ** 2).mean())
+    assert after < 0.02 * before and after < 5e-3, (before, after)
+    net.close()
+
+
+# ---- feature build, decode and masked fallback (libsfhydro) ------------------------
+
+def _small_fleet(waves=True, n_hulls=2, per_hull=256):
+    from stonefish_amd import hydro as H
+    from stonefish_amd import meshes as M
+    from stonefish_amd import waves as W
+    from stonefish_amd import workloads as WL
+    hulls = [M.ellipsoid_with_faces(1.0, 0.4, 0.3, 800), M.box_with_faces(0.6, 0.5, 0.3, 600)][:n_hulls]
+    meshes = [H.Mesh(m) for m in hulls]
+    descs = []
+    for h, (m, dm) in enumerate(zip(hulls, meshes)):
+        bd = WL.body_from_mesh(m, np.ptp(m.verts, axis=0) / 2)
+        for _ in range(per_hull):
+            descs.append(H.BodyDesc(mesh=dm, Cd=bd.Cd, Cf=bd.Cf, volume=bd.volume, surface=bd.surface,
+                                    P_CB=np.array([0.05, -0.02, 0.01])))
+    ocean = H.Ocean()
+    grid = None
+    comps = None
+    if waves:
+        comps = W.pm_components(16, 12.0, 256.0, seed=3)
+        grid = torch.empty(256 * 256, dtype=torch.float32, device="cuda")
+        W.build_grid_device(comps, 256, grid.data_ptr())
+        torch.cuda.synchronize()
+        ocean.set_waves(grid.cpu().numpy(), 256, 1, [256.0])
+    ocean.add_uniform_current([0.3, -0.2, 0.05])
+    ocean.enable_currents(True)
+    batch = H.BodyBatch(descs)
+    return batch, ocean, hulls, grid
+
+
+def test_features_match_oracle(gpu):
+    import oracle as O
+    from stonefish_amd import surrogate as S
+    batch, ocean, hulls, grid = _small_fleet()
+    rng = np.random.default_rng(4)
+    states = S.sample_states(rng, batch.n, xy_range=(0.0, 256.0))
+    ds = torch.from_numpy(states).cuda()
+    exact = torch.zeros(batch.n, 24, dtype=torch.float64, device="cuda")
+    batch.compute(ocean, ds.data_ptr(), exact.data_ptr())
+    raw = torch.zeros(batch.n, 13, dtype=torch.float32, device="cuda")
+    rt = torch.zeros(batch.n, 6, dtype=torch.float32, device="cuda")
+    S.features(batch, ocean, ds, 0, batch.n, raw=raw, exact=exact, raw_target=rt)
+    torch.cuda.synchronize()
+    raw, rt, ex = raw.cpu().numpy(), rt.cpu().numpy(), exact.cpu().numpy()
+    g = grid.cpu().numpy()
+    ocn = O.Ocean(wave_data=g, wave_n=256, channels=1, lengths=[256.0])
+    lib = O.load()
+    u = np.array([0.3, -0.2, 0.05])
+    for i in range(0, batch.n, 7):
+        s = states[i]
+        Rm = np.array(s[:16]).reshape(4, 4).T[:3, :3]   # OpenGL column-major -> rows
+        t = s[12:15]
+        exp = np.zeros(13)
+        exp[0:3] = Rm.T @ (s[16:19] - u)
+        exp[3:6] = Rm.T @ s[19:22]
+        q = _quat(Rm)
+        exp[6:10] = q
+        cb = Rm @ np.array([0.05, -0.02, 0.01]) + t
+        exp[10] = lib.orc_depth_64(C_ocean(ocn), O._d(cb))
+        d = 1.0
+        hx = lambda x, y: -lib.orc_wave_height_64(C_ocean(ocn), x, y)   # noqa: E731
+        exp[11] = (hx(cb[0] + d, cb[1]) - hx(cb[0] - d, cb[1])) / (2 * d)
+        exp[12] = (hx(cb[0], cb[1] + d) - hx(cb[0], cb[1] - d)) / (2 * d)
+        assert np.allclose(raw[i], exp, rtol=2e-6, atol=2e-6), (i, raw[i], exp)
+        # targets: CG-frame totals of the exact outputs (written fields only)
+        w = int(ex[i, 20])
+        F = sum(ex[i, 6 * k:6 * k + 3] for k in range(3) if w & (1 << (2 * k)))
+        Tq = sum(ex[i, 6 * k + 3:6 * k + 6] for k in range(3) if w & (1 << (2 * k + 1)))
+        F = np.zeros(3) + F
+        Tq = np.zeros(3) + Tq
+        et = np.concatenate([Rm.T @ F, Rm.T @ Tq])
+        assert np.allclose(rt[i], et, rtol=2e-6, atol=1e-6 * (1 + np.abs(et).max())), i
+
+
+def C_ocean(o):
+    import ctypes as C
+    return C.byref(o.s)
+
+
+def _quat(Rm):
+    # reference quaternion from the rotation matrix via its eigenvector (w >= 0)
+    K = np.array([
+        [Rm[0, 0] - Rm[1, 1] - Rm[2, 2], Rm[1, 0] + Rm[0, 1], Rm[2, 0] + Rm[0, 2], Rm[2, 1] - Rm[1, 2]],
+        [Rm[1, 0] + Rm[0, 1], Rm[1, 1] - Rm[0, 0] - Rm[2, 2], Rm[2, 1] + Rm[1, 2], Rm[0, 2] - Rm[2, 0]],
+        [Rm[2, 0] + Rm[0, 2], Rm[2, 1] + Rm[1, 2], Rm[2, 2] - Rm[0, 0] - Rm[1, 1], Rm[1, 0] - Rm[0, 1]],
+        [Rm[2, 1] - Rm[1, 2], Rm[0, 2] - Rm[2, 0], Rm[1, 0] - Rm[0, 1], Rm[0, 0] + Rm[1, 1] + Rm[2, 2]]]) / 3.0
+    w, v = np.linalg.eigh(K)
+    q = v[:, np.argmax(w)]   # (x, y, z, w)
+    return q if q[3] >= 0 else -q
+
+
+def test_normalised_inputs_and_ood(gpu):
+    from stonefish_amd import surrogate as S
+    batch, ocean, hulls, grid = _small_fleet(waves=False, n_hulls=1)
+    rng = np.random.default_rng(8)
+    states = S.sample_states(rng, batch.n, xy_range=(0.0, 10.0))
+    states[3, 16] = 50.0          # far outside any training range
+    states[5, 14] = float("nan")  # non-finite state
+    ds = torch.from_numpy(states).cuda()
+    raw = torch.zeros(batch.n, 13, dtype=torch.float32, device="cuda")
+    S.features(batch, ocean, ds, 0, batch.n, raw=raw)
+    r = raw.cpu().numpy()
+    ok = np.isfinite(r).all(1) & (np.abs(r[:, 0:3]).max(1) < 10)
+    norm = S.Norm.fit(r[ok], np.ones((ok.sum(), 6)))
+    rows = batch.n + 32
+    x = torch.full((rows, 32), 7.0, dtype=torch.bfloat16, device="cuda")
+    ood = torch.zeros(batch.n, dtype=torch.uint8, device="cuda")
+    S.features(batch, ocean, ds, 0, batch.n, norm=norm, rows=rows, x=x, ood=ood)
+    torch.cuda.synchronize()
+    xo = x.float().cpu().numpy()
+    o = ood.cpu().numpy()
+    assert o[3] == 1 and o[5] == 1 and o[ok].sum() == 0
+    expect = torch.from_numpy(((r[ok] - norm.in_shift) * norm.in_scale).astype(np.float32)).to(torch.bfloat16).float()
+    assert np.array_equal(xo[ok][:, :13], expect.numpy())
+    assert np.all(xo[:, 13:] == 0) and np.all(xo[batch.n:] == 0)
+
+
+def test_masked_compute_and_apply(gpu):
+    from stonefish_amd import _native as N
+    from stonefish_amd import surrogate as S
+    batch, ocean, hulls, grid = _small_fleet()
+    rng = np.random.default_rng(12)
+    states = S.sample_states(rng, batch.n, xy_range=(0.0, 256.0))
+    ds = torch.from_numpy(states).cuda()
+    full = torch.zeros(batch.n, 24, dtype=torch.float64, device="cuda")
+    batch.compute(ocean, ds.data_ptr(), full.data_ptr())
+    mask = torch.from_numpy((rng.random(batch.n) < 0.3).astype(np.uint8)).cuda()
+    part = torch.full((batch.n, 24), 123.0, dtype=torch.float64, device="cuda")
+    import ctypes as C
+    N.check(N.load().sfh_batch_compute_masked(batch.handle, ocean.handle, C.c_void_p(ds.data_ptr()),
+                                              C.c_void_p(mask.data_ptr()), C.c_void_p(part.data_ptr()),
+                                              C.c_void_p(0)))
+    torch.cuda.synchronize()
+    m = mask.cpu().numpy().astype(bool)
+    f, p = full.cpu().numpy(), part.cpu().numpy()
+    assert np.array_equal(p[m], f[m])          # deterministic: identical rows
+    assert np.all(p[~m] == 123.0)              # inactive rows untouched
+    # decode: feed the exact normalised targets as network outputs
+    raw = torch.zeros(batch.n, 13, dtype=torch.float32, device="cuda")
+    rt = torch.zeros(batch.n, 6, dtype=torch.float32, device="cuda")
+    S.features(batch, ocean, ds, 0, batch.n, raw=raw, exact=full, raw_target=rt)
+    norm = S.Norm.fit(raw.cpu().numpy(), rt.cpu().numpy())
+    y = torch.zeros(batch.n, 6, dtype=torch.float32, device="cuda")
+    S.features(batch, ocean, ds, 0, batch.n, norm=norm, exact=full, target=y)
+    ft = torch.zeros(batch.n, 6, dtype=torch.float64, device="cuda")
+    err = torch.zeros(batch.n, 2, dtype=torch.float32, device="cuda")
+    ood = mask
+    S.apply(batch, ds, 0, batch.n, norm, y, ft, ood=ood, exact=full, err=err)
+    torch.cuda.synchronize()
+    e = err.cpu().numpy()
+    fo = ft.cpu().numpy()
+    w = f[:, 20].astype(np.int64)
+    Fe = sum(np.where(((w >> (2 * k)) & 1)[:, None] == 1, f[:, 6 * k:6 * k + 3], 0.0) for k in range(3))
+    Te = sum(np.where(((w >> (2 * k + 1)) & 1)[:, None] == 1, f[:, 6 * k + 3:6 * k + 6], 0.0) for k in range(3))
+    assert np.array_equal(fo[m, :3], Fe[m]) and np.array_equal(fo[m, 3:], Te[m])   # fallback rows: exact
+    nF = np.linalg.norm(Fe, axis=1)
+    big = nF > 0.05 * nF.max()
+    assert np.allclose(fo[~m & big, :3], Fe[~m & big], rtol=1e-5, atol=1e-5 * nF.max())
+    assert np.all(e[big, 0] < 1e-5)
+
+
+def test_surrogate_beats_untrained_on_held_out(gpu):
+    from stonefish_amd import surrogate as S
+    batch, ocean, hulls, grid = _small_fleet(waves=True, n_hulls=1, per_hull=2048)
+    ds = S.HullDataset(batch, ocean, 1, 2048, steps=10, holdout_steps=2, seed=1, xy_range=(0.0, 256.0),
+                       depth_range=(-0.5, 1.0))
+    norm = ds.fit_norm(0)
+    x, t, ood = ds.inputs(0, norm, range(ds.train_steps))
+    net = S.Net(4096, seed=3)
+    untrained = S.evaluate(ds, 0, net, norm)
+    S.train(net, x, t, S.TrainConfig(steps=800, batch=1024, lr=2e-3, warmup=50))
+    trained = S.evaluate(ds, 0, net, norm)
+    assert trained["n"] > 0.9 * 2 * 2048
+    assert trained["F_median"] < 0.1 * untrained["F_median"], (trained, untrained)
+    assert trained["F_median"] < 0.05, trained
+    net.close()
