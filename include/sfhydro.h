@@ -153,7 +153,40 @@ typedef struct {
 int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const sfh_batch_opts* opts, sfh_batch* out);
 int sfh_batch_destroy(sfh_batch batch);
 int sfh_batch_set_opts(sfh_batch batch, const sfh_batch_opts* opts);
+/* nbodies: rows of the state and output arrays (bodies, or compounds);
+ * nchunks, nfaces: the face passes' work (over the parts of compounds). */
 int sfh_batch_info(sfh_batch batch, uint32_t* nbodies, uint32_t* nchunks, uint64_t* nfaces);
+
+/* ---- compounds ----------------------------------------------------------- */
+
+/* One part of a compound (Compound.h CompoundPart + the part solid). */
+typedef struct {
+    sfh_mesh mesh;            /* part solid getPhysicsMesh() */
+    double origin[16];        /* CompoundPart::origin (compound O frame -> part O frame) */
+    double O2C[16];           /* part solid getO2CTransform() */
+    double Cd[3], Cf[3];      /* part solid getHydrodynamicCoefficients() */
+    int32_t external;         /* CompoundPart::isExternal */
+    int32_t buoyant;          /* part solid isBuoyant() */
+    int32_t hydro;            /* part solid mode is FLOATING or SUBMERGED */
+} sfh_part_desc;
+
+/* A compound body: `body` carries the compound's T_CG2O, CG-frame AABB,
+ * volume, surface, P_CB, buoyant and hydro (its mesh, T_CG2C, Cd, Cf are not
+ * used); parts[0 .. nparts-1] in the compound's order. */
+typedef struct {
+    sfh_body_desc body;
+    const sfh_part_desc* parts;
+    uint32_t nparts;
+} sfh_compound_desc;
+
+/* Replaces Compound::ComputeHydrodynamicForces (Compound.cpp:291-437) for a
+ * batch of compounds: every part is a body of the face passes (classified
+ * with its compound's AABB, T_C = T_CG * T_CG2O * origin * O2C, corrected
+ * with its own Cd, Cf and T_O * origin), then one lane per compound composes
+ * the part results in part order.  The batch's states and outputs are per
+ * compound (n rows); every batch entry point applies. */
+int sfh_compound_batch_create(const sfh_compound_desc* comps, uint32_t n, const sfh_batch_opts* opts,
+                              sfh_batch* out);
 
 /* Evaluates every body (classification, per-face loop, FP64 reduction,
  * buoyancy, CorrectHydrodynamicForces).  d_states: n x SFH_STATE_DOUBLES,
@@ -251,6 +284,24 @@ int sfh_surrogate_features(sfh_batch batch, sfh_ocean ocean, const double* d_sta
 int sfh_surrogate_apply(sfh_batch batch, const double* d_states, uint32_t first, uint32_t count,
                         const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
                         const double* d_exact, double* d_ft, float* d_err, void* stream);
+
+/* ---- rollout harness (no counterpart in the hydrodynamics path) ------------ */
+
+/* Rigid-body data of identical bodies: mass and principal moments of inertia
+ * about the CG frame's axes. */
+typedef struct {
+    double mass;
+    double inertia[3];
+} sfh_rigid;
+
+/* Advances count body states (d_states, count x SFH_STATE_DOUBLES, device) by
+ * dt under their world-frame total force and torque d_ft (count x 6) plus
+ * gravity (the ocean's): semi-implicit Euler of the Newton-Euler equations,
+ * orientation by the exact rotation of omega dt.  The reference integrates
+ * bodies with Bullet (btRigidBody), outside this path; this step drives the
+ * mixed exact/surrogate rollout. */
+int sfh_rollout_integrate(sfh_ocean ocean, double* d_states, uint32_t count, const double* d_ft,
+                          const sfh_rigid* rigid, double dt, void* stream);
 
 #ifdef __cplusplus
 }

@@ -58,9 +58,23 @@ int sfm_param_count(sfm_net net, uint64_t* real_params);
  * copies), 0 copies out. */
 int sfm_layer_io(sfm_net net, int layer, float* weights, float* bias, int set);
 
+#define SFM_MAX_GROUP 16   /* networks per sfm_forward_group call */
+
 /* Inference: x (batch x 32, BF16 bits, row-major, 16-byte aligned, inputs
- * 13..31 zero), y (batch x 6, FP32); batch a multiple of 32. */
+ * 13..31 zero), y (batch x 6, FP32); batch a multiple of 32 (any size: the
+ * fused kernel keeps no per-batch state).  One launch: each workgroup runs a
+ * tile of rows through all five layers with the activations in LDS. */
 int sfm_forward(sfm_net net, const uint16_t* x, uint32_t batch, float* y, void* stream);
+
+/* Inference of several networks in one launch: x holds counts[0] rows for
+ * nets[0], then counts[1] rows for nets[1], ... (each count a multiple of 32,
+ * nnets <= SFM_MAX_GROUP); y likewise. */
+int sfm_forward_group(const sfm_net* nets, uint32_t nnets, const uint32_t* counts, const uint16_t* x, float* y,
+                      void* stream);
+
+/* Inference through the per-layer kernels of the training path (batch <=
+ * max_batch); same result as sfm_forward up to FP32 summation order. */
+int sfm_forward_layers(sfm_net net, const uint16_t* x, uint32_t batch, float* y, void* stream);
 
 /* One training step on a batch (a multiple of 256): forward, mean-squared-
  * error loss against target (batch x 6, FP32), backward, Adam (lr, beta1,

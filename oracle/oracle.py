@@ -249,3 +249,20 @@ def correct(ocn: Ocean, Fdq, Tdq, Fdf, Tdf, Cd, Cf, T_O):
     a, b, c, d = _d(Fdq), _d(Tdq), _d(Fdf), _d(Tdf)
     lib.orc_correct(C.byref(ocn.s), a, b, c, d, _d(Cd), _d(Cf), _d(T_O))
     return np.array(list(a)), np.array(list(b)), np.array(list(c)), np.array(list(d))
+
+
+def compound_forces(comp: orc_body, parts: list, ocn: Ocean, T_CG, v, w, init=None, full: bool = False):
+    """Compound::ComputeHydrodynamicForces (FP64).  parts: list of
+    (orc_body solid, origin[16], O2C[16], external)."""
+    lib = load()
+    arr = (orc_part * max(1, len(parts)))()
+    keep = []
+    for i, (solid, origin, o2c, external) in enumerate(parts):
+        arr[i].solid = C.pointer(solid)
+        arr[i].origin[:] = list(np.asarray(origin, dtype=np.float64).reshape(16))
+        arr[i].O2C[:] = list(np.asarray(o2c, dtype=np.float64).reshape(16))
+        arr[i].external = int(external)
+        keep.append(solid)
+    r = result(init)
+    lib.orc_compound_forces_64(C.byref(comp), arr, len(parts), C.byref(ocn.s), _d(T_CG), _d(v), _d(w), C.byref(r))
+    return (r.vec(), r.extras()) if full else r.vec()

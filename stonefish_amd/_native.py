@@ -46,6 +46,23 @@ class sfh_body_desc(C.Structure):
     ]
 
 
+class sfh_part_desc(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_void_p),
+        ("origin", C.c_double * 16),
+        ("O2C", C.c_double * 16),
+        ("Cd", C.c_double * 3),
+        ("Cf", C.c_double * 3),
+        ("external", C.c_int32),
+        ("buoyant", C.c_int32),
+        ("hydro", C.c_int32),
+    ]
+
+
+class sfh_compound_desc(C.Structure):
+    _fields_ = [("body", sfh_body_desc), ("parts", C.POINTER(sfh_part_desc)), ("nparts", C.c_uint32)]
+
+
 class sfh_batch_opts(C.Structure):
     _fields_ = [("faces_per_lane", C.c_uint32), ("waves", C.c_uint32)]
 
@@ -69,6 +86,10 @@ class sfh_surrogate_norm(C.Structure):
     ]
 
 
+class sfh_rigid(C.Structure):
+    _fields_ = [("mass", C.c_double), ("inertia", C.c_double * 3)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "sfh_last_error": (C.c_char_p, []),
@@ -90,6 +111,8 @@ _PROTOS = {
                               c_double_p]),
     "sfh_batch_create": (C.c_int, [C.POINTER(sfh_body_desc), C.c_uint32, C.POINTER(sfh_batch_opts),
                                    C.POINTER(C.c_void_p)]),
+    "sfh_compound_batch_create": (C.c_int, [C.POINTER(sfh_compound_desc), C.c_uint32, C.POINTER(sfh_batch_opts),
+                                            C.POINTER(C.c_void_p)]),
     "sfh_batch_destroy": (C.c_int, [C.c_void_p]),
     "sfh_batch_set_opts": (C.c_int, [C.c_void_p, C.POINTER(sfh_batch_opts)]),
     "sfh_batch_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint32_p, C.POINTER(C.c_uint64)]),
@@ -104,6 +127,8 @@ _PROTOS = {
                                       C.c_void_p]),
     "sfh_surrogate_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 7),
+    "sfh_rollout_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(sfh_rigid),
+                                        C.c_double, C.c_void_p]),
     "sfh_surrogate_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 6),
 }
@@ -166,6 +191,9 @@ _MLP_PROTOS = {
     "sfm_param_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "sfm_layer_io": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_int]),
     "sfm_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "sfm_forward_group": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, c_uint32_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
+    "sfm_forward_layers": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "sfm_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float, C.c_float,
                                  C.c_float, C.c_void_p, C.c_void_p]),
     "sfm_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -67,7 +67,10 @@ enum : uint32_t {
     BF_DAMPING = 4u,     // dampingForces (per-face calls); body level: always on
     BF_MODE_SURF = 8u,   // static Surface call: no classification, no Correct
     BF_MODE_SUBM = 16u,  // static Submerged call: no classification, no Correct
-    BF_EXPLICIT_TC = 32u // state carries T_C explicitly (per-face calls)
+    BF_EXPLICIT_TC = 32u,// state carries T_C explicitly (per-face calls)
+    BF_PART = 64u,       // part of a compound (Compound::ComputeHydrodynamicForces)
+    BF_EXTERNAL = 128u,  // part: CompoundPart::isExternal
+    BF_PHYDRO = 256u     // part: the part solid's mode is FLOATING or SUBMERGED
 };
 
 static constexpr int kMaxLayers = 4;
@@ -83,7 +86,7 @@ struct BodyStatic {
     uint32_t nv;
     uint32_t vchunk0;   // first vertex chunk of the body (depth pass)
     uint32_t nvchunks;
-    uint32_t pad_;
+    uint32_t sidx;      // row of the state array (and active mask) the body reads: its compound for a part
     uint64_t dep_off;   // offset of this body's vertex depths in the depth buffer
     double cg2c[12];   // basis row-major, origin
     double cg2o[9];    // basis row-major
@@ -106,6 +109,8 @@ struct LayerK {
 struct BodyDyn {
     int32_t pos;        // 0 outside, 1 inside (submerged loop), 2 crossing (surface loop)
     uint32_t flags;     // effective settings of the face loop
+    int32_t cpos;       // classification before the part rules (the compound's position for a part)
+    int32_t pad_;
     FaceBody fb;
     double rz[3];       // row 2 of R: world z of a mesh-frame vector
     double tz;          // world z of the mesh origin
@@ -224,13 +229,13 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
 {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
-    if (active && !active[b]) {
+    const BodyStatic& S = st[b];
+    if (active && !active[S.sidx]) {
         dyn[b].pos = -2;
         return;
     }
-    const BodyStatic& S = st[b];
     BodyDyn D;
-    const double* s = states + (size_t)b * (state_kind ? 38 : 22);
+    const double* s = states + (size_t)S.sidx * (state_kind ? 38 : 22);
     double Rcg[9], tcg[3], R[9], tc[3];
     gl_basis(s, Rcg);
     tcg[0] = s[12]; tcg[1] = s[13]; tcg[2] = s[14];
@@ -285,6 +290,20 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
         pos = under == 0 ? 0 : (under == 8 ? 1 : 2);
         // body level: damping always on (Ocean::ApplyFluidForces, Ocean.cpp:248-252)
         flags |= BF_DAMPING;
+    }
+    D.cpos = pos;
+    if (flags & BF_PART) {
+        // Compound::ComputeHydrodynamicForces (Compound.cpp:291-437): the parts
+        // follow the compound's classification.  Inside: drag of the external
+        // parts whose solid is hydrodynamic.  Crossing: every hydrodynamic
+        // part; an internal one only for its buoyancy, without drag.
+        const bool ext = flags & BF_EXTERNAL, ph = flags & BF_PHYDRO, pb = flags & BF_BUOYANT;
+        if (pos == 0) pos = -2;
+        else if (pos == 1 && !(ext && ph)) pos = -2;
+        else if (pos == 2) {
+            if (!ph || (!ext && !pb)) pos = -2;
+            else if (!ext) flags &= ~BF_DAMPING;
+        }
     }
     D.pos = pos;
     D.flags = flags;
@@ -762,7 +781,7 @@ __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ 
     double Fb[3] = {0, 0, 0}, Tb[3] = {0, 0, 0};
     double Swet = 0.0, Vsub = 0.0;
     if (pos == 1) {
-        if (body_level) {
+        if (body_level && !(flags & BF_PART)) {
             if (buoyant) {   // analytic buoyancy of a submerged body (1828-1836)
                 double cb[3];
                 for (int k = 0; k < 3; ++k) Fb[k] = -S.volume * oc.rho * oc.g[k];
@@ -818,6 +837,86 @@ __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ 
     if (written & 128u) o[19] = Vsub;
     o[20] = (double)written;
     o[21] = (double)pos;
+}
+
+// Per compound: static data of Compound::ComputeHydrodynamicForces.
+struct CompStatic {
+    uint32_t part0, nparts;   // the compound's parts are bodies part0 .. part0+nparts-1 of the batch
+    uint32_t flags;           // BF_BUOYANT of the compound
+    uint32_t pad_;
+    double volume, surface;
+    double P_CB[3];
+};
+
+// Composes the part rows into the compound's row (Compound.cpp:291-437), one
+// lane per compound, parts in their order.  Inside: the compound's analytic
+// buoyancy and the sum of the parts' corrected drag.  Crossing: the sums of
+// the part results, where a part's output the reference leaves untouched
+// keeps the previous part's value (the per-part temporaries are declared once
+// before the loop, Compound.cpp:392-399); internal parts add buoyancy only.
+__global__ void k_compound(const CompStatic* __restrict__ cs, const BodyStatic* __restrict__ st,
+                           const BodyDyn* __restrict__ dyn, uint32_t ncomp, OceanK oc,
+                           const uint8_t* __restrict__ active, const double* __restrict__ part_out,
+                           double* __restrict__ out)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncomp) return;
+    if (active && !active[c]) return;
+    const CompStatic& C = cs[c];
+    double* o = out + (size_t)c * SFH_OUT_DOUBLES;
+    const BodyDyn& D0 = dyn[C.part0];
+    const int cpos = D0.cpos;
+    if (cpos < 0) { o[20] = 0.0; o[21] = -1.0; return; }
+    if (cpos == 0) {
+        for (int k = 0; k < 20; ++k) o[k] = 0.0;
+        o[20] = 255.0; o[21] = 0.0;
+        return;
+    }
+    if (cpos == 1) {
+        double acc[12];
+        for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+        for (uint32_t p = C.part0; p < C.part0 + C.nparts; ++p) {
+            if (dyn[p].pos != 1) continue;
+            const double* row = part_out + (size_t)p * SFH_OUT_DOUBLES;
+            for (int k = 0; k < 12; ++k) acc[k] += row[6 + k];
+        }
+        uint32_t written = 4u | 8u | 16u | 32u | 64u | 128u;
+        if (C.flags & BF_BUOYANT) {
+            double Fb[3], Tb[3], cb[3];
+            for (int k = 0; k < 3; ++k) Fb[k] = -C.volume * oc.rho * oc.g[k];
+            mat_vec(D0.Rcg, C.P_CB, cb);
+            cross64(cb, Fb, Tb);
+            for (int k = 0; k < 3; ++k) { o[k] = Fb[k]; o[3 + k] = Tb[k]; }
+            written |= 3u;
+        }
+        for (int k = 0; k < 12; ++k) o[6 + k] = acc[k];
+        o[18] = C.surface;
+        o[19] = C.volume;
+        o[20] = (double)written;
+        o[21] = 1.0;
+        return;
+    }
+    double carry[20], acc[20];
+    for (int k = 0; k < 20; ++k) carry[k] = acc[k] = 0.0;
+    for (uint32_t p = C.part0; p < C.part0 + C.nparts; ++p) {
+        if (dyn[p].pos != 2) continue;
+        const double* row = part_out + (size_t)p * SFH_OUT_DOUBLES;
+        const uint32_t wm = (uint32_t)row[20];
+        for (int q = 0; q < 6; ++q)
+            if (wm & (1u << q))
+                for (int k = 0; k < 3; ++k) carry[3 * q + k] = row[3 * q + k];
+        if (wm & 64u) carry[18] = row[18];
+        if (wm & 128u) carry[19] = row[19];
+        if (st[p].flags & BF_EXTERNAL) {
+            for (int k = 0; k < 20; ++k) acc[k] += carry[k];
+        } else {
+            for (int k = 0; k < 6; ++k) acc[k] += carry[k];
+            acc[19] += carry[19];
+        }
+    }
+    for (int k = 0; k < 20; ++k) o[k] = acc[k];
+    o[20] = 255.0;
+    o[21] = 2.0;
 }
 
 // Standalone CorrectHydrodynamicForces for the per-body entry point.
@@ -912,7 +1011,12 @@ struct sfh_ocean_t {
 struct sfh_batch_t {
     std::vector<sfh_mesh_t*> meshes;   // one reference per body with a mesh
     std::vector<sfh_mesh_t*> meshes_by_body;
-    uint32_t n = 0;
+    uint32_t n = 0;         // bodies of the face passes (the parts, for a compound batch)
+    uint32_t nunits = 0;    // rows of the state and output arrays: bodies, or compounds
+    bool compound = false;
+    CompStatic* d_comp = nullptr;
+    double* d_part_out = nullptr;
+    double* d_unit_pcb = nullptr;   // P_CB of every unit (CG frame), for the surrogate features
     uint32_t nchunks = 0;
     uint64_t nfaces = 0;
     uint32_t K = 8;
@@ -1151,6 +1255,10 @@ static void batch_free(sfh_batch_t* b)
     hipFree(b->d_lists);
     hipFree(b->d_dyn);
     hipFree(b->d_partials);
+    hipFree(b->d_comp);
+    hipFree(b->d_part_out);
+    hipFree(b->d_unit_pcb);
+    b->d_comp = nullptr; b->d_part_out = nullptr; b->d_unit_pcb = nullptr;
     for (sfh_mesh_t* m : b->meshes) mesh_release(m);
     b->meshes.clear();
 }
@@ -1252,6 +1360,14 @@ static BodyStatic make_static(const sfh_mesh_t* m, uint32_t flags)
     return s;
 }
 
+static int upload_units(sfh_batch_t* b, const std::vector<double>& pcb)
+{
+    HIPCHK(hipMalloc(&b->d_unit_pcb, sizeof(double) * std::max<size_t>(3, pcb.size())));
+    if (!pcb.empty())
+        HIPCHK(hipMemcpy(b->d_unit_pcb, pcb.data(), sizeof(double) * pcb.size(), hipMemcpyHostToDevice));
+    return SFH_OK;
+}
+
 extern "C" int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const sfh_batch_opts* opts, sfh_batch* out)
 {
     if (!out || (!bodies && n)) return fail(SFH_ERR_INVALID, "null argument");
@@ -1265,6 +1381,7 @@ extern "C" int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const s
         if (d.hydro) flags |= BF_HYDRO;
         if (d.buoyant) flags |= BF_BUOYANT;
         BodyStatic s = make_static(d.mesh, flags);
+        s.sidx = i;
         if (d.mesh) {
             d.mesh->refs.fetch_add(1);
             b->meshes.push_back(d.mesh);
@@ -1285,8 +1402,97 @@ extern "C" int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const s
         b->nfaces += s.nf;
     }
     b->K = 0;
+    b->nunits = n;
+    std::vector<double> pcb(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) pcb[3 * (size_t)i + k] = bodies[i].P_CB[k];
     int rc = hipMalloc(&b->d_dyn, sizeof(BodyDyn) * std::max<size_t>(1, n)) == hipSuccess ? SFH_OK
              : fail(SFH_ERR_HIP, "batch alloc failed");
+    if (rc == SFH_OK) rc = upload_units(b, pcb);
+    if (rc == SFH_OK) rc = apply_opts(b, opts);
+    if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
+    *out = b;
+    return SFH_OK;
+}
+
+static void gl_mul_host(const double* a, const double* c, double* r)   // OpenGL column-major 4x4: r = a c
+{
+    double t[16];
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i)
+            t[4 * j + i] = a[i] * c[4 * j] + a[4 + i] * c[4 * j + 1] + a[8 + i] * c[4 * j + 2] + a[12 + i] * c[4 * j + 3];
+    memcpy(r, t, sizeof(t));
+}
+
+extern "C" int sfh_compound_batch_create(const sfh_compound_desc* comps, uint32_t n, const sfh_batch_opts* opts,
+                                         sfh_batch* out)
+{
+    if (!out || (!comps && n)) return fail(SFH_ERR_INVALID, "null argument");
+    uint64_t nparts = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        if (comps[c].nparts == 0 || !comps[c].parts) return fail(SFH_ERR_INVALID, "compound %u has no parts", c);
+        for (uint32_t p = 0; p < comps[c].nparts; ++p)
+            if (!comps[c].parts[p].mesh) return fail(SFH_ERR_INVALID, "compound %u part %u has no mesh", c, p);
+        nparts += comps[c].nparts;
+    }
+    if (nparts > (1u << 26)) return fail(SFH_ERR_SIZE, "too many parts");
+    sfh_batch_t* b = new sfh_batch_t();
+    b->n = (uint32_t)nparts;
+    b->nunits = n;
+    b->compound = true;
+    hipGetDevice(&b->device);
+    std::vector<CompStatic> cs(n);
+    std::vector<double> pcb(3 * (size_t)n);
+    for (uint32_t c = 0; c < n; ++c) {
+        const sfh_compound_desc& C = comps[c];
+        CompStatic k;
+        memset(&k, 0, sizeof(k));
+        k.part0 = (uint32_t)b->h_static.size();
+        k.nparts = C.nparts;
+        k.flags = C.body.buoyant ? BF_BUOYANT : 0u;
+        k.volume = C.body.volume;
+        k.surface = C.body.surface;
+        for (int i = 0; i < 3; ++i) k.P_CB[i] = pcb[3 * (size_t)c + i] = C.body.P_CB[i];
+        cs[c] = k;
+        for (uint32_t p = 0; p < C.nparts; ++p) {
+            const sfh_part_desc& P = C.parts[p];
+            uint32_t flags = BF_PART;
+            if (C.body.hydro) flags |= BF_HYDRO;
+            if (P.external) flags |= BF_EXTERNAL;
+            if (P.buoyant) flags |= BF_BUOYANT;
+            if (P.hydro) flags |= BF_PHYDRO;
+            BodyStatic s = make_static(P.mesh, flags);
+            s.sidx = c;
+            P.mesh->refs.fetch_add(1);
+            b->meshes.push_back(P.mesh);
+            b->meshes_by_body.push_back(P.mesh);
+            // T_C of the part = T_O * origin * O2C, T_O of the part = T_O * origin,
+            // with T_O = T_CG * T_CG2O (Compound.cpp:348-349, 408-409)
+            double o2p[16], cg2c[16];
+            gl_mul_host(C.body.T_CG2O, P.origin, o2p);
+            gl_mul_host(o2p, P.O2C, cg2c);
+            gl_to_rows(cg2c, s.cg2c, s.cg2c + 9);
+            gl_to_rows(o2p, s.cg2o, nullptr);
+            for (int i = 0; i < 3; ++i) {
+                s.aabb_c[i] = 0.5 * (C.body.aabb_max[i] + C.body.aabb_min[i]);
+                s.aabb_h[i] = 0.5 * (C.body.aabb_max[i] - C.body.aabb_min[i]);
+                s.Cd[i] = P.Cd[i];
+                s.Cf[i] = P.Cf[i];
+            }
+            b->h_static.push_back(s);
+            b->nfaces += s.nf;
+        }
+    }
+    b->K = 0;
+    int rc = SFH_OK;
+    if (hipMalloc(&b->d_dyn, sizeof(BodyDyn) * std::max<uint64_t>(1, nparts)) != hipSuccess ||
+        hipMalloc(&b->d_comp, sizeof(CompStatic) * std::max<uint32_t>(1, n)) != hipSuccess ||
+        hipMalloc(&b->d_part_out, sizeof(double) * SFH_OUT_DOUBLES * std::max<uint64_t>(1, nparts)) != hipSuccess)
+        rc = fail(SFH_ERR_HIP, "compound batch alloc failed");
+    if (rc == SFH_OK && n &&
+        hipMemcpy(b->d_comp, cs.data(), sizeof(CompStatic) * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SFH_ERR_HIP, "compound upload failed");
+    if (rc == SFH_OK) rc = upload_units(b, pcb);
     if (rc == SFH_OK) rc = apply_opts(b, opts);
     if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
     *out = b;
@@ -1311,7 +1517,7 @@ extern "C" int sfh_batch_set_opts(sfh_batch b, const sfh_batch_opts* opts)
 extern "C" int sfh_batch_info(sfh_batch b, uint32_t* nbodies, uint32_t* nchunks, uint64_t* nfaces)
 {
     if (!b) return fail(SFH_ERR_INVALID, "null batch");
-    if (nbodies) *nbodies = b->n;
+    if (nbodies) *nbodies = b->nunits;
     if (nchunks) *nchunks = b->nchunks;
     if (nfaces) *nfaces = b->nfaces;
     return SFH_OK;
@@ -1365,7 +1571,11 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
     int rc = launch_faces(b, oc, s, timed);
     if (rc != SFH_OK) return rc;
     if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
-    hipLaunchKernelGGL(k_epilogue, dim3(b->n), dim3(64), 0, s, b->d_static, b->d_dyn, b->n, oc, b->d_partials, d_out);
+    hipLaunchKernelGGL(k_epilogue, dim3(b->n), dim3(64), 0, s, b->d_static, b->d_dyn, b->n, oc, b->d_partials,
+                       b->compound ? b->d_part_out : d_out);
+    if (b->compound && b->nunits)
+        hipLaunchKernelGGL(k_compound, dim3((b->nunits + 127) / 128), dim3(128), 0, s, b->d_comp, b->d_static,
+                           b->d_dyn, b->nunits, oc, active, b->d_part_out, d_out);
     HIPCHK(hipGetLastError());
     return SFH_OK;
 }
@@ -1424,7 +1634,7 @@ extern "C" int sfh_batch_compute_host(sfh_batch b, sfh_ocean o, const double* st
 {
     if (!b || !states || !out) return fail(SFH_ERR_INVALID, "null argument");
     double *ds = nullptr, *dout = nullptr;
-    size_t ns = (size_t)b->n * SFH_STATE_DOUBLES, no = (size_t)b->n * SFH_OUT_DOUBLES;
+    size_t ns = (size_t)b->nunits * SFH_STATE_DOUBLES, no = (size_t)b->nunits * SFH_OUT_DOUBLES;
     HIPCHK(hipMalloc(&ds, sizeof(double) * std::max<size_t>(1, ns)));
     hipError_t e = hipMalloc(&dout, sizeof(double) * std::max<size_t>(1, no));
     if (e != hipSuccess) { hipFree(ds); return fail(SFH_ERR_HIP, "alloc: %s", hipGetErrorString(e)); }
@@ -1535,6 +1745,7 @@ static int single_prepare(sfh_mesh_t* m, uint32_t flags)
         if (S.b) { batch_free(S.b); delete S.b; S.b = nullptr; }
         sfh_batch_t* b = new sfh_batch_t();
         b->n = 1;
+        b->nunits = 1;
         b->device = dev;
         m->refs.fetch_add(1);
         b->meshes.push_back(m);
@@ -1729,7 +1940,7 @@ __device__ void rot_to_quat(const double* R, double* q)
     q[0] = x * inv; q[1] = y * inv; q[2] = z * inv; q[3] = w * inv;
 }
 
-__global__ void k_sur_features(const BodyStatic* __restrict__ st, const double* __restrict__ states, uint32_t first,
+__global__ void k_sur_features(const double* __restrict__ unit_pcb, const double* __restrict__ states, uint32_t first,
                                uint32_t count, uint32_t rows, OceanK oc, SurNorm nm, float* __restrict__ raw,
                                uint16_t* __restrict__ x, uint8_t* __restrict__ ood,
                                const double* __restrict__ exact, float* __restrict__ raw_target,
@@ -1753,7 +1964,7 @@ __global__ void k_sur_features(const BodyStatic* __restrict__ st, const double* 
     mat_t_vec(R, s + 19, f + 3);
     rot_to_quat(R, f + 6);
     double cb[3];
-    mat_vec(R, st[b].P_CB, cb);
+    mat_vec(R, unit_pcb + 3 * (size_t)b, cb);
     for (int k = 0; k < 3; ++k) cb[k] += t[k];
     f[10] = cb[2];
     f[11] = f[12] = 0.0;
@@ -1854,14 +2065,14 @@ extern "C" int sfh_surrogate_features(sfh_batch b, sfh_ocean o, const double* d_
     if (!b || !d_states) return fail(SFH_ERR_INVALID, "null argument");
     int rc = check_ocean(o);
     if (rc != SFH_OK) return rc;
-    if ((uint64_t)first + count > b->n) return fail(SFH_ERR_SIZE, "body range out of the batch");
+    if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "body range out of the batch");
     if ((d_x || d_ood || d_target) && !norm) return fail(SFH_ERR_INVALID, "normalised outputs need a norm");
     if ((d_raw_target || d_target) && !d_exact) return fail(SFH_ERR_INVALID, "targets need the exact outputs");
     if (d_x && (rows < count || rows % 32)) return fail(SFH_ERR_SIZE, "rows must be a multiple of 32 >= count");
     if (d_x && ((uintptr_t)d_x % 16)) return fail(SFH_ERR_INVALID, "d_x must be 16-byte aligned");
     const uint32_t r = d_x ? rows : count;
     if (r == 0) return SFH_OK;
-    hipLaunchKernelGGL(k_sur_features, dim3((r + 127) / 128), dim3(128), 0, (hipStream_t)stream, b->d_static,
+    hipLaunchKernelGGL(k_sur_features, dim3((r + 127) / 128), dim3(128), 0, (hipStream_t)stream, b->d_unit_pcb,
                        d_states, first, count, r, make_oceank(o), sur_norm(norm), d_raw, d_x, d_ood, d_exact,
                        d_raw_target, d_target);
     HIPCHK(hipGetLastError());
@@ -1873,10 +2084,69 @@ extern "C" int sfh_surrogate_apply(sfh_batch b, const double* d_states, uint32_t
                                    const double* d_exact, double* d_ft, float* d_err, void* stream)
 {
     if (!b || !d_states || !norm || !d_y || !d_ft) return fail(SFH_ERR_INVALID, "null argument");
-    if ((uint64_t)first + count > b->n) return fail(SFH_ERR_SIZE, "body range out of the batch");
+    if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "body range out of the batch");
     if (count == 0) return SFH_OK;
     hipLaunchKernelGGL(k_sur_apply, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_states, first,
                        count, sur_norm(norm), d_y, d_ood, d_exact, d_ft, d_err);
+    HIPCHK(hipGetLastError());
+    return SFH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rollout harness: rigid-body step of the bodies from their total F/T
+// ---------------------------------------------------------------------------
+
+// Semi-implicit Euler of the Newton-Euler equations about the CG: linear
+// velocity from F + m g, angular velocity in the CG frame from
+// T - w x (I w) with principal moments I, then position from the new
+// velocity and orientation by the exact rotation of w dt (Rodrigues).
+__global__ void k_rollout(double* __restrict__ states, uint32_t count, const double* __restrict__ ft, double mass,
+                          double I0, double I1, double I2, double g0, double g1, double g2, double dt)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double* s = states + (size_t)i * SFH_STATE_DOUBLES;
+    const double* f = ft + (size_t)i * 6;
+    double R[9];
+    gl_basis(s, R);
+    const double g[3] = {g0, g1, g2};
+    for (int k = 0; k < 3; ++k) s[16 + k] += dt * (f[k] / mass + g[k]);
+    double wb[3], tb[3];
+    mat_t_vec(R, s + 19, wb);
+    mat_t_vec(R, f + 3, tb);
+    const double I[3] = {I0, I1, I2};
+    double Iw[3] = {I0 * wb[0], I1 * wb[1], I2 * wb[2]}, gyro[3];
+    cross64(wb, Iw, gyro);
+    for (int k = 0; k < 3; ++k) wb[k] += dt * (tb[k] - gyro[k]) / I[k];
+    mat_vec(R, wb, s + 19);
+    for (int k = 0; k < 3; ++k) s[12 + k] += dt * s[16 + k];
+    // R <- R exp([wb dt]x)
+    const double ax = wb[0] * dt, ay = wb[1] * dt, az = wb[2] * dt;
+    const double th = sqrt(ax * ax + ay * ay + az * az);
+    double E[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (th > 0.0) {
+        const double kx = ax / th, ky = ay / th, kz = az / th, c = cos(th), sn = sin(th), cc = 1.0 - c;
+        E[0] = c + kx * kx * cc;      E[1] = kx * ky * cc - kz * sn; E[2] = kx * kz * cc + ky * sn;
+        E[3] = ky * kx * cc + kz * sn; E[4] = c + ky * ky * cc;      E[5] = ky * kz * cc - kx * sn;
+        E[6] = kz * kx * cc - ky * sn; E[7] = kz * ky * cc + kx * sn; E[8] = c + kz * kz * cc;
+    }
+    double Rn[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Rn[3 * r + c] = R[3 * r] * E[c] + R[3 * r + 1] * E[3 + c] + R[3 * r + 2] * E[6 + c];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) s[4 * c + r] = Rn[3 * r + c];
+}
+
+extern "C" int sfh_rollout_integrate(sfh_ocean o, double* d_states, uint32_t count, const double* d_ft,
+                                     const sfh_rigid* rigid, double dt, void* stream)
+{
+    if (!o || !d_states || !d_ft || !rigid) return fail(SFH_ERR_INVALID, "null argument");
+    if (!(rigid->mass > 0.0) || !(rigid->inertia[0] > 0.0) || !(rigid->inertia[1] > 0.0) || !(rigid->inertia[2] > 0.0))
+        return fail(SFH_ERR_INVALID, "mass and inertia must be positive");
+    if (count == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_rollout, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_states, count, d_ft,
+                       rigid->mass, rigid->inertia[0], rigid->inertia[1], rigid->inertia[2], o->g[0], o->g[1], o->g[2],
+                       dt);
     HIPCHK(hipGetLastError());
     return SFH_OK;
 }

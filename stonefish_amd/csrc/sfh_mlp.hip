@@ -174,6 +174,91 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 }
 
 // ---------------------------------------------------------------------------
+// fused inference: all five layers of a tile of rows in one workgroup
+// ---------------------------------------------------------------------------
+
+struct NetSet {
+    int nnets;
+    uint32_t row_end[SFM_MAX_GROUP];   // exclusive end row of each network's rows
+    const uint16_t* W[SFM_MAX_GROUP][SFM_LAYERS];
+    const float* b[SFM_MAX_GROUP][SFM_LAYERS];
+};
+
+static constexpr int kActStride = SFM_HIDDEN + 8;   // LDS row stride (elements): spreads rows over banks
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// One workgroup (4 wavefronts) runs TR consecutive rows of one network through
+// the whole network: the hidden activations stay in LDS (two TR x 256 BF16
+// buffers), wavefront w computes output columns [64 w, 64 w + 64) of each
+// hidden layer from weights read straight from global memory (L2-resident,
+// 400 KB per network), and the output layer is split over the row tiles.
+template <int TR>
+__global__ __launch_bounds__(256) void k_mlp_fused(NetSet ns, const uint16_t* __restrict__ x, uint32_t rows,
+                                                   float* __restrict__ y)
+{
+    constexpr int RT = TR / 32;
+    __shared__ __attribute__((aligned(16))) uint16_t act[2][TR * kActStride];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t row0 = blockIdx.x * TR;
+    if (row0 >= rows) return;
+    int k = 0;
+    while (k + 1 < ns.nnets && row0 >= ns.row_end[k]) ++k;
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+
+    for (int l = 0; l < SFM_LAYERS - 1; ++l) {
+        const int K = (l == 0) ? SFM_IN_PAD : SFM_HIDDEN;
+        const uint16_t* W = ns.W[k][l];
+        const float* bias = ns.b[k][l];
+        const uint16_t* in = act[(l + 1) & 1];
+        uint16_t* out = act[l & 1];
+        f32x16 acc[RT][2];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) { acc[rt][0] = f32x16{}; acc[rt][1] = f32x16{}; }
+        const uint16_t* wp0 = W + (size_t)(64 * wave + lr) * K + lk;
+        const uint16_t* wp1 = wp0 + (size_t)32 * K;
+        for (int ks = 0; ks < K; ks += 16) {
+            const bf16x8 b0 = ld8(wp0 + ks), b1 = ld8(wp1 + ks);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                const bf16x8 a = (l == 0) ? ld8(x + (size_t)(row0 + 32 * rt + lr) * SFM_IN_PAD + ks + lk)
+                                          : ld8(in + (32 * rt + lr) * kActStride + ks + lk);
+                acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[rt][0], 0, 0, 0);
+                acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[rt][1], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int col = 64 * wave + 32 * nt + lr;
+            const float bc = bias[col];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    out[(32 * rt + acc_row(r, lane)) * kActStride + col] = f2bf(gelu(acc[rt][nt][r] + bc));
+        }
+        __syncthreads();
+    }
+    // output layer: one 32 x 32 tile per row tile, K = 256, from act[1]
+    if (wave < RT) {
+        const uint16_t* W = ns.W[k][SFM_LAYERS - 1];
+        const float* bias = ns.b[k][SFM_LAYERS - 1];
+        const uint16_t* in = act[1];
+        f32x16 acc = {};
+        const uint16_t* wp = W + (size_t)lr * SFM_HIDDEN + lk;
+        for (int ks = 0; ks < SFM_HIDDEN; ks += 16)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(in + (32 * wave + lr) * kActStride + ks + lk),
+                                                          ld8(wp + ks), acc, 0, 0, 0);
+        if (lr < SFM_OUT) {
+            const float bc = bias[lr];
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                y[(size_t)(row0 + 32 * wave + acc_row(r, lane)) * SFM_OUT + lr] = acc[r] + bc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // small kernels
 // ---------------------------------------------------------------------------
 
@@ -456,7 +541,59 @@ static void forward_hidden(sfm_net_t* n, const uint16_t* x, uint32_t M, bool tra
     }
 }
 
+// Inference through the fused kernel: 64-row tiles when every network's row
+// count allows it, else 32-row tiles.
+static int forward_group(const sfm_net* nets, uint32_t nnets, const uint32_t* counts, const uint16_t* x, float* y,
+                         hipStream_t s)
+{
+    if (!nets || !counts || !x || !y || nnets == 0) return fail(SFM_ERR_INVALID, "null argument");
+    if (nnets > SFM_MAX_GROUP) return fail(SFM_ERR_SIZE, "at most %d networks per call", SFM_MAX_GROUP);
+    if ((uintptr_t)x % 16) return fail(SFM_ERR_INVALID, "x must be 16-byte aligned");
+    NetSet ns;
+    memset(&ns, 0, sizeof(ns));
+    ns.nnets = (int)nnets;
+    uint64_t total = 0;
+    bool even64 = true;
+    for (uint32_t i = 0; i < nnets; ++i) {
+        if (!nets[i]) return fail(SFM_ERR_INVALID, "null network");
+        if (counts[i] % 32) return fail(SFM_ERR_SIZE, "row counts must be multiples of 32");
+        even64 = even64 && counts[i] % 64 == 0;
+        total += counts[i];
+        if (total > (1ull << 31)) return fail(SFM_ERR_SIZE, "too many rows");
+        ns.row_end[i] = (uint32_t)total;
+        for (int l = 0; l < SFM_LAYERS; ++l) {
+            ns.W[i][l] = nets[i]->L[l].Wb;
+            ns.b[i][l] = nets[i]->L[l].b;
+        }
+    }
+    if (total == 0) return SFM_OK;
+    sfm_net_t* t = nets[0];
+    time_begin(t, s);
+    if (even64)
+        hipLaunchKernelGGL(k_mlp_fused<64>, dim3((unsigned)(total / 64)), dim3(256), 0, s, ns, x, (uint32_t)total, y);
+    else
+        hipLaunchKernelGGL(k_mlp_fused<32>, dim3((unsigned)(total / 32)), dim3(256), 0, s, ns, x, (uint32_t)total, y);
+    time_end(t, s, forward_flops((uint32_t)total));
+    HIPCHK(hipGetLastError());
+    return SFM_OK;
+}
+
 extern "C" int sfm_forward(sfm_net n, const uint16_t* x, uint32_t batch, float* y, void* stream)
+{
+    if (!n || !x || !y) return fail(SFM_ERR_INVALID, "null argument");
+    if (batch == 0 || batch % 32) return fail(SFM_ERR_SIZE, "batch must be a positive multiple of 32");
+    return forward_group(&n, 1, &batch, x, y, (hipStream_t)stream);
+}
+
+extern "C" int sfm_forward_group(const sfm_net* nets, uint32_t nnets, const uint32_t* counts, const uint16_t* x,
+                                 float* y, void* stream)
+{
+    return forward_group(nets, nnets, counts, x, y, (hipStream_t)stream);
+}
+
+// Inference through the per-layer kernels of the training path (kept for
+// checking the fused kernel against them).
+extern "C" int sfm_forward_layers(sfm_net n, const uint16_t* x, uint32_t batch, float* y, void* stream)
 {
     if (!n || !x || !y) return fail(SFM_ERR_INVALID, "null argument");
     if (batch == 0 || batch % 32 || batch > n->maxb) return fail(SFM_ERR_SIZE, "batch must be a multiple of 32 <= %u", n->maxb);

@@ -195,6 +195,47 @@ def cg_frame_aabb(mesh_min, mesh_max, T_CG2C, margin: float = 0.0):
     return wc - e, wc + e
 
 
+def _fill_body(d, b: BodyDesc) -> None:
+    d.mesh = b.mesh.handle if b.mesh is not None else None
+    d.T_CG2C[:] = list(_vec(b.T_CG2C, 16))
+    d.T_CG2O[:] = list(_vec(b.T_CG2O, 16))
+    if b.aabb_min is None:
+        mn, mx = cg_frame_aabb(b.mesh.aabb_min, b.mesh.aabb_max, b.T_CG2C)
+    else:
+        mn, mx = b.aabb_min, b.aabb_max
+    d.aabb_min[:] = list(_vec(mn))
+    d.aabb_max[:] = list(_vec(mx))
+    d.Cd[:] = list(_vec(b.Cd))
+    d.Cf[:] = list(_vec(b.Cf))
+    d.volume = float(b.volume)
+    d.surface = float(b.surface)
+    d.P_CB[:] = list(_vec(b.P_CB))
+    d.buoyant = int(b.buoyant)
+    d.hydro = int(b.hydro)
+
+
+@dataclass
+class PartDesc:
+    """One part of a compound (Compound.h CompoundPart and the part solid)."""
+    mesh: Mesh
+    origin: np.ndarray = field(default_factory=lambda: _ident())
+    O2C: np.ndarray = field(default_factory=lambda: _ident())
+    Cd: np.ndarray = field(default_factory=lambda: np.ones(3))
+    Cf: np.ndarray = field(default_factory=lambda: 0.1 * np.ones(3))
+    external: bool = True
+    buoyant: bool = True
+    hydro: bool = True
+
+
+@dataclass
+class CompoundDesc:
+    """A compound body (Compound::ComputeHydrodynamicForces, Compound.cpp:291-437):
+    `body` carries T_CG2O, the CG-frame AABB (required), volume, surface, P_CB,
+    buoyant, hydro; its mesh and T_CG2C are unused."""
+    body: BodyDesc
+    parts: list[PartDesc]
+
+
 class BodyBatch:
     """Many bodies evaluated in one pass (SolidEntity::ComputeHydrodynamicForces
     for each, with Ocean::ApplyFluidForces' settings {damping, buoyancy} = {true, true})."""
@@ -203,29 +244,19 @@ class BodyBatch:
         lib = N.load()
         arr = (N.sfh_body_desc * max(1, len(bodies)))()
         for i, b in enumerate(bodies):
-            d = arr[i]
-            d.mesh = b.mesh.handle
-            d.T_CG2C[:] = list(_vec(b.T_CG2C, 16))
-            d.T_CG2O[:] = list(_vec(b.T_CG2O, 16))
-            if b.aabb_min is None:
-                mn, mx = cg_frame_aabb(b.mesh.aabb_min, b.mesh.aabb_max, b.T_CG2C)
-            else:
-                mn, mx = b.aabb_min, b.aabb_max
-            d.aabb_min[:] = list(_vec(mn))
-            d.aabb_max[:] = list(_vec(mx))
-            d.Cd[:] = list(_vec(b.Cd))
-            d.Cf[:] = list(_vec(b.Cf))
-            d.volume = float(b.volume)
-            d.surface = float(b.surface)
-            d.P_CB[:] = list(_vec(b.P_CB))
-            d.buoyant = int(b.buoyant)
-            d.hydro = int(b.hydro)
+            _fill_body(arr[i], b)
         opts = N.sfh_batch_opts(faces_per_lane, waves)
         h = C.c_void_p()
         N.check(lib.sfh_batch_create(arr, len(bodies), C.byref(opts), C.byref(h)))
         self.handle = h
         self.n = len(bodies)
         self.bodies = bodies   # keep meshes alive
+
+    def compute_masked(self, ocn: Ocean, d_states: int, d_active: int | None, d_out: int, stream: int = 0) -> None:
+        """sfh_batch_compute_masked: only rows with d_active[i] != 0 (device bytes) are evaluated and written."""
+        N.check(N.load().sfh_batch_compute_masked(self.handle, ocn.handle, C.c_void_p(d_states),
+                                                  C.c_void_p(d_active or 0), C.c_void_p(d_out),
+                                                  C.c_void_p(stream)))
 
     def info(self):
         nb, nc, nf = C.c_uint32(), C.c_uint32(), C.c_uint64()
@@ -278,3 +309,37 @@ class BodyBatch:
 
 OUT_FIELDS = {"Fb": slice(0, 3), "Tb": slice(3, 6), "Fdq": slice(6, 9), "Tdq": slice(9, 12),
               "Fdf": slice(12, 15), "Tdf": slice(15, 18), "Swet": 18, "Vsub": 19, "written": 20, "position": 21}
+
+
+class CompoundBatch(BodyBatch):
+    """A batch of compounds: states and outputs per compound; the parts run
+    through the face passes as bodies (sfh_compound_batch_create)."""
+
+    def __init__(self, comps: list[CompoundDesc], faces_per_lane: int = 0, waves: int = 0):
+        lib = N.load()
+        arr = (N.sfh_compound_desc * max(1, len(comps)))()
+        self._parts = []
+        for i, c in enumerate(comps):
+            if c.body.aabb_min is None or c.body.aabb_max is None:
+                raise ValueError("a compound needs its CG-frame AABB")
+            _fill_body(arr[i].body, c.body)
+            parts = (N.sfh_part_desc * max(1, len(c.parts)))()
+            for j, p in enumerate(c.parts):
+                q = parts[j]
+                q.mesh = p.mesh.handle
+                q.origin[:] = list(_vec(p.origin, 16))
+                q.O2C[:] = list(_vec(p.O2C, 16))
+                q.Cd[:] = list(_vec(p.Cd))
+                q.Cf[:] = list(_vec(p.Cf))
+                q.external = int(p.external)
+                q.buoyant = int(p.buoyant)
+                q.hydro = int(p.hydro)
+            self._parts.append(parts)
+            arr[i].parts = parts
+            arr[i].nparts = len(c.parts)
+        opts = N.sfh_batch_opts(faces_per_lane, waves)
+        h = C.c_void_p()
+        N.check(lib.sfh_compound_batch_create(arr, len(comps), C.byref(opts), C.byref(h)))
+        self.handle = h
+        self.n = len(comps)
+        self.bodies = comps

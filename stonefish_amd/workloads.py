@@ -119,3 +119,53 @@ def random_states(w: Workload, steps: int, seed: int | None = None, positions: n
         out[s, :, 16:19] = rng.uniform(-2.0, 2.0, (nb, 3))
         out[s, :, 19:22] = rng.uniform(-1.0, 1.0, (nb, 3))
     return out
+
+
+# ---- compound bodies: an AUV-like hull with four thruster fairings -------------------
+
+@dataclass
+class PartData:
+    mesh: M.TriMesh
+    origin: np.ndarray          # OpenGL 4x4: compound O frame -> part O frame
+    Cd: np.ndarray
+    Cf: np.ndarray
+    external: bool = True
+    buoyant: bool = True
+
+
+@dataclass
+class CompoundData:
+    parts: list[PartData]
+    volume: float
+    surface: float
+    P_CB: np.ndarray            # CG frame (the CG is the origin of the compound)
+    aabb_min: np.ndarray        # CG-frame AABB of the collision shape
+    aabb_max: np.ndarray
+    mass: float
+    inertia: np.ndarray         # principal moments about the CG axes
+
+
+def auv_compound(hull_radius: float = 0.3, hull_length: float = 3.0, fairing_radius: float = 0.07,
+                 fairing_length: float = 0.4, fairing_offset: float = 0.42, density: float = 1000.0,
+                 buoyancy_margin: float = 0.02) -> CompoundData:
+    """AUV-like compound: a capsule hull of 8,192 triangles along x and four
+    capsule thruster fairings of 1,024 triangles each at the stern (at +-y and
+    +-z).  The CG is at the origin; mass is the displaced water's minus a
+    small margin (slightly positively buoyant)."""
+    hull = M.capsule(hull_radius, hull_length, n_around=64, cap_bands=12, body_bands=41, axis=0)
+    fair = M.capsule(fairing_radius, fairing_length, n_around=16, cap_bands=6, body_bands=21, axis=0)
+    x_f = -0.5 * hull_length + hull_radius + 0.5 * fairing_length
+    parts = [PartData(hull, T.identity_gl(), *coeffs_from_half_axes((0.5 * hull_length, hull_radius, hull_radius)))]
+    for (y, z) in ((fairing_offset, 0.0), (-fairing_offset, 0.0), (0.0, fairing_offset), (0.0, -fairing_offset)):
+        parts.append(PartData(fair, T.translation_gl((x_f, y, z)),
+                              *coeffs_from_half_axes((0.5 * fairing_length, fairing_radius, fairing_radius))))
+    vols = np.array([p.mesh.volume() for p in parts])
+    cents = np.array([p.mesh.verts.astype(np.float64).mean(0) + p.origin[12:15] for p in parts])
+    volume = float(vols.sum())
+    P_CB = (vols[:, None] * cents).sum(0) / volume
+    lo = np.min([p.mesh.verts.min(0) + p.origin[12:15] for p in parts], axis=0).astype(np.float64)
+    hi = np.max([p.mesh.verts.max(0) + p.origin[12:15] for p in parts], axis=0).astype(np.float64)
+    mass = density * volume * (1.0 - buoyancy_margin)
+    r, L = hull_radius, hull_length
+    inertia = np.array([0.5 * mass * r * r, mass * (3 * r * r + L * L) / 12.0, mass * (3 * r * r + L * L) / 12.0])
+    return CompoundData(parts, volume, float(sum(p.mesh.area() for p in parts)), P_CB, lo, hi, mass, inertia)

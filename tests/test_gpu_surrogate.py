@@ -53,6 +53,29 @@ def test_forward_matches_reference(gpu):
     net.close()
 
 
+def test_fused_group_and_layer_paths_agree(gpu):
+    from stonefish_amd import surrogate as S
+    nets = [_net(1024), _net(1024)]
+    ws = [R.init_weights(51), R.init_weights(52)]
+    for n, (Ws, bs) in zip(nets, ws):
+        _load(n, Ws, bs)
+    counts = [96, 160]   # 32-row tiles (not multiples of 64)
+    x, _ = R.random_inputs(53, sum(counts))
+    dx = x.to(torch.bfloat16).cuda()
+    y = torch.zeros(sum(counts), 6, dtype=torch.float32, device="cuda")
+    S.forward_group(nets, counts, dx, y)
+    yl = torch.zeros_like(y)
+    nets[0].forward_layers(dx[:96], yl[:96])
+    nets[1].forward_layers(dx[96:], yl[96:])
+    torch.cuda.synchronize()
+    yr = torch.cat([R.forward(*ws[0], x[:96])[0][:, :6], R.forward(*ws[1], x[96:])[0][:, :6]])
+    scale = yr.abs().max()
+    assert (y.cpu() - yr).abs().max() <= 3e-3 * scale
+    assert (yl.cpu() - yr).abs().max() <= 3e-3 * scale
+    for n in nets:
+        n.close()
+
+
 def test_gradients_match_reference(gpu):
     net = _net(1024)
     Ws, bs = R.init_weights(21)
@@ -125,7 +148,7 @@ def test_training_fits_a_smooth_map(gpu):
     dt = tgt.cuda().contiguous()
     y = S.predict(net, dx)
     before = float(((y - dt) ** 2).mean())
-    S.train(net, dx, dt, S.TrainConfig(steps=600, batch=1024, lr=2e-3, warmup=50))
+    S.train(net, dx, dt, S.TrainConfig(steps=1500, batch=1024, lr=2e-3, warmup=50))
     y = S.predict(net, dx)
     after = float(((y - dt) ** 2).mean())
     assert after < 0.02 * before and after < 5e-3, (before, after)
@@ -267,7 +290,7 @@ def test_masked_compute_and_apply(gpu):
     torch.cuda.synchronize()
     m = mask.cpu().numpy().astype(bool)
     f, p = full.cpu().numpy(), part.cpu().numpy()
-    assert np.array_equal(p[m], f[m])          # deterministic: identical rows
+    assert np.array_equal(p[m, :22], f[m, :22])   # deterministic: identical rows (spare columns unwritten)
     assert np.all(p[~m] == 123.0)              # inactive rows untouched
     # decode: feed the exact normalised targets as network outputs
     raw = torch.zeros(batch.n, 13, dtype=torch.float32, device="cuda")
