@@ -101,6 +101,161 @@ def parity_sample(w, states, grid, out_gpu: np.ndarray, n: int = 96) -> dict:
             "norm_rel_err_max": {k: float(f"{v:.3e}") for k, v in plain.items()}}
 
 
+def _time_replays(fn, k: int) -> float:
+    """Seconds per call of fn over k calls (after 3 untimed), bracketed by syncs."""
+    import torch
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / k
+
+
+def surrogate_section(args) -> dict:
+    """Training workload (8 hulls of the fleet distribution, samples from the
+    exact kernel), held-out error, and exact vs surrogate body-steps/s on the
+    same hull set; MFMA rate of the fused inference and of training."""
+    import torch
+    from stonefish_amd import surrogate as S
+    t0 = time.time()
+    cfg = S.TrainConfig(steps=args.sur_train_steps, batch=4096, lr=2e-3, warmup=200)
+    r = S.run_training_workload(8, args.sur_per_hull, args.sur_steps, args.sur_holdout, cfg, seed=17, log=log)
+    hulls, nets, norms, w, ocean = r["hulls"], r["nets"], r["norms"], r["workload"], r["ocean"]
+    ds = r["dataset"]
+    log(f"surrogate training workload in {time.time() - t0:.1f}s")
+    keys = ("F_median", "F_p90", "F_p99", "T_median", "T_p90", "T_p99", "F_within_1pct", "T_within_1pct")
+    n_tot = sum(h["n"] for h in hulls)
+    agg = {k: round(float(sum(h[k] * h["n"] for h in hulls) / max(1, n_tot)), 5) for k in keys}
+    # exact vs surrogate on 8 x 512 bodies of the same hulls
+    per = 512
+    batch, meshes = S.make_hull_batch(w.bodies, per)
+    nb = 8 * per
+    rng = np.random.default_rng(99)
+    states = torch.from_numpy(S.sample_states(rng, nb, (0.0, w.sea.wave_len), (-1.0, 3.0))).cuda()
+    out = torch.zeros((nb, 24), dtype=torch.float64, device="cuda")
+    gs = torch.cuda.Stream()
+    with torch.cuda.stream(gs):
+        batch.capture(ocean, states.data_ptr(), out.data_ptr(), gs.cuda_stream)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream().cuda_stream
+    t_exact = _time_replays(lambda: batch.replay(stream), 30)
+    step = S.SurrogateStep(batch, ocean, nets, norms, [per] * 8, states)
+    step.capture()
+    t_sur = _time_replays(step.replay, 200)
+    # the surrogate's error on this sample against the exact outputs
+    err = torch.zeros((nb, 2), dtype=torch.float32, device="cuda")
+    ft = torch.zeros((nb, 6), dtype=torch.float64, device="cuda")
+    for h in range(8):
+        sl = slice(h * per, (h + 1) * per)
+        S.apply(batch, states, h * per, per, norms[h], step.y[sl], ft[sl], pos=step.pos[sl], exact=out, err=err[sl])
+    sample_err = S.error_summary(err.cpu().numpy(), step.ood.cpu().numpy().astype(bool), step.pos.cpu().numpy())
+    # MFMA rate of the fused inference at a large batch (all hulls' held-out inputs, one launch)
+    xs = [ds.inputs(h, norms[h], range(ds.train_steps, ds.steps))[0] for h in range(8)]
+    x = torch.cat(xs).contiguous()
+    y = torch.empty((x.shape[0], 6), dtype=torch.float32, device="cuda")
+    counts = [xx.shape[0] for xx in xs]
+    nets[0].timing(True)
+    for _ in range(20):
+        S.forward_group(nets, counts, x, y)
+    ms, flops, cnt = nets[0].kernel_ms()
+    nets[0].timing(False)
+    inf_tflops = flops / (ms * 1e-3) / 1e12
+    train_tflops = float(np.mean([h["train_tflops"] for h in hulls]))
+    res = {
+        "workload": f"8 hulls x {args.sur_per_hull} bodies x {args.sur_steps} steps of exact-kernel samples "
+                    f"({8 * args.sur_per_hull * args.sur_steps} total, {args.sur_holdout} steps held out), "
+                    f"{args.sur_train_steps} Adam steps of 4096 per hull",
+        "held_out_error": agg,
+        "per_hull": [{k: h[k] for k in ("hull", "faces", "train_samples", "F_median", "F_p90", "T_median",
+                                        "T_p90", "n", "n_ood", "n_out_of_water", "final_loss", "train_seconds")}
+                     for h in hulls],
+        "throughput_bodies": nb,
+        "exact_body_steps_per_s": round(nb / t_exact, 1),
+        "surrogate_body_steps_per_s": round(nb / t_sur, 1),
+        "speedup": round(t_exact / t_sur, 2),
+        "throughput_sample_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in sample_err.items()},
+        "mfma": {"inference_tflops": round(inf_tflops, 1), "inference_rows": int(x.shape[0]),
+                 "inference_kernel_ms": round(ms, 4), "training_tflops": round(train_tflops, 1),
+                 "peak_tflops_bf16_dense": 2500.0, "inference_frac": round(inf_tflops / 2500.0, 4),
+                 "training_frac": round(train_tflops / 2500.0, 4)},
+        "seconds": round(time.time() - t0, 1),
+    }
+    return res
+
+
+def rollout_section(args, sea) -> dict:
+    """Mixed rollout: AUV compounds (8,192-triangle hull + four 1,024-triangle
+    fairings) at 1 kHz on the wave field, surrogate for every body, exact
+    fallback outside the training range, exact audit every 100 steps."""
+    import torch
+    from stonefish_amd import hydro as H
+    from stonefish_amd import surrogate as S
+    from stonefish_amd import workloads as WL
+    t0 = time.time()
+    n = args.rollout_bodies
+    auv = WL.auv_compound()
+    meshes = {}
+    parts = []
+    for p in auv.parts:
+        if id(p.mesh) not in meshes:
+            meshes[id(p.mesh)] = H.Mesh(p.mesh)
+        parts.append(H.PartDesc(mesh=meshes[id(p.mesh)], origin=p.origin, Cd=p.Cd, Cf=p.Cf))
+    body = H.BodyDesc(mesh=None, aabb_min=auv.aabb_min, aabb_max=auv.aabb_max, volume=auv.volume,
+                      surface=auv.surface, P_CB=auv.P_CB)
+    batch = H.CompoundBatch([H.CompoundDesc(body, parts)] * n)
+    _, _, nfaces = batch.info()
+    ocean, grid = S.make_ocean(sea)
+    ds = S.HullDataset(batch, ocean, 1, n, steps=args.rollout_data_steps, holdout_steps=2, seed=31,
+                       xy_range=(0.0, sea.wave_len), depth_range=(-1.0, 3.0))
+    norm = ds.fit_norm(0)
+    x, t, _ = ds.inputs(0, norm, range(ds.train_steps))
+    net = S.Net(65536, seed=7)
+    hist = S.train(net, x, t, S.TrainConfig(steps=args.sur_train_steps, batch=4096, lr=2e-3, warmup=200))
+    held = S.evaluate(ds, 0, net, norm)
+    log(f"rollout surrogate trained ({ds.gen_seconds:.1f}s data, {hist['seconds']:.1f}s train): "
+        f"F median {held['F_median']:.4f}")
+    rng = np.random.default_rng(77)
+    states = torch.from_numpy(S.sample_states(rng, n, (0.0, sea.wave_len), (0.0, 2.5), v_max=1.0,
+                                              w_max=0.5)).cuda()
+    # exact kernel alone on the same bodies (graph replay of one evaluation)
+    out = torch.zeros((n, 24), dtype=torch.float64, device="cuda")
+    gs = torch.cuda.Stream()
+    with torch.cuda.stream(gs):
+        batch.capture(ocean, states.data_ptr(), out.data_ptr(), gs.cuda_stream)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream().cuda_stream
+    t_exact = _time_replays(lambda: batch.replay(stream), 10)
+    ro = S.MixedRollout(batch, ocean, net, norm, states, auv.mass, auv.inertia, dt=1e-3, audit_every=100)
+    for _ in range(3):
+        ro.step()
+    torch.cuda.synchronize()
+    ro.audits.clear()
+    ro.k = 0
+    steps = args.rollout_steps
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        ro.step()
+    torch.cuda.synchronize()
+    t_ro = (time.perf_counter() - t1) / steps
+    summ = ro.audit_summary()
+    ood_frac = float(torch.cat([a[2] for a in ro.audits]).float().mean()) if ro.audits else None
+    return {
+        "workload": f"{n} AUV compounds (8,192 + 4 x 1,024 triangles, {nfaces} faces), 1 kHz, "
+                    f"1024 x 1024 PM wave grid, audit every 100 steps, {steps} steps",
+        "body_steps_per_s": round(n / t_ro, 1),
+        "ms_per_step": round(1e3 * t_ro, 4),
+        "exact_body_steps_per_s": round(n / t_exact, 1),
+        "speedup_vs_exact": round(t_exact / t_ro, 2),
+        "audit_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in summ.items()},
+        "ood_fraction_at_audits": ood_frac,
+        "held_out_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in held.items()},
+        "seconds": round(time.time() - t0, 1),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +268,16 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1, help="replay a captured HIP graph per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate and rollout sections")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 256 MiB device copy (reference dispatch for HBM byte counters)")
+    ap.add_argument("--sur-per-hull", type=int, default=8192)
+    ap.add_argument("--sur-steps", type=int, default=31)
+    ap.add_argument("--sur-holdout", type=int, default=3)
+    ap.add_argument("--sur-train-steps", type=int, default=4000)
+    ap.add_argument("--rollout-bodies", type=int, default=16384)
+    ap.add_argument("--rollout-steps", type=int, default=300)
+    ap.add_argument("--rollout-data-steps", type=int, default=14)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -127,6 +292,12 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench needs a GPU"
     H.init(0)
     dev = torch.device("cuda:0")
+    if args.calibrate:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     w = build_fleet(args.bodies, args.seed)
     nf = np.array([b.mesh.nf for b in w.bodies], dtype=np.int64)
     nv = np.array([b.mesh.nv for b in w.bodies], dtype=np.int64)
@@ -225,6 +396,11 @@ def main() -> int:
         t0 = time.time()
         res["cpu_baseline"] = cpu_baseline(w, states_h[0], grid.cpu().numpy())
         log(f"cpu baseline in {time.time() - t0:.1f}s")
+    if not args.no_surrogate:
+        graphs = None
+        res["surrogate"] = surrogate_section(args)
+        res["surrogate"]["speedup_vs_fleet_exact"] = round(res["surrogate"]["surrogate_body_steps_per_s"] / value, 2)
+        res["rollout"] = rollout_section(args, w.sea)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -265,25 +265,28 @@ typedef struct {
  *   d_x        rows x 32 BF16 network inputs (rows >= count, a multiple of 32;
  *              rows past count are zero); needs norm
  *   d_ood      count bytes, 1 where a feature is outside [in_lo, in_hi] or not
- *              finite; needs norm
+ *              finite and the body is in the water; needs norm
+ *   d_pos      count int8: the body's position (CheckBodyFluidPosition on its
+ *              AABB: 0 outside, 1 inside, 2 crossing; -1 not hydrodynamic)
  * and, from the exact outputs d_exact (the batch's n x SFH_OUT_DOUBLES array;
  * fields whose written bit is clear count as zero):
  *   d_raw_target  count x 6 FP32 CG-frame total F, T
  *   d_target      count x 6 FP32 normalised targets; needs norm */
 int sfh_surrogate_features(sfh_batch batch, sfh_ocean ocean, const double* d_states, uint32_t first,
                            uint32_t count, uint32_t rows, const sfh_surrogate_norm* norm,
-                           float* d_raw, uint16_t* d_x, uint8_t* d_ood,
+                           float* d_raw, uint16_t* d_x, uint8_t* d_ood, int8_t* d_pos,
                            const double* d_exact, float* d_raw_target, float* d_target, void* stream);
 
 /* World-frame total force and torque (count x 6 doubles, d_ft) of bodies
  * first .. first+count-1 from the network outputs d_y (count x 6 FP32): bodies
- * flagged in d_ood (optional) take the exact totals from d_exact instead.
+ * flagged in d_ood (optional) take the exact totals from d_exact instead, and
+ * bodies whose d_pos (optional) is 0 or -1 get zero force and torque.
  * With d_exact and d_err (count x 2 FP32, optional), also the audit errors of
  * the network against the exact result: |F - F_exact| / |F_exact| and
  * |T - T_exact| / |T_exact| (0 when both are 0). */
 int sfh_surrogate_apply(sfh_batch batch, const double* d_states, uint32_t first, uint32_t count,
                         const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
-                        const double* d_exact, double* d_ft, float* d_err, void* stream);
+                        const int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err, void* stream);
 
 /* ---- rollout harness (no counterpart in the hydrodynamics path) ------------ */
 

@@ -216,6 +216,39 @@ __device__ __forceinline__ void cross64(const double* a, const double* b, double
 
 __device__ __forceinline__ f3 to_f3(const double* x) { return mk3((float)x[0], (float)x[1], (float)x[2]); }
 
+// CheckBodyFluidPosition (SolidEntity.cpp:1239-1261) on the world AABB
+// btCompoundShape::getAabb(T_CG) of the CG-frame AABB (centre c, half extents
+// h): 0 outside, 1 inside, 2 crossing.
+__device__ int classify64(const double* Rcg, const double* tcg, const double* ac, const double* ah, const OceanK& oc)
+{
+    double c[3], e[3], mn[3], mx[3], d[3];
+    for (int i = 0; i < 3; ++i) {
+        c[i] = (Rcg[3 * i] * ac[0] + Rcg[3 * i + 1] * ac[1] + Rcg[3 * i + 2] * ac[2]) + tcg[i];
+        e[i] = fabs(Rcg[3 * i]) * ah[0] + fabs(Rcg[3 * i + 1]) * ah[1] + fabs(Rcg[3 * i + 2]) * ah[2];
+        mn[i] = c[i] - e[i];
+        mx[i] = c[i] + e[i];
+        d[i] = mx[i] - mn[i];
+    }
+    int under = 0;
+    for (int k = 0; k < 8; ++k) {
+        double q[3];
+        if (k == 1) { q[0] = mx[0]; q[1] = mx[1]; q[2] = mx[2]; }
+        else {
+            // corners min, +x, +y, +xy, +z, +xz, +yz in the order of the reference
+            const int sx[8] = {0, 0, 1, 0, 1, 0, 1, 0};
+            const int sy[8] = {0, 0, 0, 1, 1, 0, 0, 1};
+            const int sz[8] = {0, 0, 0, 0, 0, 1, 1, 1};
+            q[0] = mn[0] + (sx[k] ? d[0] : 0.0);
+            q[1] = mn[1] + (sy[k] ? d[1] : 0.0);
+            q[2] = mn[2] + (sz[k] ? d[2] : 0.0);
+        }
+        double depth = q[2];
+        if (oc.nlayers) depth += wave_sum64(oc, q[0], q[1]);
+        if (depth > 0.0) ++under;
+    }
+    return under == 0 ? 0 : (under == 8 ? 1 : 2);
+}
+
 // ---------------------------------------------------------------------------
 // k_prologue: one lane per body
 // ---------------------------------------------------------------------------
@@ -260,34 +293,7 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     else if (flags & BF_MODE_SUBM) pos = 1;
     else if (!(flags & BF_HYDRO)) pos = -1;
     else {
-        // CheckBodyFluidPosition (SolidEntity.cpp:1239-1261) on the world AABB
-        // btCompoundShape::getAabb(T_CG) of the CG-frame AABB.
-        double c[3], e[3], mn[3], mx[3], d[3];
-        for (int i = 0; i < 3; ++i) {
-            c[i] = (Rcg[3 * i] * S.aabb_c[0] + Rcg[3 * i + 1] * S.aabb_c[1] + Rcg[3 * i + 2] * S.aabb_c[2]) + tcg[i];
-            e[i] = fabs(Rcg[3 * i]) * S.aabb_h[0] + fabs(Rcg[3 * i + 1]) * S.aabb_h[1] + fabs(Rcg[3 * i + 2]) * S.aabb_h[2];
-            mn[i] = c[i] - e[i];
-            mx[i] = c[i] + e[i];
-            d[i] = mx[i] - mn[i];
-        }
-        int under = 0;
-        for (int k = 0; k < 8; ++k) {
-            double q[3];
-            if (k == 1) { q[0] = mx[0]; q[1] = mx[1]; q[2] = mx[2]; }
-            else {
-                // corners min, +x, +y, +xy, +z, +xz, +yz in the order of the reference
-                const int sx[8] = {0, 0, 1, 0, 1, 0, 1, 0};
-                const int sy[8] = {0, 0, 0, 1, 1, 0, 0, 1};
-                const int sz[8] = {0, 0, 0, 0, 0, 1, 1, 1};
-                q[0] = mn[0] + (sx[k] ? d[0] : 0.0);
-                q[1] = mn[1] + (sy[k] ? d[1] : 0.0);
-                q[2] = mn[2] + (sz[k] ? d[2] : 0.0);
-            }
-            double depth = q[2];
-            if (oc.nlayers) depth += wave_sum64(oc, q[0], q[1]);
-            if (depth > 0.0) ++under;
-        }
-        pos = under == 0 ? 0 : (under == 8 ? 1 : 2);
+        pos = classify64(Rcg, tcg, S.aabb_c, S.aabb_h, oc);
         // body level: damping always on (Ocean::ApplyFluidForces, Ocean.cpp:248-252)
         flags |= BF_DAMPING;
     }
@@ -839,6 +845,14 @@ __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ 
     o[21] = (double)pos;
 }
 
+// Per row of the state array (a body, or a compound): what the surrogate's
+// feature build needs.
+struct UnitStatic {
+    double P_CB[3];
+    double aabb_c[3], aabb_h[3];
+    uint32_t hydro, pad_;
+};
+
 // Per compound: static data of Compound::ComputeHydrodynamicForces.
 struct CompStatic {
     uint32_t part0, nparts;   // the compound's parts are bodies part0 .. part0+nparts-1 of the batch
@@ -1016,7 +1030,7 @@ struct sfh_batch_t {
     bool compound = false;
     CompStatic* d_comp = nullptr;
     double* d_part_out = nullptr;
-    double* d_unit_pcb = nullptr;   // P_CB of every unit (CG frame), for the surrogate features
+    UnitStatic* d_units = nullptr;  // per unit: P_CB and AABB (CG frame), for the surrogate features
     uint32_t nchunks = 0;
     uint64_t nfaces = 0;
     uint32_t K = 8;
@@ -1257,8 +1271,8 @@ static void batch_free(sfh_batch_t* b)
     hipFree(b->d_partials);
     hipFree(b->d_comp);
     hipFree(b->d_part_out);
-    hipFree(b->d_unit_pcb);
-    b->d_comp = nullptr; b->d_part_out = nullptr; b->d_unit_pcb = nullptr;
+    hipFree(b->d_units);
+    b->d_comp = nullptr; b->d_part_out = nullptr; b->d_units = nullptr;
     for (sfh_mesh_t* m : b->meshes) mesh_release(m);
     b->meshes.clear();
 }
@@ -1360,11 +1374,24 @@ static BodyStatic make_static(const sfh_mesh_t* m, uint32_t flags)
     return s;
 }
 
-static int upload_units(sfh_batch_t* b, const std::vector<double>& pcb)
+static UnitStatic make_unit(const sfh_body_desc& d)
 {
-    HIPCHK(hipMalloc(&b->d_unit_pcb, sizeof(double) * std::max<size_t>(3, pcb.size())));
-    if (!pcb.empty())
-        HIPCHK(hipMemcpy(b->d_unit_pcb, pcb.data(), sizeof(double) * pcb.size(), hipMemcpyHostToDevice));
+    UnitStatic u;
+    memset(&u, 0, sizeof(u));
+    for (int k = 0; k < 3; ++k) {
+        u.P_CB[k] = d.P_CB[k];
+        u.aabb_c[k] = 0.5 * (d.aabb_max[k] + d.aabb_min[k]);
+        u.aabb_h[k] = 0.5 * (d.aabb_max[k] - d.aabb_min[k]);
+    }
+    u.hydro = d.hydro ? 1u : 0u;
+    return u;
+}
+
+static int upload_units(sfh_batch_t* b, const std::vector<UnitStatic>& units)
+{
+    HIPCHK(hipMalloc(&b->d_units, sizeof(UnitStatic) * std::max<size_t>(1, units.size())));
+    if (!units.empty())
+        HIPCHK(hipMemcpy(b->d_units, units.data(), sizeof(UnitStatic) * units.size(), hipMemcpyHostToDevice));
     return SFH_OK;
 }
 
@@ -1403,12 +1430,11 @@ extern "C" int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const s
     }
     b->K = 0;
     b->nunits = n;
-    std::vector<double> pcb(3 * (size_t)n);
-    for (uint32_t i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) pcb[3 * (size_t)i + k] = bodies[i].P_CB[k];
+    std::vector<UnitStatic> units(n);
+    for (uint32_t i = 0; i < n; ++i) units[i] = make_unit(bodies[i]);
     int rc = hipMalloc(&b->d_dyn, sizeof(BodyDyn) * std::max<size_t>(1, n)) == hipSuccess ? SFH_OK
              : fail(SFH_ERR_HIP, "batch alloc failed");
-    if (rc == SFH_OK) rc = upload_units(b, pcb);
+    if (rc == SFH_OK) rc = upload_units(b, units);
     if (rc == SFH_OK) rc = apply_opts(b, opts);
     if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
     *out = b;
@@ -1442,7 +1468,7 @@ extern "C" int sfh_compound_batch_create(const sfh_compound_desc* comps, uint32_
     b->compound = true;
     hipGetDevice(&b->device);
     std::vector<CompStatic> cs(n);
-    std::vector<double> pcb(3 * (size_t)n);
+    std::vector<UnitStatic> units(n);
     for (uint32_t c = 0; c < n; ++c) {
         const sfh_compound_desc& C = comps[c];
         CompStatic k;
@@ -1452,8 +1478,9 @@ extern "C" int sfh_compound_batch_create(const sfh_compound_desc* comps, uint32_
         k.flags = C.body.buoyant ? BF_BUOYANT : 0u;
         k.volume = C.body.volume;
         k.surface = C.body.surface;
-        for (int i = 0; i < 3; ++i) k.P_CB[i] = pcb[3 * (size_t)c + i] = C.body.P_CB[i];
+        for (int i = 0; i < 3; ++i) k.P_CB[i] = C.body.P_CB[i];
         cs[c] = k;
+        units[c] = make_unit(C.body);
         for (uint32_t p = 0; p < C.nparts; ++p) {
             const sfh_part_desc& P = C.parts[p];
             uint32_t flags = BF_PART;
@@ -1492,7 +1519,7 @@ extern "C" int sfh_compound_batch_create(const sfh_compound_desc* comps, uint32_
     if (rc == SFH_OK && n &&
         hipMemcpy(b->d_comp, cs.data(), sizeof(CompStatic) * n, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(SFH_ERR_HIP, "compound upload failed");
-    if (rc == SFH_OK) rc = upload_units(b, pcb);
+    if (rc == SFH_OK) rc = upload_units(b, units);
     if (rc == SFH_OK) rc = apply_opts(b, opts);
     if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
     *out = b;
@@ -1940,11 +1967,11 @@ __device__ void rot_to_quat(const double* R, double* q)
     q[0] = x * inv; q[1] = y * inv; q[2] = z * inv; q[3] = w * inv;
 }
 
-__global__ void k_sur_features(const double* __restrict__ unit_pcb, const double* __restrict__ states, uint32_t first,
-                               uint32_t count, uint32_t rows, OceanK oc, SurNorm nm, float* __restrict__ raw,
-                               uint16_t* __restrict__ x, uint8_t* __restrict__ ood,
-                               const double* __restrict__ exact, float* __restrict__ raw_target,
-                               float* __restrict__ target)
+__global__ void k_sur_features(const UnitStatic* __restrict__ units, const double* __restrict__ states,
+                               uint32_t first, uint32_t count, uint32_t rows, OceanK oc, SurNorm nm,
+                               float* __restrict__ raw, uint16_t* __restrict__ x, uint8_t* __restrict__ ood,
+                               int8_t* __restrict__ posout, const double* __restrict__ exact,
+                               float* __restrict__ raw_target, float* __restrict__ target)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows) return;
@@ -1963,8 +1990,11 @@ __global__ void k_sur_features(const double* __restrict__ unit_pcb, const double
     mat_t_vec(R, vr, f);
     mat_t_vec(R, s + 19, f + 3);
     rot_to_quat(R, f + 6);
+    const UnitStatic& U = units[b];
+    const int pos = U.hydro ? classify64(R, t, U.aabb_c, U.aabb_h, oc) : -1;
+    if (posout) posout[i] = (int8_t)pos;
     double cb[3];
-    mat_vec(R, unit_pcb + 3 * (size_t)b, cb);
+    mat_vec(R, U.P_CB, cb);
     for (int k = 0; k < 3; ++k) cb[k] += t[k];
     f[10] = cb[2];
     f[11] = f[12] = 0.0;
@@ -1982,6 +2012,7 @@ __global__ void k_sur_features(const double* __restrict__ unit_pcb, const double
             const float v = (float)f[k];
             if (!(v >= nm.in_lo[k] && v <= nm.in_hi[k])) o = 1;   // also catches NaN
         }
+        if (pos <= 0) o = 0;   // outside the water (or not hydrodynamic): the result is known
         ood[i] = o;
     }
     if (x) {
@@ -2008,7 +2039,8 @@ __global__ void k_sur_features(const double* __restrict__ unit_pcb, const double
 
 __global__ void k_sur_apply(const double* __restrict__ states, uint32_t first, uint32_t count, SurNorm nm,
                             const float* __restrict__ y, const uint8_t* __restrict__ ood,
-                            const double* __restrict__ exact, double* __restrict__ ft, float* __restrict__ err)
+                            const int8_t* __restrict__ pos, const double* __restrict__ exact,
+                            double* __restrict__ ft, float* __restrict__ err)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -2018,6 +2050,8 @@ __global__ void k_sur_apply(const double* __restrict__ states, uint32_t first, u
     for (int k = 0; k < 6; ++k) yb[k] = (double)(y[(size_t)i * 6 + k] * nm.out_scale[k] + nm.out_shift[k]);
     mat_vec(R, yb, Fs);
     mat_vec(R, yb + 3, Ts);
+    if (pos && pos[i] <= 0)   // outside the water or not hydrodynamic: no force
+        for (int k = 0; k < 3; ++k) Fs[k] = Ts[k] = 0.0;
     double Fe[3], Te[3];
     if (exact) exact_totals(exact + (size_t)b * SFH_OUT_DOUBLES, Fe, Te);
     const bool fallback = exact && ood && ood[i];
@@ -2059,8 +2093,8 @@ static SurNorm sur_norm(const sfh_surrogate_norm* n)
 
 extern "C" int sfh_surrogate_features(sfh_batch b, sfh_ocean o, const double* d_states, uint32_t first,
                                       uint32_t count, uint32_t rows, const sfh_surrogate_norm* norm, float* d_raw,
-                                      uint16_t* d_x, uint8_t* d_ood, const double* d_exact, float* d_raw_target,
-                                      float* d_target, void* stream)
+                                      uint16_t* d_x, uint8_t* d_ood, int8_t* d_pos, const double* d_exact,
+                                      float* d_raw_target, float* d_target, void* stream)
 {
     if (!b || !d_states) return fail(SFH_ERR_INVALID, "null argument");
     int rc = check_ocean(o);
@@ -2072,8 +2106,8 @@ extern "C" int sfh_surrogate_features(sfh_batch b, sfh_ocean o, const double* d_
     if (d_x && ((uintptr_t)d_x % 16)) return fail(SFH_ERR_INVALID, "d_x must be 16-byte aligned");
     const uint32_t r = d_x ? rows : count;
     if (r == 0) return SFH_OK;
-    hipLaunchKernelGGL(k_sur_features, dim3((r + 127) / 128), dim3(128), 0, (hipStream_t)stream, b->d_unit_pcb,
-                       d_states, first, count, r, make_oceank(o), sur_norm(norm), d_raw, d_x, d_ood, d_exact,
+    hipLaunchKernelGGL(k_sur_features, dim3((r + 127) / 128), dim3(128), 0, (hipStream_t)stream, b->d_units,
+                       d_states, first, count, r, make_oceank(o), sur_norm(norm), d_raw, d_x, d_ood, d_pos, d_exact,
                        d_raw_target, d_target);
     HIPCHK(hipGetLastError());
     return SFH_OK;
@@ -2081,13 +2115,14 @@ extern "C" int sfh_surrogate_features(sfh_batch b, sfh_ocean o, const double* d_
 
 extern "C" int sfh_surrogate_apply(sfh_batch b, const double* d_states, uint32_t first, uint32_t count,
                                    const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
-                                   const double* d_exact, double* d_ft, float* d_err, void* stream)
+                                   const int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err,
+                                   void* stream)
 {
     if (!b || !d_states || !norm || !d_y || !d_ft) return fail(SFH_ERR_INVALID, "null argument");
     if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "body range out of the batch");
     if (count == 0) return SFH_OK;
     hipLaunchKernelGGL(k_sur_apply, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_states, first,
-                       count, sur_norm(norm), d_y, d_ood, d_exact, d_ft, d_err);
+                       count, sur_norm(norm), d_y, d_ood, d_pos, d_exact, d_ft, d_err);
     HIPCHK(hipGetLastError());
     return SFH_OK;
 }

@@ -250,9 +250,9 @@ def test_normalised_inputs_and_ood(gpu):
     from stonefish_amd import surrogate as S
     batch, ocean, hulls, grid = _small_fleet(waves=False, n_hulls=1)
     rng = np.random.default_rng(8)
-    states = S.sample_states(rng, batch.n, xy_range=(0.0, 10.0))
+    states = S.sample_states(rng, batch.n, xy_range=(0.0, 10.0), depth_range=(0.0, 0.5))   # all in the water
     states[3, 16] = 50.0          # far outside any training range
-    states[5, 14] = float("nan")  # non-finite state
+    states[5, 17] = float("nan")  # non-finite velocity
     ds = torch.from_numpy(states).cuda()
     raw = torch.zeros(batch.n, 13, dtype=torch.float32, device="cuda")
     S.features(batch, ocean, ds, 0, batch.n, raw=raw)
@@ -268,7 +268,7 @@ def test_normalised_inputs_and_ood(gpu):
     o = ood.cpu().numpy()
     assert o[3] == 1 and o[5] == 1 and o[ok].sum() == 0
     expect = torch.from_numpy(((r[ok] - norm.in_shift) * norm.in_scale).astype(np.float32)).to(torch.bfloat16).float()
-    assert np.array_equal(xo[ok][:, :13], expect.numpy())
+    assert np.array_equal(xo[:batch.n][ok][:, :13], expect.numpy())
     assert np.all(xo[:, 13:] == 0) and np.all(xo[batch.n:] == 0)
 
 
@@ -279,7 +279,7 @@ def test_masked_compute_and_apply(gpu):
     rng = np.random.default_rng(12)
     states = S.sample_states(rng, batch.n, xy_range=(0.0, 256.0))
     ds = torch.from_numpy(states).cuda()
-    full = torch.zeros(batch.n, 24, dtype=torch.float64, device="cuda")
+    full = torch.full((batch.n, 24), 123.0, dtype=torch.float64, device="cuda")   # untouched outputs stay 123
     batch.compute(ocean, ds.data_ptr(), full.data_ptr())
     mask = torch.from_numpy((rng.random(batch.n) < 0.3).astype(np.uint8)).cuda()
     part = torch.full((batch.n, 24), 123.0, dtype=torch.float64, device="cuda")
@@ -327,7 +327,7 @@ def test_surrogate_beats_untrained_on_held_out(gpu):
     untrained = S.evaluate(ds, 0, net, norm)
     S.train(net, x, t, S.TrainConfig(steps=800, batch=1024, lr=2e-3, warmup=50))
     trained = S.evaluate(ds, 0, net, norm)
-    assert trained["n"] > 0.9 * 2 * 2048
+    assert trained["n"] >= 0.9 * (2 * 2048 - trained["n_out_of_water"] - trained["n_ood"]), trained
     assert trained["F_median"] < 0.1 * untrained["F_median"], (trained, untrained)
     assert trained["F_median"] < 0.05, trained
     net.close()
