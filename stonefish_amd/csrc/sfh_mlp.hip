@@ -23,8 +23,13 @@
 #include <vector>
 
 #include "../../include/sfmlp.h"
+#include "sfm_gelu.h"
+
+using sfm::gelu;
+using sfm::gelu_grad;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 static thread_local char g_err[512] = "";
@@ -55,22 +60,20 @@ static constexpr int kSplitK = 256;   // batch rows per weight-gradient split
 // numerics
 // ---------------------------------------------------------------------------
 
+// FP32 -> BF16 bits, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                                             // round to nearest even
-    return (uint16_t)(u >> 16);
+    const __bf16 h = (__bf16)f;
+    return *reinterpret_cast<const uint16_t*>(&h);
 }
 
-__device__ __forceinline__ float gelu(float z)
+// Two FP32 -> packed BF16 pair (low half = a).
+__device__ __forceinline__ uint32_t f2bf2(float a, float b)
 {
-    return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
-}
-
-__device__ __forceinline__ float gelu_grad(float z)
-{
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    bf16x2 v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    return *reinterpret_cast<const uint32_t*>(&v);
 }
 
 // ---------------------------------------------------------------------------
@@ -188,11 +191,16 @@ static constexpr int kActStride = SFM_HIDDEN + 8;   // LDS row stride (elements)
 
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
-// One workgroup (4 wavefronts) runs TR consecutive rows of one network through
-// the whole network: the hidden activations stay in LDS (two TR x 256 BF16
-// buffers), wavefront w computes output columns [64 w, 64 w + 64) of each
-// hidden layer from weights read straight from global memory (L2-resident,
-// 400 KB per network), and the output layer is split over the row tiles.
+// One workgroup (4 wavefronts) runs TR consecutive rows (samples) of one
+// network through the whole network; the hidden activations stay in LDS (two
+// TR x 256 BF16 buffers, row-major [sample][feature]).  Each layer is computed
+// transposed, H'^T = W H^T: the weights are the MFMA's A operand (16-byte
+// loads of W[feature][k..k+7] from global memory, L2-resident, 405 KB per
+// network) and the activations its B operand (16-byte LDS reads of
+// H[sample][k..k+7]), so an accumulator lane holds four consecutive output
+// features of one sample and stores them with one 8-byte LDS write.
+// Wavefront w owns output features [64 w, 64 w + 64) of each hidden layer;
+// the output layer is one 32-feature tile per 32-sample tile.
 template <int TR>
 __global__ __launch_bounds__(256) void k_mlp_fused(NetSet ns, const uint16_t* __restrict__ x, uint32_t rows,
                                                    float* __restrict__ y)
@@ -204,7 +212,7 @@ __global__ __launch_bounds__(256) void k_mlp_fused(NetSet ns, const uint16_t* __
     if (row0 >= rows) return;
     int k = 0;
     while (k + 1 < ns.nnets && row0 >= ns.row_end[k]) ++k;
-    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
 
     for (int l = 0; l < SFM_LAYERS - 1; ++l) {
         const int K = (l == 0) ? SFM_IN_PAD : SFM_HIDDEN;
@@ -212,34 +220,42 @@ __global__ __launch_bounds__(256) void k_mlp_fused(NetSet ns, const uint16_t* __
         const float* bias = ns.b[k][l];
         const uint16_t* in = act[(l + 1) & 1];
         uint16_t* out = act[l & 1];
-        f32x16 acc[RT][2];
+        f32x16 acc[2][RT];
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) { acc[rt][0] = f32x16{}; acc[rt][1] = f32x16{}; }
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < RT; ++nt) acc[mt][nt] = f32x16{};
         const uint16_t* wp0 = W + (size_t)(64 * wave + lr) * K + lk;
         const uint16_t* wp1 = wp0 + (size_t)32 * K;
         for (int ks = 0; ks < K; ks += 16) {
-            const bf16x8 b0 = ld8(wp0 + ks), b1 = ld8(wp1 + ks);
+            const bf16x8 a0 = ld8(wp0 + ks), a1 = ld8(wp1 + ks);
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                const bf16x8 a = (l == 0) ? ld8(x + (size_t)(row0 + 32 * rt + lr) * SFM_IN_PAD + ks + lk)
-                                          : ld8(in + (32 * rt + lr) * kActStride + ks + lk);
-                acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[rt][0], 0, 0, 0);
-                acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[rt][1], 0, 0, 0);
+            for (int nt = 0; nt < RT; ++nt) {
+                const bf16x8 b = (l == 0) ? ld8(x + (size_t)(row0 + 32 * nt + lr) * SFM_IN_PAD + ks + lk)
+                                          : ld8(in + (32 * nt + lr) * kActStride + ks + lk);
+                acc[0][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][nt], 0, 0, 0);
+                acc[1][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][nt], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            const int col = 64 * wave + 32 * nt + lr;
-            const float bc = bias[col];
+        for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
+            for (int q = 0; q < 4; ++q) {
+                const int f0 = 64 * wave + 32 * mt + 8 * q + lq;   // four consecutive output features
+                const float4 bq = *reinterpret_cast<const float4*>(bias + f0);
 #pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    out[(32 * rt + acc_row(r, lane)) * kActStride + col] = f2bf(gelu(acc[rt][nt][r] + bc));
+                for (int nt = 0; nt < RT; ++nt) {
+                    const f32x16& A = acc[mt][nt];
+                    uint2 pk;
+                    pk.x = f2bf2(gelu(A[4 * q] + bq.x), gelu(A[4 * q + 1] + bq.y));
+                    pk.y = f2bf2(gelu(A[4 * q + 2] + bq.z), gelu(A[4 * q + 3] + bq.w));
+                    *reinterpret_cast<uint2*>(out + (32 * nt + lr) * kActStride + f0) = pk;
+                }
+            }
         }
         __syncthreads();
     }
-    // output layer: one 32 x 32 tile per row tile, K = 256, from act[1]
+    // output layer: features 0..31 (6 real) of sample tile `wave`, K = 256, from act[1]
     if (wave < RT) {
         const uint16_t* W = ns.W[k][SFM_LAYERS - 1];
         const float* bias = ns.b[k][SFM_LAYERS - 1];
@@ -247,14 +263,13 @@ __global__ __launch_bounds__(256) void k_mlp_fused(NetSet ns, const uint16_t* __
         f32x16 acc = {};
         const uint16_t* wp = W + (size_t)lr * SFM_HIDDEN + lk;
         for (int ks = 0; ks < SFM_HIDDEN; ks += 16)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(in + (32 * wave + lr) * kActStride + ks + lk),
-                                                          ld8(wp + ks), acc, 0, 0, 0);
-        if (lr < SFM_OUT) {
-            const float bc = bias[lr];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(wp + ks), ld8(in + (32 * wave + lr) * kActStride + ks + lk),
+                                                          acc, 0, 0, 0);
+        // lane: features lq .. lq+3 (registers 0..3) of sample 32 wave + lr
+        float* yo = y + (size_t)(row0 + 32 * wave + lr) * SFM_OUT;
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                y[(size_t)(row0 + 32 * wave + acc_row(r, lane)) * SFM_OUT + lr] = acc[r] + bc;
-        }
+        for (int i = 0; i < 4; ++i)
+            if (lq + i < SFM_OUT) yo[lq + i] = acc[i] + bias[lq + i];
     }
 }
 

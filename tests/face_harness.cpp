@@ -83,3 +83,14 @@ extern "C" void harness_body(int mode, const float* verts, const uint32_t* faces
         }
     }
 }
+
+// Host build of the surrogate's GELU (stonefish_amd/csrc/sfm_gelu.h).
+#include "../stonefish_amd/csrc/sfm_gelu.h"
+
+extern "C" void harness_gelu(const float* z, uint32_t n, float* g, float* dg)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        g[i] = sfm::gelu(z[i]);
+        dg[i] = sfm::gelu_grad(z[i]);
+    }
+}

@@ -102,3 +102,22 @@ def test_fleet_crossing_host(harness):
         x.append(e)
     assert len(g) > 10
     check(np.stack(g), np.stack(r), np.stack(x), ("Fb", "Tb", "Fdq", "Tdq", "Fdf", "Tdf"))
+
+
+def test_gelu_against_erf(harness):
+    """The surrogate's GELU (erf by Abramowitz-Stegun 7.1.26, |error| <= 1.5e-7)
+    against the exact erf in FP64: value and derivative within 5e-7 of the
+    scale (single-precision rounding included)."""
+    from math import erf, exp, pi, sqrt
+    z = np.concatenate([np.linspace(-8, 8, 4001), [0.0, -0.0, 1e-6, -1e-6, 30.0, -30.0]]).astype(np.float32)
+    g = np.zeros_like(z)
+    dg = np.zeros_like(z)
+    fp = C.POINTER(C.c_float)
+    harness.harness_gelu.argtypes = [fp, C.c_uint32, fp, fp]
+    harness.harness_gelu(z.ctypes.data_as(fp), z.size, g.ctypes.data_as(fp), dg.ctypes.data_as(fp))
+    zd = z.astype(np.float64)
+    ge = np.array([0.5 * t * (1 + erf(t / sqrt(2))) for t in zd])
+    de = np.array([0.5 * (1 + erf(t / sqrt(2))) + t * exp(-0.5 * t * t) / sqrt(2 * pi) for t in zd])
+    assert np.all(np.abs(g - ge) <= 5e-7 * np.maximum(1.0, np.abs(zd)))
+    assert np.all(np.abs(dg - de) <= 5e-7 * np.maximum(1.0, np.abs(zd)))
+    assert g[np.argmin(np.abs(zd - 30))] == np.float32(30.0) and g[np.argmin(np.abs(zd + 30))] == 0.0
