@@ -2,13 +2,10 @@
 // hydrodynamics (SolidEntity.cpp:1239-1850, Ocean.cpp:149-201,
 // OpenGLRealOcean.cpp:280-331, Compound-free body path).
 //
-// One evaluation pass over a batch of bodies is six launches on one stream:
+// One evaluation pass over a batch of bodies is five launches on one stream:
 //   k_prologue  one lane per body, FP64: C transform, body/fluid classification
 //               (8 AABB corners against the surface), per-body FP32 constants of
 //               the face loop in the mesh frame, FP64 constants of the epilogue;
-//   k_classify  one workgroup: this step's work lists (chunks of submerged and
-//               of crossing bodies, vertex chunks of crossing bodies), in body
-//               order;
 //   k_depths    FP64 depth of every vertex of the bodies crossing the surface;
 //   k_faces_sub / k_faces_cross
 //               one wavefront per chunk of up to 64*K faces of one body: FP32
@@ -258,13 +255,15 @@ __device__ int classify64(const double* Rcg, const double* tcg, const double* ac
 // active (optional): bodies with active[b] == 0 get position -2 and are left
 // out of the step entirely.
 __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
-                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active, BodyDyn* __restrict__ dyn)
+                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active, BodyDyn* __restrict__ dyn,
+                           int8_t* __restrict__ bpos)
 {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
     const BodyStatic& S = st[b];
     if (active && !active[S.sidx]) {
         dyn[b].pos = -2;
+        bpos[b] = -2;
         return;
     }
     BodyDyn D;
@@ -313,6 +312,7 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     }
     D.pos = pos;
     D.flags = flags;
+    bpos[b] = (int8_t)pos;
 
     // mesh-frame FP32 constants
     double x[3], y[3], vl[3], wl[3], ul[3], wd[3];
@@ -363,6 +363,83 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
 struct Chunk {
     uint32_t body, first, count, pad;
 };
+
+// Per-step work lists.  k_prologue writes every body's position to bpos;
+// k_scan (one workgroup) turns the positions into each body's offsets in the
+// three lists (chunks of inside bodies, chunks of crossing bodies, vertex
+// chunks of crossing bodies) by an exclusive prefix sum in body order, and
+// k_scatter (one wavefront per body, grid-strided) writes the chunk ids.  The
+// passes then take list entries w, w + W, w + 2W, ... (W wavefronts), so every
+// wavefront gets the same number of chunks of its kind.
+static constexpr int kScanThreads = 1024;
+
+struct BodyChunks {
+    uint32_t chunk0, nchunks, vchunk0, nvchunks;
+};
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(const int8_t* __restrict__ bpos,
+                                                       const BodyChunks* __restrict__ bch, uint32_t n,
+                                                       uint4* __restrict__ boff, uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t wsum[3][kScanThreads / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
+    const uint32_t b0 = min(t * per, n), b1 = min(b0 + per, n);
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const int p = bpos[b];
+        const BodyChunks bc = bch[b];
+        c0 += (p == 1) ? bc.nchunks : 0u;
+        c1 += (p == 2) ? bc.nchunks : 0u;
+        c2 += (p == 2) ? bc.nvchunks : 0u;
+    }
+    uint32_t i0 = c0, i1 = c1, i2 = c2;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y0 = __shfl_up(i0, off, 64), y1 = __shfl_up(i1, off, 64), y2 = __shfl_up(i2, off, 64);
+        if ((int)lane >= off) { i0 += y0; i1 += y1; i2 += y2; }
+    }
+    if (lane == 63) { wsum[0][w] = i0; wsum[1][w] = i1; wsum[2][w] = i2; }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t r0 = 0, r1 = 0, r2 = 0;
+        for (int i = 0; i < kScanThreads / 64; ++i) {
+            const uint32_t x0 = wsum[0][i], x1 = wsum[1][i], x2 = wsum[2][i];
+            wsum[0][i] = r0; wsum[1][i] = r1; wsum[2][i] = r2;
+            r0 += x0; r1 += x1; r2 += x2;
+        }
+        counts[0] = r0; counts[1] = r1; counts[2] = r2;
+    }
+    __syncthreads();
+    uint32_t o0 = wsum[0][w] + i0 - c0, o1 = wsum[1][w] + i1 - c1, o2 = wsum[2][w] + i2 - c2;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const int p = bpos[b];
+        const BodyChunks bc = bch[b];
+        boff[b] = make_uint4(o0, o1, o2, 0u);
+        o0 += (p == 1) ? bc.nchunks : 0u;
+        o1 += (p == 2) ? bc.nchunks : 0u;
+        o2 += (p == 2) ? bc.nvchunks : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const int8_t* __restrict__ bpos, const BodyChunks* __restrict__ bch,
+                                                 const uint4* __restrict__ boff, uint32_t n,
+                                                 uint32_t* __restrict__ list_sub, uint32_t* __restrict__ list_cross,
+                                                 uint32_t* __restrict__ list_vert)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t b = wave; b < n; b += nwaves) {
+        const int p = bpos[b];
+        if (p != 1 && p != 2) continue;
+        const BodyChunks bc = bch[b];
+        const uint4 o = boff[b];
+        uint32_t* L = (p == 1) ? list_sub + o.x : list_cross + o.y;
+        for (uint32_t i = lane; i < bc.nchunks; i += 64) L[i] = bc.chunk0 + i;
+        if (p == 2)
+            for (uint32_t i = lane; i < bc.nvchunks; i += 64) list_vert[o.z + i] = bc.vchunk0 + i;
+    }
+}
 
 // Near the waterline a depth is a small difference of body-scale values, and
 // the clip points of thin wet slivers are built from it, so it is evaluated in
@@ -499,7 +576,7 @@ __device__ __forceinline__ int xreduce12_slot(int lane)
 // Submerged loop (bodies fully below the surface): per-face arithmetic in FP32
 // with FP32 per-lane partial sums over the chunk's K faces per lane, reduced
 // across the wavefront and across chunks in FP64.  `list` holds the ids of the
-// chunks of this step's submerged bodies (built by k_classify).
+// chunks of this step's inside bodies.
 template <int K>
 __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ count,
@@ -553,7 +630,8 @@ __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ 
 }
 
 // Surface loop (bodies crossing the surface): waterline clip with the vertex
-// depths of k_depths, tetra volumes, pressure buoyancy with waves, drag.
+// depths of k_depths, tetra volumes, pressure buoyancy with waves, drag; the
+// chunks of this step's crossing bodies.
 template <int K>
 __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
@@ -633,68 +711,6 @@ __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// k_classify: deterministic per-step work lists (one workgroup)
-// ---------------------------------------------------------------------------
-
-// After k_prologue has set each body's position, writes the ids of the face
-// chunks of submerged bodies, of crossing bodies, and of the vertex chunks of
-// crossing bodies, in body order, with their counts in counts[0..2].
-static constexpr int kClassifyThreads = 1024;
-
-__global__ __launch_bounds__(kClassifyThreads) void k_classify(const BodyStatic* __restrict__ st,
-                                                               const BodyDyn* __restrict__ dyn, uint32_t n,
-                                                               uint32_t* __restrict__ list_sub,
-                                                               uint32_t* __restrict__ list_cross,
-                                                               uint32_t* __restrict__ list_vert,
-                                                               uint32_t* __restrict__ counts)
-{
-    __shared__ uint32_t wsum[3][kClassifyThreads / 64];
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t per = (n + kClassifyThreads - 1) / kClassifyThreads;
-    const uint32_t b0 = min(t * per, n), b1 = min(b0 + per, n);
-    uint32_t c[3] = {0, 0, 0};
-    for (uint32_t b = b0; b < b1; ++b) {
-        const int pos = dyn[b].pos;
-        const uint32_t nc = st[b].nchunks, nvc = st[b].nvchunks;
-        if (pos == 1) c[0] += nc;
-        if (pos == 2) { c[1] += nc; c[2] += nvc; }
-    }
-    // exclusive scan over threads: wave scan, then over wave totals
-    uint32_t incl[3] = {c[0], c[1], c[2]};
-    for (int off = 1; off < 64; off <<= 1)
-        for (int k = 0; k < 3; ++k) {
-            uint32_t y = __shfl_up(incl[k], off, 64);
-            if ((int)lane >= off) incl[k] += y;
-        }
-    if (lane == 63)
-        for (int k = 0; k < 3; ++k) wsum[k][w] = incl[k];
-    __syncthreads();
-    if (t == 0) {
-        uint32_t run[3] = {0, 0, 0};
-        for (int i = 0; i < kClassifyThreads / 64; ++i)
-            for (int k = 0; k < 3; ++k) { uint32_t x = wsum[k][i]; wsum[k][i] = run[k]; run[k] += x; }
-        for (int k = 0; k < 3; ++k) counts[k] = run[k];
-    }
-    __syncthreads();
-    uint32_t o[3];
-    for (int k = 0; k < 3; ++k) o[k] = wsum[k][w] + incl[k] - c[k];
-    for (uint32_t b = b0; b < b1; ++b) {
-        const int pos = dyn[b].pos;
-        const uint32_t c0 = st[b].chunk0, nc = st[b].nchunks, v0 = st[b].vchunk0, nvc = st[b].nvchunks;
-        if (pos == 1) {
-            for (uint32_t i = 0; i < nc; ++i) list_sub[o[0] + i] = c0 + i;
-            o[0] += nc;
-        }
-        if (pos == 2) {
-            for (uint32_t i = 0; i < nc; ++i) list_cross[o[1] + i] = c0 + i;
-            for (uint32_t i = 0; i < nvc; ++i) list_vert[o[2] + i] = v0 + i;
-            o[1] += nc;
-            o[2] += nvc;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // k_epilogue: one wavefront per body
 // ---------------------------------------------------------------------------
 
@@ -730,19 +746,15 @@ __device__ void correct64(double rho, const double* BO, const double* Cd, const 
     }
 }
 
-__global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
-                                                 uint32_t n, OceanK oc, const double* __restrict__ partials,
-                                                 double* __restrict__ out)
+__device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t b,
+                              const OceanK& oc, const double* __restrict__ partials, double* __restrict__ out)
 {
-    const uint32_t b = blockIdx.x;
     const int lane = threadIdx.x;
-    if (b >= n) return;
     const BodyStatic& S = st[b];
     const BodyDyn& D = dyn[b];
     double* o = out + (size_t)b * SFH_OUT_DOUBLES;
     const int pos = D.pos;
     const uint32_t flags = D.flags;
-    if (pos == -2) return;   // inactive in a masked step: row not written
     if (pos < 0) {   // not a hydrodynamic body: nothing assigned
         if (lane == 0) { o[20] = 0.0; o[21] = -1.0; }
         return;
@@ -843,6 +855,16 @@ __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ 
     if (written & 128u) o[19] = Vsub;
     o[20] = (double)written;
     o[21] = (double)pos;
+}
+
+// One wavefront per body, bodies strided over the grid; inactive bodies of a
+// masked step are skipped after one byte load.
+__global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
+                                                 const int8_t* __restrict__ bpos, uint32_t n, OceanK oc,
+                                                 const double* __restrict__ partials, double* __restrict__ out)
+{
+    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x)
+        if (bpos[b] != -2) epilogue_body(st, dyn, b, oc, partials, out);
 }
 
 // Per row of the state array (a body, or a compound): what the surrogate's
@@ -1042,6 +1064,9 @@ struct sfh_batch_t {
     Chunk* d_chunks = nullptr;
     Chunk* d_vchunks = nullptr;
     float* d_depth = nullptr;
+    int8_t* d_bpos = nullptr;       // per body: this step's position (-2 inactive, -1 not hydrodynamic, 0, 1, 2)
+    BodyChunks* d_bch = nullptr;    // per body: its face and vertex chunk ranges
+    uint4* d_boff = nullptr;        // per body: this step's offsets in the three lists
     uint32_t* d_lists = nullptr;    // list_sub [nchunks], list_cross [nchunks], list_vert [nvchunks], counts [4]
     BodyDyn* d_dyn = nullptr;
     double* d_partials = nullptr;
@@ -1266,6 +1291,9 @@ static void batch_free(sfh_batch_t* b)
     hipFree(b->d_chunks);
     hipFree(b->d_vchunks);
     hipFree(b->d_depth);
+    hipFree(b->d_bpos);
+    hipFree(b->d_bch);
+    hipFree(b->d_boff);
     hipFree(b->d_lists);
     hipFree(b->d_dyn);
     hipFree(b->d_partials);
@@ -1326,9 +1354,12 @@ static int batch_build_chunks(sfh_batch_t* b)
     hipFree(b->d_vchunks);
     hipFree(b->d_depth);
     hipFree(b->d_partials);
+    hipFree(b->d_bpos);
+    hipFree(b->d_bch);
+    hipFree(b->d_boff);
     hipFree(b->d_lists);
     b->d_static = nullptr; b->d_chunks = nullptr; b->d_vchunks = nullptr; b->d_depth = nullptr;
-    b->d_partials = nullptr; b->d_lists = nullptr;
+    b->d_partials = nullptr; b->d_bpos = nullptr; b->d_bch = nullptr; b->d_boff = nullptr; b->d_lists = nullptr;
     b->nchunks = (uint32_t)chunks.size();
     b->nvchunks = (uint32_t)vchunks.size();
     b->nverts = dep;
@@ -1342,6 +1373,18 @@ static int batch_build_chunks(sfh_batch_t* b)
         HIPCHK(hipMemcpy(b->d_vchunks, vchunks.data(), sizeof(Chunk) * vchunks.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&b->d_depth, sizeof(float) * std::max<uint64_t>(1, dep)));
     HIPCHK(hipMalloc(&b->d_partials, sizeof(double) * kChunkDoubles * std::max<size_t>(1, chunks.size())));
+    HIPCHK(hipMalloc(&b->d_bpos, std::max<size_t>(1, b->n)));
+    {
+        std::vector<BodyChunks> bch(b->n);
+        for (uint32_t i = 0; i < b->n; ++i) {
+            const BodyStatic& st = b->h_static[i];
+            bch[i].chunk0 = st.chunk0; bch[i].nchunks = st.nchunks;
+            bch[i].vchunk0 = st.vchunk0; bch[i].nvchunks = st.nvchunks;
+        }
+        HIPCHK(hipMalloc(&b->d_bch, sizeof(BodyChunks) * std::max<size_t>(1, b->n)));
+        if (b->n) HIPCHK(hipMemcpy(b->d_bch, bch.data(), sizeof(BodyChunks) * b->n, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&b->d_boff, sizeof(uint4) * std::max<size_t>(1, b->n)));
     HIPCHK(hipMalloc(&b->d_lists, sizeof(uint32_t) * (2 * (size_t)b->nchunks + b->nvchunks + 8)));
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -1550,20 +1593,21 @@ extern "C" int sfh_batch_info(sfh_batch b, uint32_t* nbodies, uint32_t* nchunks,
     return SFH_OK;
 }
 
-// Launches the per-step work-list build, the depth pass and the two face
-// passes.
+// Launches the work-list build, the depth pass and the two face passes.
 static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool timed)
 {
     if (b->nchunks == 0) {
         if (timed) { HIPCHK(hipEventRecord(b->ev[1], s)); HIPCHK(hipEventRecord(b->ev[2], s)); }
         return SFH_OK;
     }
-    uint32_t* list_sub = b->d_lists;   // list_sub [nchunks], list_cross [nchunks], list_vert [nvchunks], counts [8]
+    uint32_t* list_sub = b->d_lists;
     uint32_t* list_cross = list_sub + b->nchunks;
     uint32_t* list_vert = list_cross + b->nchunks;
-    uint32_t* counts = list_vert + b->nvchunks;   // list sizes
-    hipLaunchKernelGGL(k_classify, dim3(1), dim3(kClassifyThreads), 0, s, b->d_static, b->d_dyn, b->n, list_sub,
-                       list_cross, list_vert, counts);
+    uint32_t* counts = list_vert + b->nvchunks;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_boff, counts);
+    const uint32_t swaves = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
+    hipLaunchKernelGGL(k_scatter, dim3((swaves + 3) / 4), dim3(256), 0, s, b->d_bpos, b->d_bch, b->d_boff, b->n,
+                       list_sub, list_cross, list_vert);
     if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
     uint32_t wv = std::min<uint32_t>(b->waves, std::max<uint32_t>(1, b->nvchunks));
     hipLaunchKernelGGL(k_depths, dim3((wv + 3) / 4), dim3(256), 0, s, list_vert, counts + 2, b->d_vchunks,
@@ -1571,8 +1615,8 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
     if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
     uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
     uint32_t blocks = (waves + 3) / 4;
-#define LAUNCH_K(KK)                                                                                     \
-    case KK:                                                                                             \
+#define LAUNCH_K(KK)                                                                                      \
+    case KK:                                                                                              \
         hipLaunchKernelGGL(k_faces_sub<KK>, dim3(blocks), dim3(256), 0, s, list_sub, counts, b->d_chunks, \
                            b->d_static, b->d_dyn, b->d_partials);                                       \
         hipLaunchKernelGGL(k_faces_cross<KK>, dim3(blocks), dim3(256), 0, s, list_cross, counts + 1,     \
@@ -1594,12 +1638,13 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
     uint32_t blocks = (b->n + 127) / 128;
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
     hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, active,
-                       b->d_dyn);
+                       b->d_dyn, b->d_bpos);
     int rc = launch_faces(b, oc, s, timed);
     if (rc != SFH_OK) return rc;
     if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
-    hipLaunchKernelGGL(k_epilogue, dim3(b->n), dim3(64), 0, s, b->d_static, b->d_dyn, b->n, oc, b->d_partials,
-                       b->compound ? b->d_part_out : d_out);
+    const uint32_t eblocks = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
+    hipLaunchKernelGGL(k_epilogue, dim3(eblocks), dim3(64), 0, s, b->d_static, b->d_dyn, b->d_bpos, b->n, oc,
+                       b->d_partials, b->compound ? b->d_part_out : d_out);
     if (b->compound && b->nunits)
         hipLaunchKernelGGL(k_compound, dim3((b->nunits + 127) / 128), dim3(128), 0, s, b->d_comp, b->d_static,
                            b->d_dyn, b->nunits, oc, active, b->d_part_out, d_out);
