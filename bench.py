@@ -150,7 +150,8 @@ def surrogate_section(args) -> dict:
     ft = torch.zeros((nb, 6), dtype=torch.float64, device="cuda")
     for h in range(8):
         sl = slice(h * per, (h + 1) * per)
-        S.apply(batch, states, h * per, per, norms[h], step.y[sl], ft[sl], pos=step.pos[sl], exact=out, err=err[sl])
+        S.apply(batch, ocean, states, h * per, per, norms[h], step.y[sl], ft[sl], pos=step.pos[sl], exact=out,
+                err=err[sl])
     sample_err = S.error_summary(err.cpu().numpy(), step.ood.cpu().numpy().astype(bool), step.pos.cpu().numpy())
     # MFMA rate of the fused inference at a large batch (all hulls' held-out inputs, one launch)
     xs = [ds.inputs(h, norms[h], range(ds.train_steps, ds.steps))[0] for h in range(8)]

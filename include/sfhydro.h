@@ -244,8 +244,13 @@ int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_
  *   10    depth of the centre of buoyancy below the local surface (Ocean::GetDepth)
  *   11-12 slope of the depth offset along world x and y (central difference
  *         over one texel of the first wave layer; 0 on a flat sea)
- * and predicts the total force and torque about the CG, CG frame:
- *   R^T (Fb + Fdq + Fdf), R^T (Tb + Tdq + Tdf) after CorrectHydrodynamicForces. */
+ * and predicts the total force and torque about the CG, CG frame,
+ *   F = R^T (Fb + Fdq + Fdf), T = R^T (Tb + Tdq + Tdf) after CorrectHydrodynamicForces,
+ * relative to a hydrostatic estimate (Fb0, Tb0: the analytic buoyancy of the
+ * immersed fraction of the body, P_CB x Fb0) and in units of the body's
+ * force and torque scales sF = rho g V + rho A/4 (u^3 + u), sT = sF Lc
+ * (u = |v_rel| + |omega| Lc, Lc the largest half extent of the CG-frame AABB):
+ *   network output = ((F - Fb0) / sF, (T - Tb0) / sT), then standardised. */
 #define SFH_SUR_FEATURES 13
 #define SFH_SUR_ROW 32        /* network input row: 13 features, zero-padded */
 
@@ -270,8 +275,8 @@ typedef struct {
  *              AABB: 0 outside, 1 inside, 2 crossing; -1 not hydrodynamic)
  * and, from the exact outputs d_exact (the batch's n x SFH_OUT_DOUBLES array;
  * fields whose written bit is clear count as zero):
- *   d_raw_target  count x 6 FP32 CG-frame total F, T
- *   d_target      count x 6 FP32 normalised targets; needs norm */
+ *   d_raw_target  count x 6 FP32 ((F - Fb0) / sF, (T - Tb0) / sT)
+ *   d_target      count x 6 FP32 the same standardised (network targets); needs norm */
 int sfh_surrogate_features(sfh_batch batch, sfh_ocean ocean, const double* d_states, uint32_t first,
                            uint32_t count, uint32_t rows, const sfh_surrogate_norm* norm,
                            float* d_raw, uint16_t* d_x, uint8_t* d_ood, int8_t* d_pos,
@@ -280,13 +285,14 @@ int sfh_surrogate_features(sfh_batch batch, sfh_ocean ocean, const double* d_sta
 /* World-frame total force and torque (count x 6 doubles, d_ft) of bodies
  * first .. first+count-1 from the network outputs d_y (count x 6 FP32): bodies
  * flagged in d_ood (optional) take the exact totals from d_exact instead, and
- * bodies whose d_pos (optional) is 0 or -1 get zero force and torque.
+ * bodies out of the water (position 0) or not hydrodynamic get zero force and
+ * torque; d_pos (optional) receives the positions.
  * With d_exact and d_err (count x 2 FP32, optional), also the audit errors of
  * the network against the exact result: |F - F_exact| / |F_exact| and
  * |T - T_exact| / |T_exact| (0 when both are 0). */
-int sfh_surrogate_apply(sfh_batch batch, const double* d_states, uint32_t first, uint32_t count,
+int sfh_surrogate_apply(sfh_batch batch, sfh_ocean ocean, const double* d_states, uint32_t first, uint32_t count,
                         const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
-                        const int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err, void* stream);
+                        int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err, void* stream);
 
 /* ---- rollout harness (no counterpart in the hydrodynamics path) ------------ */
 

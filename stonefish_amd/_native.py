@@ -129,7 +129,7 @@ _PROTOS = {
                                          C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 8),
     "sfh_rollout_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(sfh_rigid),
                                         C.c_double, C.c_void_p]),
-    "sfh_surrogate_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+    "sfh_surrogate_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 7),
 }
 

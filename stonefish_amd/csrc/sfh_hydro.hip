@@ -872,7 +872,8 @@ __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ 
 struct UnitStatic {
     double P_CB[3];
     double aabb_c[3], aabb_h[3];
-    uint32_t hydro, pad_;
+    double volume, surface;
+    uint32_t hydro, buoyant;
 };
 
 // Per compound: static data of Compound::ComputeHydrodynamicForces.
@@ -1426,7 +1427,10 @@ static UnitStatic make_unit(const sfh_body_desc& d)
         u.aabb_c[k] = 0.5 * (d.aabb_max[k] + d.aabb_min[k]);
         u.aabb_h[k] = 0.5 * (d.aabb_max[k] - d.aabb_min[k]);
     }
+    u.volume = d.volume;
+    u.surface = d.surface;
     u.hydro = d.hydro ? 1u : 0u;
+    u.buoyant = d.buoyant ? 1u : 0u;
     return u;
 }
 
@@ -2012,6 +2016,68 @@ __device__ void rot_to_quat(const double* R, double* q)
     q[0] = x * inv; q[1] = y * inv; q[2] = z * inv; q[3] = w * inv;
 }
 
+// Everything the surrogate derives from one body's state: the rotation, the
+// 13 features, the body/fluid position, and the physical baseline and scales
+// the network's outputs are relative to.  The network predicts
+//   y_F = (F - Fb0) / sF,  y_T = (T - Tb0) / sT   (CG frame)
+// where Fb0, Tb0 is a hydrostatic estimate (the analytic buoyancy inside the
+// water; the same scaled by the immersed fraction of the body's vertical
+// extent, from the CoB depth, when crossing) and
+//   sF = rho g V + rho (A/4) (u^3 + u),  sT = sF Lc,
+// u = |v_rel| + |omega| Lc, Lc the largest half extent of the body's AABB, A
+// its surface: the sizes of the buoyancy, quadratic-drag and skin-friction
+// terms.  Outputs stay O(1) whatever the speed and immersion, so the error of
+// the network is a relative error per body.
+struct SurCtx {
+    double R[9], t[3];
+    double f[SFH_SUR_FEATURES];
+    double base[6];
+    double sF, sT;
+    int pos;
+};
+
+__device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc, SurCtx& C)
+{
+    gl_basis(s, C.R);
+    for (int k = 0; k < 3; ++k) C.t[k] = s[12 + k];
+    double vr[3];
+    for (int k = 0; k < 3; ++k) vr[k] = s[16 + k] - oc.u[k];
+    double* f = C.f;
+    mat_t_vec(C.R, vr, f);
+    mat_t_vec(C.R, s + 19, f + 3);
+    rot_to_quat(C.R, f + 6);
+    C.pos = U.hydro ? classify64(C.R, C.t, U.aabb_c, U.aabb_h, oc) : -1;
+    double cb[3];
+    mat_vec(C.R, U.P_CB, cb);
+    for (int k = 0; k < 3; ++k) cb[k] += C.t[k];
+    f[10] = cb[2];
+    f[11] = f[12] = 0.0;
+    if (oc.nlayers) {
+        const double d = oc.len[0] / (double)oc.n;
+        f[10] += wave_sum64(oc, cb[0], cb[1]);
+        f[11] = (wave_sum64(oc, cb[0] + d, cb[1]) - wave_sum64(oc, cb[0] - d, cb[1])) / (2.0 * d);
+        f[12] = (wave_sum64(oc, cb[0], cb[1] + d) - wave_sum64(oc, cb[0], cb[1] - d)) / (2.0 * d);
+    }
+    // scales
+    const double Lc = fmax(fmax(U.aabb_h[0], U.aabb_h[1]), U.aabb_h[2]);
+    const double u = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) + sqrt(f[3] * f[3] + f[4] * f[4] + f[5] * f[5]) * Lc;
+    const double gn = sqrt(oc.g[0] * oc.g[0] + oc.g[1] * oc.g[1] + oc.g[2] * oc.g[2]);
+    C.sF = oc.rho * gn * U.volume + oc.rho * 0.25 * U.surface * (u * u * u + u);
+    if (!(C.sF > 0.0)) C.sF = 1.0;
+    C.sT = C.sF * fmax(Lc, 1e-6);
+    // hydrostatic estimate
+    double frac = 0.0;
+    if (U.buoyant && C.pos == 1) frac = 1.0;
+    if (U.buoyant && C.pos == 2) {
+        const double hz = fabs(C.R[6]) * U.aabb_h[0] + fabs(C.R[7]) * U.aabb_h[1] + fabs(C.R[8]) * U.aabb_h[2];
+        frac = hz > 0.0 ? fmin(1.0, fmax(0.0, (f[10] + hz) / (2.0 * hz))) : 0.0;
+    }
+    double gb[3];
+    mat_t_vec(C.R, oc.g, gb);
+    for (int k = 0; k < 3; ++k) C.base[k] = -oc.rho * U.volume * frac * gb[k];
+    cross64(U.P_CB, C.base, C.base + 3);
+}
+
 __global__ void k_sur_features(const UnitStatic* __restrict__ units, const double* __restrict__ states,
                                uint32_t first, uint32_t count, uint32_t rows, OceanK oc, SurNorm nm,
                                float* __restrict__ raw, uint16_t* __restrict__ x, uint8_t* __restrict__ ood,
@@ -2028,27 +2094,11 @@ __global__ void k_sur_features(const UnitStatic* __restrict__ units, const doubl
         return;
     }
     const uint32_t b = first + i;
-    const double* s = states + (size_t)b * SFH_STATE_DOUBLES;
-    double R[9], t[3] = {s[12], s[13], s[14]}, vr[3], f[SFH_SUR_FEATURES];
-    gl_basis(s, R);
-    for (int k = 0; k < 3; ++k) vr[k] = s[16 + k] - oc.u[k];
-    mat_t_vec(R, vr, f);
-    mat_t_vec(R, s + 19, f + 3);
-    rot_to_quat(R, f + 6);
-    const UnitStatic& U = units[b];
-    const int pos = U.hydro ? classify64(R, t, U.aabb_c, U.aabb_h, oc) : -1;
+    SurCtx C;
+    sur_context(units[b], states + (size_t)b * SFH_STATE_DOUBLES, oc, C);
+    const double* f = C.f;
+    const int pos = C.pos;
     if (posout) posout[i] = (int8_t)pos;
-    double cb[3];
-    mat_vec(R, U.P_CB, cb);
-    for (int k = 0; k < 3; ++k) cb[k] += t[k];
-    f[10] = cb[2];
-    f[11] = f[12] = 0.0;
-    if (oc.nlayers) {
-        const double d = oc.len[0] / (double)oc.n;
-        f[10] += wave_sum64(oc, cb[0], cb[1]);
-        f[11] = (wave_sum64(oc, cb[0] + d, cb[1]) - wave_sum64(oc, cb[0] - d, cb[1])) / (2.0 * d);
-        f[12] = (wave_sum64(oc, cb[0], cb[1] + d) - wave_sum64(oc, cb[0], cb[1] - d)) / (2.0 * d);
-    }
     if (raw)
         for (int k = 0; k < SFH_SUR_FEATURES; ++k) raw[(size_t)i * SFH_SUR_FEATURES + k] = (float)f[k];
     if (ood) {
@@ -2073,30 +2123,41 @@ __global__ void k_sur_features(const UnitStatic* __restrict__ units, const doubl
     if (exact && (raw_target || target)) {
         double F[3], T[3], y[6];
         exact_totals(exact + (size_t)b * SFH_OUT_DOUBLES, F, T);
-        mat_t_vec(R, F, y);
-        mat_t_vec(R, T, y + 3);
+        mat_t_vec(C.R, F, y);
+        mat_t_vec(C.R, T, y + 3);
         for (int k = 0; k < 6; ++k) {
-            if (raw_target) raw_target[(size_t)i * 6 + k] = (float)y[k];
-            if (target) target[(size_t)i * 6 + k] = (float)(((float)y[k] - nm.out_shift[k]) / nm.out_scale[k]);
+            const float r = (float)((y[k] - C.base[k]) / (k < 3 ? C.sF : C.sT));
+            if (raw_target) raw_target[(size_t)i * 6 + k] = r;
+            if (target) target[(size_t)i * 6 + k] = (r - nm.out_shift[k]) / nm.out_scale[k];
         }
     }
 }
 
-__global__ void k_sur_apply(const double* __restrict__ states, uint32_t first, uint32_t count, SurNorm nm,
-                            const float* __restrict__ y, const uint8_t* __restrict__ ood,
-                            const int8_t* __restrict__ pos, const double* __restrict__ exact,
-                            double* __restrict__ ft, float* __restrict__ err)
+// CG-frame network output -> world-frame F, T of one body.
+__device__ __forceinline__ void sur_decode(const SurCtx& C, const SurNorm& nm, const float* y, double* F, double* T)
+{
+    double yb[6];
+    for (int k = 0; k < 6; ++k)
+        yb[k] = C.base[k] + (k < 3 ? C.sF : C.sT) * (double)(y[k] * nm.out_scale[k] + nm.out_shift[k]);
+    mat_vec(C.R, yb, F);
+    mat_vec(C.R, yb + 3, T);
+    if (C.pos <= 0)   // outside the water or not hydrodynamic: no force
+        for (int k = 0; k < 3; ++k) F[k] = T[k] = 0.0;
+}
+
+__global__ void k_sur_apply(const UnitStatic* __restrict__ units, const double* __restrict__ states, uint32_t first,
+                            uint32_t count, OceanK oc, SurNorm nm, const float* __restrict__ y,
+                            const uint8_t* __restrict__ ood, int8_t* __restrict__ posout,
+                            const double* __restrict__ exact, double* __restrict__ ft, float* __restrict__ err)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const uint32_t b = first + i;
-    double R[9], yb[6], Fs[3], Ts[3];
-    gl_basis(states + (size_t)b * SFH_STATE_DOUBLES, R);
-    for (int k = 0; k < 6; ++k) yb[k] = (double)(y[(size_t)i * 6 + k] * nm.out_scale[k] + nm.out_shift[k]);
-    mat_vec(R, yb, Fs);
-    mat_vec(R, yb + 3, Ts);
-    if (pos && pos[i] <= 0)   // outside the water or not hydrodynamic: no force
-        for (int k = 0; k < 3; ++k) Fs[k] = Ts[k] = 0.0;
+    SurCtx C;
+    sur_context(units[b], states + (size_t)b * SFH_STATE_DOUBLES, oc, C);
+    if (posout) posout[i] = (int8_t)C.pos;
+    double Fs[3], Ts[3];
+    sur_decode(C, nm, y + (size_t)i * 6, Fs, Ts);
     double Fe[3], Te[3];
     if (exact) exact_totals(exact + (size_t)b * SFH_OUT_DOUBLES, Fe, Te);
     const bool fallback = exact && ood && ood[i];
@@ -2158,16 +2219,19 @@ extern "C" int sfh_surrogate_features(sfh_batch b, sfh_ocean o, const double* d_
     return SFH_OK;
 }
 
-extern "C" int sfh_surrogate_apply(sfh_batch b, const double* d_states, uint32_t first, uint32_t count,
+extern "C" int sfh_surrogate_apply(sfh_batch b, sfh_ocean o, const double* d_states, uint32_t first, uint32_t count,
                                    const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
-                                   const int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err,
+                                   int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err,
                                    void* stream)
 {
     if (!b || !d_states || !norm || !d_y || !d_ft) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(o);
+    if (rc != SFH_OK) return rc;
     if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "body range out of the batch");
     if (count == 0) return SFH_OK;
-    hipLaunchKernelGGL(k_sur_apply, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_states, first,
-                       count, sur_norm(norm), d_y, d_ood, d_pos, d_exact, d_ft, d_err);
+    hipLaunchKernelGGL(k_sur_apply, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, b->d_units,
+                       d_states, first, count, make_oceank(o), sur_norm(norm), d_y, d_ood, d_pos, d_exact, d_ft,
+                       d_err);
     HIPCHK(hipGetLastError());
     return SFH_OK;
 }
