@@ -147,11 +147,9 @@ def surrogate_section(args) -> dict:
     t_sur = _time_replays(step.replay, 200)
     # the surrogate's error on this sample against the exact outputs
     err = torch.zeros((nb, 2), dtype=torch.float32, device="cuda")
-    ft = torch.zeros((nb, 6), dtype=torch.float64, device="cuda")
-    for h in range(8):
-        sl = slice(h * per, (h + 1) * per)
-        S.apply(batch, ocean, states, h * per, per, norms[h], step.y[sl], ft[sl], pos=step.pos[sl], exact=out,
-                err=err[sl])
+    step.run()
+    step.plan.merge(out, step.ft.clone(), err=err)
+    torch.cuda.synchronize()
     sample_err = S.error_summary(err.cpu().numpy(), step.ood.cpu().numpy().astype(bool), step.pos.cpu().numpy())
     # MFMA rate of the fused inference at a large batch (all hulls' held-out inputs, one launch)
     xs = [ds.inputs(h, norms[h], range(ds.train_steps, ds.steps))[0] for h in range(8)]
@@ -233,6 +231,7 @@ def rollout_section(args, sea) -> dict:
     for _ in range(3):
         ro.step()
     torch.cuda.synchronize()
+    ro.capture()
     ro.audits.clear()
     ro.k = 0
     steps = args.rollout_steps

@@ -294,6 +294,35 @@ int sfh_surrogate_apply(sfh_batch batch, sfh_ocean ocean, const double* d_states
                         const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
                         int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err, void* stream);
 
+/* One-launch surrogate step.  A plan binds a batch whose units are grouped
+ * by hull (counts[g] consecutive units per group, g < 16) to one network and
+ * one normalisation per group; the network's device weights come from
+ * sfm_weights (sfmlp.h).  The plan must not outlive the batch or the networks. */
+typedef struct sfh_sur_plan_t* sfh_sur_plan;
+typedef struct {
+    const uint16_t* weights[5];   /* BF16 [out x in] padded to {32,256,256,256,256,32} */
+    const float* bias[5];
+} sfh_net_ptrs;
+
+int sfh_surrogate_plan_create(sfh_batch batch, uint32_t ngroups, const uint32_t* counts,
+                              const sfh_surrogate_norm* norms, const sfh_net_ptrs* nets, sfh_sur_plan* out);
+int sfh_surrogate_plan_destroy(sfh_sur_plan plan);
+
+/* Every unit of the batch in one launch (tiles of 32 or 64 units per
+ * workgroup): the features, the out-of-distribution flag (d_ood, optional)
+ * and position (d_pos, optional) as sfh_surrogate_features gives them, the
+ * network (BF16 MFMA, activations in LDS) and the decode to world-frame F, T
+ * (d_ft, n x 6) as sfh_surrogate_apply gives it. */
+int sfh_surrogate_run(sfh_sur_plan plan, sfh_ocean ocean, const double* d_states, uint8_t* d_ood, int8_t* d_pos,
+                      double* d_ft, void* stream);
+
+/* After an exact evaluation d_exact (n x SFH_OUT_DOUBLES, e.g. by
+ * sfh_batch_compute_masked on the OOD units): with d_err (n x 2, optional)
+ * the surrogate's F and T errors against it for every unit, then the exact
+ * totals replace d_ft for the units flagged in d_ood (optional). */
+int sfh_surrogate_merge(sfh_sur_plan plan, const uint8_t* d_ood, const double* d_exact, double* d_ft, float* d_err,
+                        void* stream);
+
 /* ---- rollout harness (no counterpart in the hydrodynamics path) ------------ */
 
 /* Rigid-body data of identical bodies: mass and principal moments of inertia

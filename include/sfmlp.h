@@ -87,6 +87,12 @@ int sfm_train_step(sfm_net net, const uint16_t* x, const float* target, uint32_t
  * [out x in] padded and biases [out]; for checking against a reference. */
 int sfm_gradients(sfm_net net, int layer, float* dweights, float* dbias, void* stream);
 
+/* Device pointers of the network's BF16 weights (w[l]: [out x in] padded,
+ * row-major) and FP32 biases (b[l]), l = 0..4, for a fused caller (the
+ * surrogate step of sfhydro.h).  They stay valid while the network lives;
+ * sfm_train_step updates them in place. */
+int sfm_weights(sfm_net net, const uint16_t** w, const float** b);
+
 /* Converts n FP32 values to BF16 bits (round to nearest even), device to device. */
 int sfm_to_bf16(const float* x, uint16_t* y, uint64_t n, void* stream);
 

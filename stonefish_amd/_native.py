@@ -86,6 +86,10 @@ class sfh_surrogate_norm(C.Structure):
     ]
 
 
+class sfh_net_ptrs(C.Structure):
+    _fields_ = [("weights", C.c_void_p * 5), ("bias", C.c_void_p * 5)]
+
+
 class sfh_rigid(C.Structure):
     _fields_ = [("mass", C.c_double), ("inertia", C.c_double * 3)]
 
@@ -127,6 +131,11 @@ _PROTOS = {
                                       C.c_void_p]),
     "sfh_surrogate_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 8),
+    "sfh_surrogate_plan_create": (C.c_int, [C.c_void_p, C.c_uint32, c_uint32_p, C.POINTER(sfh_surrogate_norm),
+                                            C.POINTER(sfh_net_ptrs), C.POINTER(C.c_void_p)]),
+    "sfh_surrogate_plan_destroy": (C.c_int, [C.c_void_p]),
+    "sfh_surrogate_run": (C.c_int, [C.c_void_p] * 7),
+    "sfh_surrogate_merge": (C.c_int, [C.c_void_p] * 6),
     "sfh_rollout_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(sfh_rigid),
                                         C.c_double, C.c_void_p]),
     "sfh_surrogate_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
@@ -199,6 +208,7 @@ _MLP_PROTOS = {
     "sfm_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfm_to_bf16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "sfm_probe_mfma": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfm_weights": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "sfm_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sfm_kernel_ms": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_uint32_p]),
 }

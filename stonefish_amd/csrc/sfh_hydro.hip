@@ -27,6 +27,7 @@
 
 #include "../../include/sfhydro.h"
 #include "sfh_face.h"
+#include "sfm_device.h"
 
 using namespace sfh;
 
@@ -2078,6 +2079,26 @@ __device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, c
     cross64(U.P_CB, C.base, C.base + 3);
 }
 
+// Network input row (BF16 bits, 32 per row) of a body's features.
+__device__ __forceinline__ void sur_input_row(const double* f, const SurNorm& nm, uint32_t w[SFH_SUR_ROW / 2])
+{
+    for (int k = 0; k < SFH_SUR_ROW / 2; ++k) w[k] = 0;
+    for (int k = 0; k < SFH_SUR_FEATURES; ++k) {
+        const uint32_t h = bf16_bits(((float)f[k] - nm.in_shift[k]) * nm.in_scale[k]);
+        w[k >> 1] |= h << (16 * (k & 1));
+    }
+}
+
+__device__ __forceinline__ uint8_t sur_ood(const double* f, int pos, const SurNorm& nm)
+{
+    uint8_t o = 0;
+    for (int k = 0; k < SFH_SUR_FEATURES; ++k) {
+        const float v = (float)f[k];
+        if (!(v >= nm.in_lo[k] && v <= nm.in_hi[k])) o = 1;   // also catches NaN
+    }
+    return pos > 0 ? o : (uint8_t)0;   // out of the water (or not hydrodynamic): the result is known
+}
+
 __global__ void k_sur_features(const UnitStatic* __restrict__ units, const double* __restrict__ states,
                                uint32_t first, uint32_t count, uint32_t rows, OceanK oc, SurNorm nm,
                                float* __restrict__ raw, uint16_t* __restrict__ x, uint8_t* __restrict__ ood,
@@ -2101,22 +2122,10 @@ __global__ void k_sur_features(const UnitStatic* __restrict__ units, const doubl
     if (posout) posout[i] = (int8_t)pos;
     if (raw)
         for (int k = 0; k < SFH_SUR_FEATURES; ++k) raw[(size_t)i * SFH_SUR_FEATURES + k] = (float)f[k];
-    if (ood) {
-        uint8_t o = 0;
-        for (int k = 0; k < SFH_SUR_FEATURES; ++k) {
-            const float v = (float)f[k];
-            if (!(v >= nm.in_lo[k] && v <= nm.in_hi[k])) o = 1;   // also catches NaN
-        }
-        if (pos <= 0) o = 0;   // outside the water (or not hydrodynamic): the result is known
-        ood[i] = o;
-    }
+    if (ood) ood[i] = sur_ood(f, pos, nm);
     if (x) {
         uint32_t w[SFH_SUR_ROW / 2];
-        for (int k = 0; k < SFH_SUR_ROW / 2; ++k) w[k] = 0;
-        for (int k = 0; k < SFH_SUR_FEATURES; ++k) {
-            const uint32_t h = bf16_bits(((float)f[k] - nm.in_shift[k]) * nm.in_scale[k]);
-            w[k >> 1] |= h << (16 * (k & 1));
-        }
+        sur_input_row(f, nm, w);
         uint4* row = reinterpret_cast<uint4*>(x + (size_t)i * SFH_SUR_ROW);
         for (int k = 0; k < SFH_SUR_ROW / 8; ++k) row[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
     }
@@ -2134,16 +2143,24 @@ __global__ void k_sur_features(const UnitStatic* __restrict__ units, const doubl
 }
 
 // CG-frame network output -> world-frame F, T of one body.
-__device__ __forceinline__ void sur_decode(const SurCtx& C, const SurNorm& nm, const float* y, double* F, double* T)
+__device__ __forceinline__ void sur_decode(const double* R, const double* base, double sF, double sT, int pos,
+                                           const SurNorm& nm, const float* y, double* F, double* T)
 {
     double yb[6];
     for (int k = 0; k < 6; ++k)
-        yb[k] = C.base[k] + (k < 3 ? C.sF : C.sT) * (double)(y[k] * nm.out_scale[k] + nm.out_shift[k]);
-    mat_vec(C.R, yb, F);
-    mat_vec(C.R, yb + 3, T);
-    if (C.pos <= 0)   // outside the water or not hydrodynamic: no force
+        yb[k] = base[k] + (k < 3 ? sF : sT) * (double)(y[k] * nm.out_scale[k] + nm.out_shift[k]);
+    mat_vec(R, yb, F);
+    mat_vec(R, yb + 3, T);
+    if (pos <= 0)   // outside the water or not hydrodynamic: no force
         for (int k = 0; k < 3; ++k) F[k] = T[k] = 0.0;
 }
+
+__device__ __forceinline__ void sur_decode(const SurCtx& C, const SurNorm& nm, const float* y, double* F, double* T)
+{
+    sur_decode(C.R, C.base, C.sF, C.sT, C.pos, nm, y, F, T);
+}
+
+
 
 __global__ void k_sur_apply(const UnitStatic* __restrict__ units, const double* __restrict__ states, uint32_t first,
                             uint32_t count, OceanK oc, SurNorm nm, const float* __restrict__ y,
@@ -2291,6 +2308,219 @@ extern "C" int sfh_rollout_integrate(sfh_ocean o, double* d_states, uint32_t cou
     hipLaunchKernelGGL(k_rollout, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_states, count, d_ft,
                        rigid->mass, rigid->inertia[0], rigid->inertia[1], rigid->inertia[2], o->g[0], o->g[1], o->g[2],
                        dt);
+    HIPCHK(hipGetLastError());
+    return SFH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one-launch surrogate step: features, network, decode
+// ---------------------------------------------------------------------------
+
+struct SurPlanDev {
+    const uint16_t* W[SFM_MAX_GROUP][SFM_LAYERS];
+    const float* B[SFM_MAX_GROUP][SFM_LAYERS];
+    SurNorm norm[SFM_MAX_GROUP];
+};
+
+struct SurTile {
+    uint32_t unit0, nvalid, group, pad_;
+};
+
+struct SurRowCtx {
+    double R[9], base[6], sF, sT;
+    int32_t pos, pad_;
+};
+
+// One workgroup per tile of TR units of one group (one network): lanes
+// 0..TR-1 build the features of their unit (the FP64 context of
+// k_sur_features) into the network's input rows in LDS, the four wavefronts
+// run the fused network (sfm_device.h), and lanes 0..TR-1 decode their
+// unit's outputs to world-frame F, T.
+template <int TR>
+__global__ __launch_bounds__(256) void k_sur_step(const SurPlanDev* __restrict__ P, const SurTile* __restrict__ tiles,
+                                                  const UnitStatic* __restrict__ units,
+                                                  const double* __restrict__ states, OceanK oc,
+                                                  uint8_t* __restrict__ ood, int8_t* __restrict__ posout,
+                                                  double* __restrict__ ft)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t act[2][TR * sfm::kActStride];
+    __shared__ SurRowCtx ctx[TR];
+    __shared__ float yo[TR][SFM_OUT];
+    const SurTile tl = tiles[blockIdx.x];
+    const int g = (int)tl.group;
+    const int t = threadIdx.x;
+    if (t < TR) {
+        uint4* xr = reinterpret_cast<uint4*>(act[1] + t * sfm::kActStride);
+        if ((uint32_t)t < tl.nvalid) {
+            const uint32_t u = tl.unit0 + (uint32_t)t;
+            SurCtx C;
+            sur_context(units[u], states + (size_t)u * SFH_STATE_DOUBLES, oc, C);
+            const SurNorm& nm = P->norm[g];
+            uint32_t w[SFH_SUR_ROW / 2];
+            sur_input_row(C.f, nm, w);
+            for (int k = 0; k < SFH_SUR_ROW / 8; ++k) xr[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+            if (ood) ood[u] = sur_ood(C.f, C.pos, nm);
+            if (posout) posout[u] = (int8_t)C.pos;
+            SurRowCtx& X = ctx[t];
+            for (int k = 0; k < 9; ++k) X.R[k] = C.R[k];
+            for (int k = 0; k < 6; ++k) X.base[k] = C.base[k];
+            X.sF = C.sF;
+            X.sT = C.sT;
+            X.pos = C.pos;
+        } else {
+            for (int k = 0; k < SFH_SUR_ROW / 8; ++k) xr[k] = make_uint4(0, 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    sfm::mlp_hidden<TR>(P->W[g], P->B[g], act, nullptr);
+    const int lane = t & 63, wave = t >> 6;
+    if (wave < TR / 32) {
+        float o[4];
+        sfm::mlp_output(P->W[g][SFM_LAYERS - 1], P->B[g][SFM_LAYERS - 1], act[1], o);
+        const int lq = (lane >> 5) * 4, row = 32 * wave + (lane & 31);
+        for (int i = 0; i < 4; ++i)
+            if (lq + i < SFM_OUT) yo[row][lq + i] = o[i];
+    }
+    __syncthreads();
+    if (t < TR && (uint32_t)t < tl.nvalid) {
+        const uint32_t u = tl.unit0 + (uint32_t)t;
+        const SurRowCtx& X = ctx[t];
+        double F[3], T[3];
+        sur_decode(X.R, X.base, X.sF, X.sT, X.pos, P->norm[g], yo[t], F, T);
+        double* o = ft + (size_t)u * 6;
+        for (int k = 0; k < 3; ++k) { o[k] = F[k]; o[3 + k] = T[k]; }
+    }
+}
+
+// Exact totals for the out-of-distribution units; with err, first the
+// surrogate's error against the exact totals for every unit.
+__global__ void k_sur_merge(uint32_t n, const uint8_t* __restrict__ ood, const double* __restrict__ exact,
+                            double* __restrict__ ft, float* __restrict__ err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* o = ft + (size_t)i * 6;
+    double Fe[3], Te[3];
+    exact_totals(exact + (size_t)i * SFH_OUT_DOUBLES, Fe, Te);
+    if (err) {
+        double dF = 0, nF = 0, dT = 0, nT = 0;
+        for (int k = 0; k < 3; ++k) {
+            dF += (o[k] - Fe[k]) * (o[k] - Fe[k]);
+            nF += Fe[k] * Fe[k];
+            dT += (o[3 + k] - Te[k]) * (o[3 + k] - Te[k]);
+            nT += Te[k] * Te[k];
+        }
+        err[2 * (size_t)i] = (float)(dF == 0.0 ? 0.0 : sqrt(dF / nF));
+        err[2 * (size_t)i + 1] = (float)(dT == 0.0 ? 0.0 : sqrt(dT / nT));
+    }
+    if (ood && ood[i])
+        for (int k = 0; k < 3; ++k) { o[k] = Fe[k]; o[3 + k] = Te[k]; }
+}
+
+struct sfh_sur_plan_t {
+    sfh_batch_t* batch = nullptr;
+    SurPlanDev* d_plan = nullptr;
+    SurTile* d_tiles = nullptr;
+    uint32_t ntiles = 0;
+    int TR = 32;
+};
+
+extern "C" int sfh_surrogate_plan_create(sfh_batch b, uint32_t ngroups, const uint32_t* counts,
+                                         const sfh_surrogate_norm* norms, const sfh_net_ptrs* nets,
+                                         sfh_sur_plan* out)
+{
+    if (!b || !counts || !norms || !nets || !out) return fail(SFH_ERR_INVALID, "null argument");
+    if (ngroups == 0 || ngroups > SFM_MAX_GROUP) return fail(SFH_ERR_SIZE, "1 to %d groups", SFM_MAX_GROUP);
+    uint64_t total = 0;
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        total += counts[g];
+        for (int l = 0; l < SFM_LAYERS; ++l)
+            if (!nets[g].weights[l] || !nets[g].bias[l]) return fail(SFH_ERR_INVALID, "group %u: null weights", g);
+    }
+    if (total != b->nunits) return fail(SFH_ERR_SIZE, "group counts must add up to the batch's %u units", b->nunits);
+    SurPlanDev h;
+    memset(&h, 0, sizeof(h));
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        for (int l = 0; l < SFM_LAYERS; ++l) {
+            h.W[g][l] = nets[g].weights[l];
+            h.B[g][l] = nets[g].bias[l];
+        }
+        h.norm[g] = sur_norm(&norms[g]);
+    }
+    int cus = 256, dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        cus = prop.multiProcessorCount;
+    const int TR = (total / 64 >= 2 * (uint64_t)cus) ? 64 : 32;
+    std::vector<SurTile> tiles;
+    uint64_t u0 = 0;
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        for (uint32_t r = 0; r < counts[g]; r += TR) {
+            SurTile t;
+            t.unit0 = (uint32_t)(u0 + r);
+            t.nvalid = std::min<uint32_t>(TR, counts[g] - r);
+            t.group = g;
+            t.pad_ = 0;
+            tiles.push_back(t);
+        }
+        u0 += counts[g];
+    }
+    sfh_sur_plan_t* p = new sfh_sur_plan_t();
+    p->batch = b;
+    p->TR = TR;
+    p->ntiles = (uint32_t)tiles.size();
+    hipError_t e = hipMalloc(&p->d_plan, sizeof(SurPlanDev));
+    if (e == hipSuccess) e = hipMalloc(&p->d_tiles, sizeof(SurTile) * std::max<size_t>(1, tiles.size()));
+    if (e == hipSuccess) e = hipMemcpy(p->d_plan, &h, sizeof(h), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !tiles.empty())
+        e = hipMemcpy(p->d_tiles, tiles.data(), sizeof(SurTile) * tiles.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(p->d_plan);
+        hipFree(p->d_tiles);
+        delete p;
+        return fail(SFH_ERR_HIP, "surrogate plan: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return SFH_OK;
+}
+
+extern "C" int sfh_surrogate_plan_destroy(sfh_sur_plan p)
+{
+    if (!p) return fail(SFH_ERR_INVALID, "null plan");
+    hipDeviceSynchronize();
+    hipFree(p->d_plan);
+    hipFree(p->d_tiles);
+    delete p;
+    return SFH_OK;
+}
+
+extern "C" int sfh_surrogate_run(sfh_sur_plan p, sfh_ocean o, const double* d_states, uint8_t* d_ood, int8_t* d_pos,
+                                 double* d_ft, void* stream)
+{
+    if (!p || !d_states || !d_ft) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(o);
+    if (rc != SFH_OK) return rc;
+    if (p->ntiles == 0) return SFH_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const OceanK oc = make_oceank(o);
+    if (p->TR == 64)
+        hipLaunchKernelGGL(k_sur_step<64>, dim3(p->ntiles), dim3(256), 0, s, p->d_plan, p->d_tiles, p->batch->d_units,
+                           d_states, oc, d_ood, d_pos, d_ft);
+    else
+        hipLaunchKernelGGL(k_sur_step<32>, dim3(p->ntiles), dim3(256), 0, s, p->d_plan, p->d_tiles, p->batch->d_units,
+                           d_states, oc, d_ood, d_pos, d_ft);
+    HIPCHK(hipGetLastError());
+    return SFH_OK;
+}
+
+extern "C" int sfh_surrogate_merge(sfh_sur_plan p, const uint8_t* d_ood, const double* d_exact, double* d_ft,
+                                   float* d_err, void* stream)
+{
+    if (!p || !d_exact || !d_ft) return fail(SFH_ERR_INVALID, "null argument");
+    const uint32_t n = p->batch->nunits;
+    if (n == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_sur_merge, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, n, d_ood, d_exact, d_ft,
+                       d_err);
     HIPCHK(hipGetLastError());
     return SFH_OK;
 }

@@ -23,14 +23,17 @@
 #include <vector>
 
 #include "../../include/sfmlp.h"
-#include "sfm_gelu.h"
+#include "sfm_device.h"
 
+using sfm::acc_row;
+using sfm::bf16x8;
+using sfm::f2bf;
+using sfm::f2bf2;
+using sfm::f32x16;
 using sfm::gelu;
 using sfm::gelu_grad;
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
+using sfm::kActStride;
+using sfm::ld8;
 
 static thread_local char g_err[512] = "";
 
@@ -60,21 +63,7 @@ static constexpr int kSplitK = 256;   // batch rows per weight-gradient split
 // numerics
 // ---------------------------------------------------------------------------
 
-// FP32 -> BF16 bits, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
-__device__ __forceinline__ uint16_t f2bf(float f)
-{
-    const __bf16 h = (__bf16)f;
-    return *reinterpret_cast<const uint16_t*>(&h);
-}
 
-// Two FP32 -> packed BF16 pair (low half = a).
-__device__ __forceinline__ uint32_t f2bf2(float a, float b)
-{
-    bf16x2 v;
-    v[0] = (__bf16)a;
-    v[1] = (__bf16)b;
-    return *reinterpret_cast<const uint32_t*>(&v);
-}
 
 // ---------------------------------------------------------------------------
 // GEMM tile kernel
@@ -98,9 +87,6 @@ struct GemmArgs {
     float* part;            // [ksplit][rows x cols] FP32  (WGRAD)
 };
 
-// Accumulator register r of lane l of a 32x32 MFMA tile holds row
-// 8 (r / 4) + 4 (l / 32) + r % 4 and column l % 32 of the tile.
-__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
@@ -187,89 +173,27 @@ struct NetSet {
     const float* b[SFM_MAX_GROUP][SFM_LAYERS];
 };
 
-static constexpr int kActStride = SFM_HIDDEN + 8;   // LDS row stride (elements): spreads rows over banks
-
-__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
-
-// One workgroup (4 wavefronts) runs TR consecutive rows (samples) of one
-// network through the whole network; the hidden activations stay in LDS (two
-// TR x 256 BF16 buffers, row-major [sample][feature]).  Each layer is computed
-// transposed, H'^T = W H^T: the weights are the MFMA's A operand (16-byte
-// loads of W[feature][k..k+7] from global memory, L2-resident, 405 KB per
-// network) and the activations its B operand (16-byte LDS reads of
-// H[sample][k..k+7]), so an accumulator lane holds four consecutive output
-// features of one sample and stores them with one 8-byte LDS write.
-// Wavefront w owns output features [64 w, 64 w + 64) of each hidden layer;
-// the output layer is one 32-feature tile per 32-sample tile.
+// Inference of TR consecutive rows of one network per workgroup (the fused
+// network of sfm_device.h).
 template <int TR>
 __global__ __launch_bounds__(256) void k_mlp_fused(NetSet ns, const uint16_t* __restrict__ x, uint32_t rows,
                                                    float* __restrict__ y)
 {
-    constexpr int RT = TR / 32;
     __shared__ __attribute__((aligned(16))) uint16_t act[2][TR * kActStride];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t row0 = blockIdx.x * TR;
     if (row0 >= rows) return;
     int k = 0;
     while (k + 1 < ns.nnets && row0 >= ns.row_end[k]) ++k;
-    const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
-
-    for (int l = 0; l < SFM_LAYERS - 1; ++l) {
-        const int K = (l == 0) ? SFM_IN_PAD : SFM_HIDDEN;
-        const uint16_t* W = ns.W[k][l];
-        const float* bias = ns.b[k][l];
-        const uint16_t* in = act[(l + 1) & 1];
-        uint16_t* out = act[l & 1];
-        f32x16 acc[2][RT];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < RT; ++nt) acc[mt][nt] = f32x16{};
-        const uint16_t* wp0 = W + (size_t)(64 * wave + lr) * K + lk;
-        const uint16_t* wp1 = wp0 + (size_t)32 * K;
-        for (int ks = 0; ks < K; ks += 16) {
-            const bf16x8 a0 = ld8(wp0 + ks), a1 = ld8(wp1 + ks);
-#pragma unroll
-            for (int nt = 0; nt < RT; ++nt) {
-                const bf16x8 b = (l == 0) ? ld8(x + (size_t)(row0 + 32 * nt + lr) * SFM_IN_PAD + ks + lk)
-                                          : ld8(in + (32 * nt + lr) * kActStride + ks + lk);
-                acc[0][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][nt], 0, 0, 0);
-                acc[1][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][nt], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int f0 = 64 * wave + 32 * mt + 8 * q + lq;   // four consecutive output features
-                const float4 bq = *reinterpret_cast<const float4*>(bias + f0);
-#pragma unroll
-                for (int nt = 0; nt < RT; ++nt) {
-                    const f32x16& A = acc[mt][nt];
-                    uint2 pk;
-                    pk.x = f2bf2(gelu(A[4 * q] + bq.x), gelu(A[4 * q + 1] + bq.y));
-                    pk.y = f2bf2(gelu(A[4 * q + 2] + bq.z), gelu(A[4 * q + 3] + bq.w));
-                    *reinterpret_cast<uint2*>(out + (32 * nt + lr) * kActStride + f0) = pk;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // output layer: features 0..31 (6 real) of sample tile `wave`, K = 256, from act[1]
-    if (wave < RT) {
-        const uint16_t* W = ns.W[k][SFM_LAYERS - 1];
-        const float* bias = ns.b[k][SFM_LAYERS - 1];
-        const uint16_t* in = act[1];
-        f32x16 acc = {};
-        const uint16_t* wp = W + (size_t)lr * SFM_HIDDEN + lk;
-        for (int ks = 0; ks < SFM_HIDDEN; ks += 16)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(wp + ks), ld8(in + (32 * wave + lr) * kActStride + ks + lk),
-                                                          acc, 0, 0, 0);
-        // lane: features lq .. lq+3 (registers 0..3) of sample 32 wave + lr
-        float* yo = y + (size_t)(row0 + 32 * wave + lr) * SFM_OUT;
+    sfm::mlp_hidden<TR>(ns.W[k], ns.b[k], act, x + (size_t)row0 * SFM_IN_PAD);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave < TR / 32) {
+        float o[4];
+        sfm::mlp_output(ns.W[k][SFM_LAYERS - 1], ns.b[k][SFM_LAYERS - 1], act[1], o);
+        const int lq = (lane >> 5) * 4;
+        float* yo = y + (size_t)(row0 + 32 * wave + (lane & 31)) * SFM_OUT;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            if (lq + i < SFM_OUT) yo[lq + i] = acc[i] + bias[lq + i];
+            if (lq + i < SFM_OUT) yo[lq + i] = o[i];
     }
 }
 
@@ -710,6 +634,16 @@ extern "C" int sfm_probe_mfma(const uint16_t* a, const uint16_t* b, float* c, vo
     if (!a || !b || !c) return fail(SFM_ERR_INVALID, "null argument");
     hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, c);
     HIPCHK(hipGetLastError());
+    return SFM_OK;
+}
+
+extern "C" int sfm_weights(sfm_net n, const uint16_t** w, const float** b)
+{
+    if (!n || !w || !b) return fail(SFM_ERR_INVALID, "null argument");
+    for (int l = 0; l < SFM_LAYERS; ++l) {
+        w[l] = n->L[l].Wb;
+        b[l] = n->L[l].b;
+    }
     return SFM_OK;
 }
 

@@ -92,12 +92,25 @@ def test_mixed_rollout_runs_with_audits(gpu):
     net = S.Net(1024, seed=1)
     S.train(net, x, t, S.TrainConfig(steps=100, batch=256, warmup=10))
     rng = np.random.default_rng(9)
-    states = torch.from_numpy(S.sample_states(rng, n, xy_range=(0.0, 100.0), depth_range=(0.5, 2.0))).cuda()
-    ro = S.MixedRollout(batch, ocean, net, norm, states, auv.mass, auv.inertia, dt=1e-3, audit_every=10)
-    for _ in range(25):
-        ro.step()
-    torch.cuda.synchronize()
-    assert torch.isfinite(states).all()
-    summ = ro.audit_summary()
-    assert summ["audits"] == 3 and summ["n"] > 0
+    init = torch.from_numpy(S.sample_states(rng, n, xy_range=(0.0, 100.0), depth_range=(0.5, 2.0))).cuda()
+    # the same rollout eagerly and with the ordinary step replayed from a HIP graph
+    runs = []
+    for graph in (False, True):
+        states = init.clone()
+        ro = S.MixedRollout(batch, ocean, net, norm, states, auv.mass, auv.inertia, dt=1e-3, audit_every=10)
+        if graph:
+            ro.step()   # audit step 0 eagerly, then capture (the capture itself does not run)
+            ro.capture()
+            for _ in range(24):
+                ro.step()
+        else:
+            for _ in range(25):
+                ro.step()
+        torch.cuda.synchronize()
+        assert torch.isfinite(states).all()
+        summ = ro.audit_summary()
+        assert summ["audits"] == 3 and summ["n"] > 0
+        runs.append(states.cpu().numpy())
+    assert np.array_equal(runs[0], runs[1])
+    assert not np.array_equal(runs[0], init.cpu().numpy())
     net.close()

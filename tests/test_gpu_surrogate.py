@@ -327,3 +327,73 @@ def test_surrogate_beats_untrained_on_held_out(gpu):
     assert trained["F_median"] < 0.25 * untrained["F_median"], (trained, untrained)
     assert trained["T_median"] < 0.5 * untrained["T_median"], (trained, untrained)
     net.close()
+
+
+def test_one_launch_step_matches_separate_kernels(gpu):
+    """sfh_surrogate_run (features + network + decode in one launch, two
+    groups with their own networks and norms) against features, the fused
+    forward and the decode run separately; merge against the exact rows."""
+    from stonefish_amd import surrogate as S
+    batch, ocean, hulls, grid = _small_fleet(waves=True, n_hulls=2, per_hull=256)
+    rng = np.random.default_rng(31)
+    st = torch.from_numpy(S.sample_states(rng, batch.n, xy_range=(0.0, 256.0), depth_range=(-0.5, 1.5))).cuda()
+    exact = torch.full((batch.n, 24), 0.0, dtype=torch.float64, device="cuda")
+    batch.compute(ocean, st.data_ptr(), exact.data_ptr())
+    raw = torch.zeros(batch.n, 13, dtype=torch.float32, device="cuda")
+    rt = torch.zeros(batch.n, 6, dtype=torch.float32, device="cuda")
+    S.features(batch, ocean, st, 0, batch.n, raw=raw, exact=exact, raw_target=rt)
+    norms = [S.Norm.fit(raw[:256].cpu().numpy(), rt[:256].cpu().numpy()),
+             S.Norm.fit(raw[256:].cpu().numpy(), rt[256:].cpu().numpy())]
+    # tighten the training range so that some units are out of distribution
+    for nm in norms:
+        nm.in_hi[0] = 1.0
+    nets = [S.Net(1024, seed=4), S.Net(1024, seed=5)]
+    counts = [224, 288]   # groups need not match the hulls or the tile size
+    plan = S.Plan(batch, nets, norms_for(norms, counts), counts)
+    ft = torch.zeros(batch.n, 6, dtype=torch.float64, device="cuda")
+    ood = torch.zeros(batch.n, dtype=torch.uint8, device="cuda")
+    pos = torch.zeros(batch.n, dtype=torch.int8, device="cuda")
+    plan.run(ocean, st, ft, ood=ood, pos=pos)
+    # separate path
+    ref = torch.zeros_like(ft)
+    off = 0
+    for g, c in enumerate(counts):
+        nm = norms_for(norms, counts)[g]
+        rows = (c + 31) // 32 * 32
+        x = torch.zeros(rows, 32, dtype=torch.bfloat16, device="cuda")
+        o2 = torch.zeros(c, dtype=torch.uint8, device="cuda")
+        p2 = torch.zeros(c, dtype=torch.int8, device="cuda")
+        S.features(batch, ocean, st, off, c, norm=nm, rows=rows, x=x, ood=o2, pos=p2)
+        y = torch.zeros(rows, 6, dtype=torch.float32, device="cuda")
+        nets[g].forward(x, y)
+        S.apply(batch, ocean, st, off, c, nm, y[:c], ref[off:off + c])
+        assert torch.equal(o2, ood[off:off + c]) and torch.equal(p2, pos[off:off + c])
+        off += c
+    torch.cuda.synchronize()
+    f, r = ft.cpu().numpy(), ref.cpu().numpy()
+    scale = np.abs(r).max(0) + 1e-9
+    assert np.all(np.abs(f - r) <= 1e-5 * scale + 1e-6 * np.abs(r)), np.abs(f - r).max(0)
+    assert int(ood.sum()) > 0
+    # merge: exact totals on the flagged units, errors against the exact rows
+    err = torch.zeros(batch.n, 2, dtype=torch.float32, device="cuda")
+    sur = ft.clone()
+    plan.merge(exact, ft, ood=ood, err=err)
+    torch.cuda.synchronize()
+    o = ood.cpu().numpy().astype(bool)
+    ex = exact.cpu().numpy()
+    w = ex[:, 20].astype(np.int64)
+    Fe = sum(np.where(((w >> (2 * k)) & 1)[:, None] == 1, ex[:, 6 * k:6 * k + 3], 0.0) for k in range(3))
+    fm = ft.cpu().numpy()
+    assert np.array_equal(fm[o, :3], Fe[o]) and np.array_equal(fm[~o], sur.cpu().numpy()[~o])
+    e = err.cpu().numpy()
+    nF = np.linalg.norm(Fe, axis=1)
+    m = nF > 0
+    d = np.linalg.norm(sur.cpu().numpy()[:, :3] - Fe, axis=1)
+    np.testing.assert_allclose(e[m, 0], d[m] / nF[m], rtol=1e-5)
+    plan.close()
+    for n in nets:
+        n.close()
+
+
+def norms_for(norms, counts):
+    return [norms[0], norms[1]][:len(counts)]
