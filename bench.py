@@ -121,7 +121,7 @@ def surrogate_section(args) -> dict:
     import torch
     from stonefish_amd import surrogate as S
     t0 = time.time()
-    cfg = S.TrainConfig(steps=args.sur_train_steps, batch=4096, lr=2e-3, warmup=200)
+    cfg = S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=3e-3, warmup=200)
     r = S.run_training_workload(8, args.sur_per_hull, args.sur_steps, args.sur_holdout, cfg, seed=17, log=log)
     hulls, nets, norms, w, ocean = r["hulls"], r["nets"], r["norms"], r["workload"], r["ocean"]
     ds = r["dataset"]
@@ -132,6 +132,7 @@ def surrogate_section(args) -> dict:
     # exact vs surrogate on 8 x 512 bodies of the same hulls
     per = 512
     batch, meshes = S.make_hull_batch(w.bodies, per)
+    S.attach_tables(batch, ocean, [(h * per, per) for h in range(8)], tables=r["tables"])
     nb = 8 * per
     rng = np.random.default_rng(99)
     states = torch.from_numpy(S.sample_states(rng, nb, (0.0, w.sea.wave_len), (-1.0, 3.0))).cuda()
@@ -166,7 +167,7 @@ def surrogate_section(args) -> dict:
     res = {
         "workload": f"8 hulls x {args.sur_per_hull} bodies x {args.sur_steps} steps of exact-kernel samples "
                     f"({8 * args.sur_per_hull * args.sur_steps} total, {args.sur_holdout} steps held out), "
-                    f"{args.sur_train_steps} Adam steps of 4096 per hull",
+                    f"{args.sur_train_steps} Adam steps of {args.sur_batch} per hull",
         "held_out_error": agg,
         "per_hull": [{k: h[k] for k in ("hull", "faces", "train_samples", "F_median", "F_p90", "T_median",
                                         "T_p90", "n", "n_ood", "n_out_of_water", "final_loss", "train_seconds")}
@@ -209,10 +210,11 @@ def rollout_section(args, sea) -> dict:
     ocean, grid = S.make_ocean(sea)
     ds = S.HullDataset(batch, ocean, 1, n, steps=args.rollout_data_steps, holdout_steps=2, seed=31,
                        xy_range=(0.0, sea.wave_len), depth_range=(-1.0, 3.0))
+    tables = S.attach_tables(batch, ocean, [(0, n)])
     norm = ds.fit_norm(0)
     x, t, _ = ds.inputs(0, norm, range(ds.train_steps))
     net = S.Net(65536, seed=7)
-    hist = S.train(net, x, t, S.TrainConfig(steps=args.sur_train_steps, batch=4096, lr=2e-3, warmup=200))
+    hist = S.train(net, x, t, S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=3e-3, warmup=200))
     held = S.evaluate(ds, 0, net, norm)
     log(f"rollout surrogate trained ({ds.gen_seconds:.1f}s data, {hist['seconds']:.1f}s train): "
         f"F median {held['F_median']:.4f}")
@@ -269,12 +271,14 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate and rollout sections")
+    ap.add_argument("--rollout-only", action="store_true", help="run only the mixed rollout section (profiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB device copy (reference dispatch for HBM byte counters)")
     ap.add_argument("--sur-per-hull", type=int, default=8192)
     ap.add_argument("--sur-steps", type=int, default=31)
     ap.add_argument("--sur-holdout", type=int, default=3)
-    ap.add_argument("--sur-train-steps", type=int, default=4000)
+    ap.add_argument("--sur-train-steps", type=int, default=20000)
+    ap.add_argument("--sur-batch", type=int, default=16384)
     ap.add_argument("--rollout-bodies", type=int, default=16384)
     ap.add_argument("--rollout-steps", type=int, default=300)
     ap.add_argument("--rollout-data-steps", type=int, default=14)
@@ -298,6 +302,9 @@ def main() -> int:
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.rollout_only:
+        print(json.dumps({"rollout": rollout_section(args, WL.fleet(1, seed=args.seed).sea)}), flush=True)
+        return 0
     w = build_fleet(args.bodies, args.seed)
     nf = np.array([b.mesh.nf for b in w.bodies], dtype=np.int64)
     nv = np.array([b.mesh.nv for b in w.bodies], dtype=np.int64)

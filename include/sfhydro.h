@@ -241,9 +241,13 @@ int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_
  *   0-2   linear velocity relative to the current at the CG, CG frame (R^T (v - u))
  *   3-5   angular velocity, CG frame (R^T omega)
  *   6-9   orientation of the CG frame, unit quaternion (x, y, z, w), w >= 0
- *   10    depth of the centre of buoyancy below the local surface (Ocean::GetDepth)
- *   11-12 slope of the depth offset along world x and y (central difference
- *         over one texel of the first wave layer; 0 on a flat sea)
+ *   10    depth of the centre of buoyancy below the local surface
+ *   11-12 slope of the local surface's depth offset along world x and y
+ * where the local surface is the plane fitted to the wave field (the heights
+ * of Ocean::GetDepth) over the body's horizontal footprint: a 3 x 3 stencil
+ * at offsets of the world AABB's half extents (at least one texel of the
+ * first layer), weights (1/4, 1/2, 1/4) per axis; a flat sea gives the
+ * CoB's depth and zero slopes.
  * and predicts the total force and torque about the CG, CG frame,
  *   F = R^T (Fb + Fdq + Fdf), T = R^T (Tb + Tdq + Tdf) after CorrectHydrodynamicForces,
  * relative to a hydrostatic estimate (Fb0, Tb0: the analytic buoyancy of the
@@ -293,6 +297,20 @@ int sfh_surrogate_features(sfh_batch batch, sfh_ocean ocean, const double* d_sta
 int sfh_surrogate_apply(sfh_batch batch, sfh_ocean ocean, const double* d_states, uint32_t first, uint32_t count,
                         const sfh_surrogate_norm* norm, const float* d_y, const uint8_t* d_ood,
                         int8_t* d_pos, const double* d_exact, double* d_ft, float* d_err, void* stream);
+
+/* Hydrostatic table of a hull, the surrogate's baseline for bodies crossing
+ * the surface (without one, the analytic buoyancy scaled by the immersed
+ * fraction of the AABB's height is used): d_table (device, FP32) holds
+ * ns x ntheta x nphi entries of 6 floats, F / (rho g V) and T / (rho g V Lc)
+ * in the CG frame, of the hull at rest on a flat surface, against
+ *   s      = (CoB depth + g^.(aabb_c - P_CB)) / (g^.|aabb_h|) on ns nodes over [-1, 1],
+ *   theta  = acos(g^_z) on ntheta nodes over [0, pi],
+ *   phi    = atan2(g^_y, g^_x) on nphi periodic nodes over [-pi, pi),
+ * g^ the unit gravity direction in the CG frame (trilinear lookup).  The
+ * table is built with the exact kernel (surrogate.py build_hydrostatic_table)
+ * and must stay alive with the batch; NULL removes it. */
+int sfh_surrogate_set_table(sfh_batch batch, uint32_t first, uint32_t count, const float* d_table,
+                            uint32_t ns, uint32_t ntheta, uint32_t nphi);
 
 /* One-launch surrogate step.  A plan binds a batch whose units are grouped
  * by hull (counts[g] consecutive units per group, g < 16) to one network and

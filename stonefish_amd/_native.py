@@ -131,6 +131,8 @@ _PROTOS = {
                                       C.c_void_p]),
     "sfh_surrogate_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 8),
+    "sfh_surrogate_set_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32,
+                                          C.c_uint32]),
     "sfh_surrogate_plan_create": (C.c_int, [C.c_void_p, C.c_uint32, c_uint32_p, C.POINTER(sfh_surrogate_norm),
                                             C.POINTER(sfh_net_ptrs), C.POINTER(C.c_void_p)]),
     "sfh_surrogate_plan_destroy": (C.c_int, [C.c_void_p]),

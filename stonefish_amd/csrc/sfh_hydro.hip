@@ -365,60 +365,137 @@ struct Chunk {
     uint32_t body, first, count, pad;
 };
 
-// Per-step work lists.  k_prologue writes every body's position to bpos;
-// k_scan (one workgroup) turns the positions into each body's offsets in the
-// three lists (chunks of inside bodies, chunks of crossing bodies, vertex
-// chunks of crossing bodies) by an exclusive prefix sum in body order, and
-// k_scatter (one wavefront per body, grid-strided) writes the chunk ids.  The
-// passes then take list entries w, w + W, w + 2W, ... (W wavefronts), so every
-// wavefront gets the same number of chunks of its kind.
-static constexpr int kScanThreads = 1024;
+// Per-step work lists.  k_prologue writes every body's position to bpos; a
+// three-phase prefix sum in body order turns the positions into each body's
+// offsets in the three lists (chunks of inside bodies, chunks of crossing
+// bodies, vertex chunks of crossing bodies): k_scan_part sums blocks of
+// kScanBlock bodies, k_scan_top scans the block sums (one workgroup),
+// k_scan_off scans within each block; k_scatter (one wavefront per body,
+// grid-strided) writes the chunk ids.  The passes then take list entries
+// w, w + W, w + 2W, ... (W wavefronts), so every wavefront gets the same
+// number of chunks of its kind.
+static constexpr int kScanThreads = 256;
+static constexpr int kScanPer = 8;                              // consecutive bodies per thread
+static constexpr uint32_t kScanBlock = kScanThreads * kScanPer; // bodies per workgroup
+static constexpr int kTopThreads = 1024;
 
 struct BodyChunks {
     uint32_t chunk0, nchunks, vchunk0, nvchunks;
 };
 
-__global__ __launch_bounds__(kScanThreads) void k_scan(const int8_t* __restrict__ bpos,
-                                                       const BodyChunks* __restrict__ bch, uint32_t n,
-                                                       uint4* __restrict__ boff, uint32_t* __restrict__ counts)
+__device__ __forceinline__ void body_counts(const int8_t* __restrict__ bpos, const BodyChunks* __restrict__ bch,
+                                            uint32_t b, uint32_t& c0, uint32_t& c1, uint32_t& c2)
 {
-    __shared__ uint32_t wsum[3][kScanThreads / 64];
+    const int p = bpos[b];
+    const BodyChunks bc = bch[b];
+    c0 = (p == 1) ? bc.nchunks : 0u;
+    c1 = (p == 2) ? bc.nchunks : 0u;
+    c2 = (p == 2) ? bc.nvchunks : 0u;
+}
+
+// Exclusive scan of three counters over the threads of a workgroup (in
+// thread order); the workgroup totals land in tot of thread 0.
+template <int NT>
+__device__ __forceinline__ void block_scan3(uint32_t (&v)[3], uint32_t (&tot)[3])
+{
+    __shared__ uint32_t wsum[3][NT / 64];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = min(t * per, n), b1 = min(b0 + per, n);
-    uint32_t c0 = 0, c1 = 0, c2 = 0;
-    for (uint32_t b = b0; b < b1; ++b) {
-        const int p = bpos[b];
-        const BodyChunks bc = bch[b];
-        c0 += (p == 1) ? bc.nchunks : 0u;
-        c1 += (p == 2) ? bc.nchunks : 0u;
-        c2 += (p == 2) ? bc.nvchunks : 0u;
-    }
-    uint32_t i0 = c0, i1 = c1, i2 = c2;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y0 = __shfl_up(i0, off, 64), y1 = __shfl_up(i1, off, 64), y2 = __shfl_up(i2, off, 64);
-        if ((int)lane >= off) { i0 += y0; i1 += y1; i2 += y2; }
-    }
-    if (lane == 63) { wsum[0][w] = i0; wsum[1][w] = i1; wsum[2][w] = i2; }
+    uint32_t inc[3] = {v[0], v[1], v[2]};
+    for (int off = 1; off < 64; off <<= 1)
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t y = __shfl_up(inc[k], off, 64);
+            if ((int)lane >= off) inc[k] += y;
+        }
+    if (lane == 63)
+        for (int k = 0; k < 3; ++k) wsum[k][w] = inc[k];
     __syncthreads();
     if (t == 0) {
-        uint32_t r0 = 0, r1 = 0, r2 = 0;
-        for (int i = 0; i < kScanThreads / 64; ++i) {
-            const uint32_t x0 = wsum[0][i], x1 = wsum[1][i], x2 = wsum[2][i];
-            wsum[0][i] = r0; wsum[1][i] = r1; wsum[2][i] = r2;
-            r0 += x0; r1 += x1; r2 += x2;
-        }
-        counts[0] = r0; counts[1] = r1; counts[2] = r2;
+        uint32_t run[3] = {0, 0, 0};
+        for (int i = 0; i < NT / 64; ++i)
+            for (int k = 0; k < 3; ++k) { const uint32_t x = wsum[k][i]; wsum[k][i] = run[k]; run[k] += x; }
+        for (int k = 0; k < 3; ++k) tot[k] = run[k];
     }
     __syncthreads();
-    uint32_t o0 = wsum[0][w] + i0 - c0, o1 = wsum[1][w] + i1 - c1, o2 = wsum[2][w] + i2 - c2;
-    for (uint32_t b = b0; b < b1; ++b) {
-        const int p = bpos[b];
-        const BodyChunks bc = bch[b];
-        boff[b] = make_uint4(o0, o1, o2, 0u);
-        o0 += (p == 1) ? bc.nchunks : 0u;
-        o1 += (p == 2) ? bc.nchunks : 0u;
-        o2 += (p == 2) ? bc.nvchunks : 0u;
+    for (int k = 0; k < 3; ++k) v[k] = wsum[k][w] + inc[k] - v[k];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_part(const int8_t* __restrict__ bpos,
+                                                            const BodyChunks* __restrict__ bch, uint32_t n,
+                                                            uint4* __restrict__ part)
+{
+    __shared__ uint32_t red[3][kScanThreads / 64];
+    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    uint32_t c[3] = {0, 0, 0};
+    for (int j = 0; j < kScanPer; ++j) {
+        const uint32_t b = b0 + j;
+        if (b < n) {
+            uint32_t x0, x1, x2;
+            body_counts(bpos, bch, b, x0, x1, x2);
+            c[0] += x0; c[1] += x1; c[2] += x2;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int k = 0; k < 3; ++k) c[k] += __shfl_xor(c[k], off, 64);
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) red[k][w] = c[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s[3] = {0, 0, 0};
+        for (int i = 0; i < kScanThreads / 64; ++i)
+            for (int k = 0; k < 3; ++k) s[k] += red[k][i];
+        part[blockIdx.x] = make_uint4(s[0], s[1], s[2], 0u);
+    }
+}
+
+// One workgroup: exclusive scan of the block sums (in place) and the totals.
+__global__ __launch_bounds__(kTopThreads) void k_scan_top(uint4* __restrict__ part, uint32_t nblocks,
+                                                          uint32_t* __restrict__ counts)
+{
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nblocks + kTopThreads - 1) / kTopThreads;
+    const uint32_t i0 = min(t * per, nblocks), i1 = min(i0 + per, nblocks);
+    uint32_t v[3] = {0, 0, 0};
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint4 p = part[i];
+        v[0] += p.x; v[1] += p.y; v[2] += p.z;
+    }
+    uint32_t tot[3] = {0, 0, 0};
+    block_scan3<kTopThreads>(v, tot);
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint4 p = part[i];
+        part[i] = make_uint4(v[0], v[1], v[2], 0u);
+        v[0] += p.x; v[1] += p.y; v[2] += p.z;
+    }
+    if (t == 0) { counts[0] = tot[0]; counts[1] = tot[1]; counts[2] = tot[2]; }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_off(const int8_t* __restrict__ bpos,
+                                                           const BodyChunks* __restrict__ bch, uint32_t n,
+                                                           const uint4* __restrict__ part, uint4* __restrict__ boff)
+{
+    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    uint32_t c[3] = {0, 0, 0};
+    for (int j = 0; j < kScanPer; ++j) {
+        const uint32_t b = b0 + j;
+        if (b < n) {
+            uint32_t x0, x1, x2;
+            body_counts(bpos, bch, b, x0, x1, x2);
+            c[0] += x0; c[1] += x1; c[2] += x2;
+        }
+    }
+    uint32_t tot[3];
+    block_scan3<kScanThreads>(c, tot);
+    const uint4 base = part[blockIdx.x];
+    uint32_t o[3] = {base.x + c[0], base.y + c[1], base.z + c[2]};
+    for (int j = 0; j < kScanPer; ++j) {
+        const uint32_t b = b0 + j;
+        if (b < n) {
+            uint32_t x0, x1, x2;
+            body_counts(bpos, bch, b, x0, x1, x2);
+            boff[b] = make_uint4(o[0], o[1], o[2], 0u);
+            o[0] += x0; o[1] += x1; o[2] += x2;
+        }
     }
 }
 
@@ -875,6 +952,8 @@ struct UnitStatic {
     double aabb_c[3], aabb_h[3];
     double volume, surface;
     uint32_t hydro, buoyant;
+    const float* hst;            // hydrostatic table of the unit's hull (null: none)
+    uint32_t hs, hth, hph, pad_; // its dimensions: immersion, polar angle, azimuth
 };
 
 // Per compound: static data of Compound::ComputeHydrodynamicForces.
@@ -1069,6 +1148,7 @@ struct sfh_batch_t {
     int8_t* d_bpos = nullptr;       // per body: this step's position (-2 inactive, -1 not hydrodynamic, 0, 1, 2)
     BodyChunks* d_bch = nullptr;    // per body: its face and vertex chunk ranges
     uint4* d_boff = nullptr;        // per body: this step's offsets in the three lists
+    uint4* d_scan = nullptr;        // per block of kScanBlock bodies: sums, then exclusive offsets
     uint32_t* d_lists = nullptr;    // list_sub [nchunks], list_cross [nchunks], list_vert [nvchunks], counts [4]
     BodyDyn* d_dyn = nullptr;
     double* d_partials = nullptr;
@@ -1296,6 +1376,7 @@ static void batch_free(sfh_batch_t* b)
     hipFree(b->d_bpos);
     hipFree(b->d_bch);
     hipFree(b->d_boff);
+    hipFree(b->d_scan);
     hipFree(b->d_lists);
     hipFree(b->d_dyn);
     hipFree(b->d_partials);
@@ -1359,9 +1440,11 @@ static int batch_build_chunks(sfh_batch_t* b)
     hipFree(b->d_bpos);
     hipFree(b->d_bch);
     hipFree(b->d_boff);
+    hipFree(b->d_scan);
     hipFree(b->d_lists);
     b->d_static = nullptr; b->d_chunks = nullptr; b->d_vchunks = nullptr; b->d_depth = nullptr;
     b->d_partials = nullptr; b->d_bpos = nullptr; b->d_bch = nullptr; b->d_boff = nullptr; b->d_lists = nullptr;
+    b->d_scan = nullptr;
     b->nchunks = (uint32_t)chunks.size();
     b->nvchunks = (uint32_t)vchunks.size();
     b->nverts = dep;
@@ -1387,6 +1470,7 @@ static int batch_build_chunks(sfh_batch_t* b)
         if (b->n) HIPCHK(hipMemcpy(b->d_bch, bch.data(), sizeof(BodyChunks) * b->n, hipMemcpyHostToDevice));
     }
     HIPCHK(hipMalloc(&b->d_boff, sizeof(uint4) * std::max<size_t>(1, b->n)));
+    HIPCHK(hipMalloc(&b->d_scan, sizeof(uint4) * std::max<size_t>(1, (b->n + kScanBlock - 1) / kScanBlock)));
     HIPCHK(hipMalloc(&b->d_lists, sizeof(uint32_t) * (2 * (size_t)b->nchunks + b->nvchunks + 8)));
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -1609,7 +1693,11 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
     uint32_t* list_cross = list_sub + b->nchunks;
     uint32_t* list_vert = list_cross + b->nchunks;
     uint32_t* counts = list_vert + b->nvchunks;
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_boff, counts);
+    const uint32_t nblk = std::max<uint32_t>(1, (b->n + kScanBlock - 1) / kScanBlock);
+    hipLaunchKernelGGL(k_scan_part, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kTopThreads), 0, s, b->d_scan, nblk, counts);
+    hipLaunchKernelGGL(k_scan_off, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan,
+                       b->d_boff);
     const uint32_t swaves = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
     hipLaunchKernelGGL(k_scatter, dim3((swaves + 3) / 4), dim3(256), 0, s, b->d_bpos, b->d_bch, b->d_boff, b->n,
                        list_sub, list_cross, list_vert);
@@ -2037,6 +2125,32 @@ struct SurCtx {
     int pos;
 };
 
+// Trilinear lookup of a hydrostatic table: s in [-1, 1] (immersion of the
+// AABB: -1 top at the surface, 1 bottom at the surface), gh the unit gravity
+// direction in the CG frame, as polar angle acos(gh.z) on hth nodes over
+// [0, pi] and azimuth atan2(gh.y, gh.x) on hph periodic nodes over [-pi, pi).
+// Entries: 6 floats, F / (rho g V) and T / (rho g V Lc), CG frame.
+__device__ void hst_lookup(const UnitStatic& U, double sv, const double* gh, double* out)
+{
+    const double th = acos(fmin(1.0, fmax(-1.0, gh[2])));
+    const double ph = atan2(gh[1], gh[0]);
+    double fs = (fmin(1.0, fmax(-1.0, sv)) + 1.0) * 0.5 * (double)(U.hs - 1);
+    double ft = th * (1.0 / 3.141592653589793) * (double)(U.hth - 1);
+    double fp = (ph + 3.141592653589793) * (0.5 / 3.141592653589793) * (double)U.hph;
+    int is = min((int)fs, (int)U.hs - 2), it = min((int)ft, (int)U.hth - 2), ip = (int)floor(fp);
+    const double as = fs - is, at = ft - it, ap = fp - ip;
+    ip %= (int)U.hph;
+    if (ip < 0) ip += (int)U.hph;
+    const int ip1 = (ip + 1 == (int)U.hph) ? 0 : ip + 1;
+    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    for (int c = 0; c < 8; ++c) {
+        const int ds = c & 1, dt = (c >> 1) & 1, dp = (c >> 2) & 1;
+        const double wgt = (ds ? as : 1.0 - as) * (dt ? at : 1.0 - at) * (dp ? ap : 1.0 - ap);
+        const size_t idx = (((size_t)(is + ds) * U.hth + (size_t)(it + dt)) * U.hph + (size_t)(dp ? ip1 : ip)) * 6;
+        for (int k = 0; k < 6; ++k) out[k] += wgt * (double)U.hst[idx + k];
+    }
+}
+
 __device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc, SurCtx& C)
 {
     gl_basis(s, C.R);
@@ -2054,10 +2168,30 @@ __device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, c
     f[10] = cb[2];
     f[11] = f[12] = 0.0;
     if (oc.nlayers) {
+        // the local surface is the plane fitted to the wave field over the
+        // body's horizontal footprint (half extents ex, ey of its world AABB,
+        // at least one texel): weights (1/4, 1/2, 1/4) on a 3 x 3 stencil give
+        // its height at the CoB and least squares its slope
         const double d = oc.len[0] / (double)oc.n;
-        f[10] += wave_sum64(oc, cb[0], cb[1]);
-        f[11] = (wave_sum64(oc, cb[0] + d, cb[1]) - wave_sum64(oc, cb[0] - d, cb[1])) / (2.0 * d);
-        f[12] = (wave_sum64(oc, cb[0], cb[1] + d) - wave_sum64(oc, cb[0], cb[1] - d)) / (2.0 * d);
+        double ex = 0.0, ey = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            ex += fabs(C.R[j]) * U.aabb_h[j];
+            ey += fabs(C.R[3 + j]) * U.aabb_h[j];
+        }
+        ex = fmax(ex, d);
+        ey = fmax(ey, d);
+        const double w1[3] = {0.25, 0.5, 0.25};
+        double hm = 0.0, sx = 0.0, sy = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) {
+                const double h = wave_sum64(oc, cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey);
+                hm += w1[a] * w1[c] * h;
+                sx += w1[a] * w1[c] * (a - 1) * h;
+                sy += w1[a] * w1[c] * (c - 1) * h;
+            }
+        f[10] += hm;
+        f[11] = sx / (0.5 * ex);   // sum of w (a-1)^2 = 1/2
+        f[12] = sy / (0.5 * ey);
     }
     // scales
     const double Lc = fmax(fmax(U.aabb_h[0], U.aabb_h[1]), U.aabb_h[2]);
@@ -2067,14 +2201,29 @@ __device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, c
     if (!(C.sF > 0.0)) C.sF = 1.0;
     C.sT = C.sF * fmax(Lc, 1e-6);
     // hydrostatic estimate
+    double gb[3];
+    mat_t_vec(C.R, oc.g, gb);
+    if (U.buoyant && C.pos == 2 && U.hst && gn > 0.0) {
+        // the hull's table: static buoyancy on a flat surface against the
+        // immersion of its AABB and the direction of gravity (CG frame)
+        double gh[3] = {gb[0] / gn, gb[1] / gn, gb[2] / gn};
+        const double hz = fabs(gh[0]) * U.aabb_h[0] + fabs(gh[1]) * U.aabb_h[1] + fabs(gh[2]) * U.aabb_h[2];
+        double sv = 0.0;
+        for (int k = 0; k < 3; ++k) sv += gh[k] * (U.aabb_c[k] - U.P_CB[k]);
+        sv = hz > 0.0 ? (f[10] + sv) / hz : 0.0;
+        double tab[6];
+        hst_lookup(U, sv, gh, tab);
+        const double sb = oc.rho * gn * U.volume;
+        for (int k = 0; k < 3; ++k) { C.base[k] = sb * tab[k]; C.base[3 + k] = sb * Lc * tab[3 + k]; }
+        return;
+    }
     double frac = 0.0;
     if (U.buoyant && C.pos == 1) frac = 1.0;
     if (U.buoyant && C.pos == 2) {
-        const double hz = fabs(C.R[6]) * U.aabb_h[0] + fabs(C.R[7]) * U.aabb_h[1] + fabs(C.R[8]) * U.aabb_h[2];
+        const double hz = gn > 0.0 ? (fabs(gb[0]) * U.aabb_h[0] + fabs(gb[1]) * U.aabb_h[1] + fabs(gb[2]) * U.aabb_h[2]) / gn
+                                   : 0.0;
         frac = hz > 0.0 ? fmin(1.0, fmax(0.0, (f[10] + hz) / (2.0 * hz))) : 0.0;
     }
-    double gb[3];
-    mat_t_vec(C.R, oc.g, gb);
     for (int k = 0; k < 3; ++k) C.base[k] = -oc.rho * U.volume * frac * gb[k];
     cross64(U.P_CB, C.base, C.base + 3);
 }
@@ -2522,5 +2671,24 @@ extern "C" int sfh_surrogate_merge(sfh_sur_plan p, const uint8_t* d_ood, const d
     hipLaunchKernelGGL(k_sur_merge, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream, n, d_ood, d_exact, d_ft,
                        d_err);
     HIPCHK(hipGetLastError());
+    return SFH_OK;
+}
+
+extern "C" int sfh_surrogate_set_table(sfh_batch b, uint32_t first, uint32_t count, const float* d_table,
+                                       uint32_t ns, uint32_t ntheta, uint32_t nphi)
+{
+    if (!b) return fail(SFH_ERR_INVALID, "null batch");
+    if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "unit range out of the batch");
+    if (d_table && (ns < 2 || ntheta < 2 || nphi < 2)) return fail(SFH_ERR_SIZE, "table dimensions must be >= 2");
+    if (count == 0) return SFH_OK;
+    std::vector<UnitStatic> u(count);
+    HIPCHK(hipMemcpy(u.data(), b->d_units + first, sizeof(UnitStatic) * count, hipMemcpyDeviceToHost));
+    for (UnitStatic& x : u) {
+        x.hst = d_table;
+        x.hs = d_table ? ns : 0;
+        x.hth = d_table ? ntheta : 0;
+        x.hph = d_table ? nphi : 0;
+    }
+    HIPCHK(hipMemcpy(b->d_units + first, u.data(), sizeof(UnitStatic) * count, hipMemcpyHostToDevice));
     return SFH_OK;
 }

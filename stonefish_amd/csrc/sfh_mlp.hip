@@ -103,10 +103,17 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     const uint16_t* ap = g.A + (size_t)(tm * 32 + (lane & 31)) * g.lda + k0 + kk;
     const uint16_t* bp = g.Bt + (size_t)(tn * 32 + (lane & 31)) * g.ldb + k0 + kk;
     f32x16 acc = {};
+    // operands of the next k-step are loaded while the current one multiplies
+    bf16x8 a = ld8(ap), b = ld8(bp);
     for (int k = 0; k < kper; k += 16) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(ap + k);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(bp + k);
+        bf16x8 na = a, nb = b;
+        if (k + 16 < kper) {
+            na = ld8(ap + k + 16);
+            nb = ld8(bp + k + 16);
+        }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        a = na;
+        b = nb;
     }
     const int n = tn * 32 + (lane & 31);
     const int mb = tm * 32;
@@ -210,15 +217,25 @@ __global__ void k_transpose(const uint16_t* __restrict__ x, int rows, int cols, 
     xt[(size_t)c * rows + r] = x[i];
 }
 
-// Row sums of xt [n x m] (BF16): one wavefront per row, fixed summation order.
-__global__ __launch_bounds__(64) void k_rowsum(const uint16_t* __restrict__ xt, int m, float* __restrict__ out)
+// Row sums of xt [n x m] (BF16): one 256-thread workgroup per row, each
+// thread summing 4 consecutive values at a time, then a fixed-order tree
+// (deterministic).  m is a multiple of 256.
+__global__ __launch_bounds__(256) void k_rowsum(const uint16_t* __restrict__ xt, int m, float* __restrict__ out)
 {
-    const int row = blockIdx.x, lane = threadIdx.x;
+    __shared__ float red[4];
+    const int row = blockIdx.x, t = threadIdx.x;
+    const uint16_t* p = xt + (size_t)row * m;
     float s = 0.f;
-    for (int i = lane; i < m; i += 64) s += __uint_as_float((uint32_t)xt[(size_t)row * m + i] << 16);
+    for (int i = 4 * t; i < m; i += 1024) {
+        const uint2 v = *reinterpret_cast<const uint2*>(p + i);
+        s += __uint_as_float(v.x << 16) + __uint_as_float(v.x & 0xffff0000u) + __uint_as_float(v.y << 16) +
+             __uint_as_float(v.y & 0xffff0000u);
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) out[row] = s;
+    if ((t & 63) == 0) red[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) out[row] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // Adam on n parameters of a [rows x cols] matrix (cols = 1 for biases): the
@@ -589,7 +606,7 @@ extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target,
             g.ksplit = splits;
             g.part = n->part;
             launch_gemm<EPI_WGRAD>(g, s);
-            hipLaunchKernelGGL(k_rowsum, dim3(dout), dim3(64), 0, s, n->dZt[cur], M, n->dbias);
+            hipLaunchKernelGGL(k_rowsum, dim3(dout), dim3(256), 0, s, n->dZt[cur], M, n->dbias);
             // dH = dZ W  (with the pre-update weights), then dZ of the layer below
             if (l > 0) {
                 GemmArgs h = gemm(n->dZ[cur], dout, L.Wbt, dout, M, din, dout);

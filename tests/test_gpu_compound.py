@@ -113,8 +113,15 @@ def test_compound_masked_and_features(gpu):
     raw = torch.zeros(n, 13, dtype=torch.float32, device="cuda")
     S.features(batch, go, ds, 0, n, raw=raw)
     r = raw.cpu().numpy()
+    import ctypes
+    half = 0.5 * (auv.aabb_max - auv.aabb_min)
+    wts = np.outer([0.25, 0.5, 0.25], [0.25, 0.5, 0.25])
     for i in range(0, n, 9):
         Rm = st[i, :16].reshape(4, 4).T[:3, :3]
         cb = Rm @ auv.P_CB + st[i, 12:15]
-        d = O.load().orc_depth_64(__import__("ctypes").byref(oo.s), O._d(cb))
+        # depth below the plane fitted over the footprint of the compound's AABB
+        ex, ey = max(np.abs(Rm[0]) @ half, 1.0), max(np.abs(Rm[1]) @ half, 1.0)
+        Hs = np.array([[-O.load().orc_wave_height_64(ctypes.byref(oo.s), cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey)
+                        for c in range(3)] for a in range(3)])
+        d = cb[2] + (wts * Hs).sum()
         assert abs(r[i, 10] - d) <= 2e-6 * (1 + abs(d))

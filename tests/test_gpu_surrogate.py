@@ -213,11 +213,15 @@ def test_features_match_oracle(gpu):
         q = _quat(Rm)
         exp[6:10] = q
         cb = Rm @ np.array([0.05, -0.02, 0.01]) + t
-        exp[10] = lib.orc_depth_64(C_ocean(ocn), O._d(cb))
-        d = 1.0
+        # local surface: plane fitted over the footprint of the body's world AABB
         hx = lambda x, y: -lib.orc_wave_height_64(C_ocean(ocn), x, y)   # noqa: E731
-        exp[11] = (hx(cb[0] + d, cb[1]) - hx(cb[0] - d, cb[1])) / (2 * d)
-        exp[12] = (hx(cb[0], cb[1] + d) - hx(cb[0], cb[1] - d)) / (2 * d)
+        half = 0.5 * np.ptp(hulls[i // 256].verts.astype(np.float64), axis=0)
+        ex, ey = max(np.abs(Rm[0]) @ half, 1.0), max(np.abs(Rm[1]) @ half, 1.0)
+        wts = np.outer([0.25, 0.5, 0.25], [0.25, 0.5, 0.25])
+        H = np.array([[hx(cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey) for c in range(3)] for a in range(3)])
+        exp[10] = cb[2] + (wts * H).sum()
+        exp[11] = (wts * H * (np.arange(3) - 1)[:, None]).sum() / (0.5 * ex)
+        exp[12] = (wts * H * (np.arange(3) - 1)[None, :]).sum() / (0.5 * ey)
         assert np.allclose(raw[i], exp, rtol=2e-6, atol=2e-6), (i, raw[i], exp)
     # the targets are checked by their round trip through the decode (test_masked_compute_and_apply)
 
@@ -397,3 +401,31 @@ def test_one_launch_step_matches_separate_kernels(gpu):
 
 def norms_for(norms, counts):
     return [norms[0], norms[1]][:len(counts)]
+
+
+def test_hydrostatic_table_reproduces_static_buoyancy(gpu):
+    """A hull at rest on a flat sea: with its hydrostatic table the
+    surrogate's baseline is the exact static buoyancy up to the table's
+    interpolation error, so the network's target (the residual in units of
+    the body's scales) is small; without the table it is not."""
+    from stonefish_amd import surrogate as S
+    batch, ocean, hulls, grid = _small_fleet(waves=False, n_hulls=1, per_hull=256)
+    rng = np.random.default_rng(41)
+    st = S.sample_states(rng, batch.n, xy_range=(0.0, 10.0), depth_range=(-0.2, 0.2), v_max=0.0, w_max=0.0)
+    ds = torch.from_numpy(st).cuda()
+    exact = torch.zeros(batch.n, 24, dtype=torch.float64, device="cuda")
+    batch.compute(ocean, ds.data_ptr(), exact.data_ptr())
+    def residual():
+        rt = torch.zeros(batch.n, 6, dtype=torch.float32, device="cuda")
+        pos = torch.zeros(batch.n, dtype=torch.int8, device="cuda")
+        S.features(batch, ocean, ds, 0, batch.n, pos=pos, exact=exact, raw_target=rt)
+        p = pos.cpu().numpy()
+        return rt.cpu().numpy()[p == 2]
+    before = np.abs(residual()[:, :3]).max()
+    tables = S.attach_tables(batch, ocean, [(0, batch.n)], ns=33, nth=17, nph=32)
+    r = residual()
+    assert r.shape[0] > 50
+    after = np.abs(r[:, :3]).max()
+    print(f"[surrogate] static residual max {before:.4f} without table, {after:.5f} with", flush=True)
+    assert after < 0.02 and after < 0.2 * before
+    del tables
