@@ -255,4 +255,29 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
     return r;
 }
 
+// All-wet face (case G of face_clip, every vertex depth >= 0): the same
+// operations in the same order, without the case analysis.
+template <typename T>
+SFH_HD Wet face_allwet(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
+{
+    Wet r;
+    r.ok = false;
+    const f3 base = p1;
+    const f3 fv1 = p2 - p1, fv2 = p3 - p1;
+    f3 n = cross(fv1, fv2);
+    tetra_term(base - B.o, f3{0.f, 0.f, 0.f}, n, fv1 + fv2, acc);
+    const f3 csum = fv1 + fv2;
+    n = cross(fv1, fv2);
+    float n2 = dot(n, n);
+    if (n2 < SFH_DEGENERATE_N2) return r;
+    float inv = frsq(n2);
+    float len = n2 * inv;
+    r.n1 = n * inv;
+    r.base = base;
+    r.A = 0.5f * len;
+    r.off = csum * (1.0f / 3.0f);
+    r.ok = true;
+    return r;
+}
+
 }  // namespace sfh

@@ -707,9 +707,31 @@ __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ 
     }
 }
 
+// Contributions of a wet (possibly clipped) face: pressure buoyancy at the
+// wet centroid when the surface has waves (1649-1658), drag, wetted area.
+__device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, const BodyDyn& D, const OceanK& oc,
+                                            bool wave_buoy, bool damping, Acc<float>& acc)
+{
+    if (!wt.ok) return;
+    f3 cpt = wt.base + wt.off;
+    if (wave_buoy) {
+        float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x, (double)wt.base.y + (double)wt.off.y,
+                                  (double)wt.base.z + (double)wt.off.z);
+        f3 fbi = wt.n1 * (-wt.A * dc);
+        add3(acc.Fb, fbi);
+        add3(acc.Mb, cross(cpt, fbi));
+    }
+    if (damping) face_drag(fb, cpt, wt.n1, wt.A, acc);
+    acc.Swet += wt.A;
+}
+
 // Surface loop (bodies crossing the surface): waterline clip with the vertex
 // depths of k_depths, tetra volumes, pressure buoyancy with waves, drag; the
 // chunks of this step's crossing bodies.
+// The chunk's all-wet faces are processed as they come; faces the waterline
+// cuts (clip cases A-F) are queued in LDS by each wavefront and processed
+// together afterwards, 64 at a time, so the clip code runs once per 64 cut
+// faces rather than on every pass over the chunk.
 template <int K>
 __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
@@ -718,7 +740,10 @@ __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict_
                                                      const BodyDyn* __restrict__ dyn, OceanK oc,
                                                      const float* __restrict__ depth, double* __restrict__ partials)
 {
+    __shared__ uint16_t cutq[4][64 * K];
     const int lane = threadIdx.x & 63;
+    const int wl = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t n = count[0];
@@ -745,31 +770,36 @@ __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict_
         uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
         f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
         float d0 = dep[a0], d1 = dep[a1], d2 = dep[a2];
+        uint32_t ncut = 0;
         for (int j = 0; j < K; ++j) {
             const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
             f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
             float e0 = dep[b0], e1 = dep[b1], e2 = dep[b2];
             uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
-            if (64u * j + lane < nfc && !(d0 < 0.f && d1 < 0.f && d2 < 0.f)) {
-                Wet wt = face_clip(fb, p1, p2, p3, d0, d1, d2, acc);
-                if (wt.ok) {
-                    f3 cpt = wt.base + wt.off;
-                    if (wave_buoy) {   // pressure at the wet centroid (1649-1658)
-                        float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x,
-                                                  (double)wt.base.y + (double)wt.off.y,
-                                                  (double)wt.base.z + (double)wt.off.z);
-                        f3 fbi = wt.n1 * (-wt.A * dc);
-                        add3(acc.Fb, fbi);
-                        add3(acc.Mb, cross(cpt, fbi));
-                    }
-                    if (damping) face_drag(fb, cpt, wt.n1, wt.A, acc);
-                    acc.Swet += wt.A;
-                }
-            }
+            const bool valid = 64u * j + lane < nfc;
+            const bool allwet = !(d0 < 0.f) && !(d1 < 0.f) && !(d2 < 0.f);
+            const bool cut = valid && !allwet && !(d0 < 0.f && d1 < 0.f && d2 < 0.f);
+            if (valid && allwet) wet_contrib(face_allwet(fb, p1, p2, p3, acc), fb, D, oc, wave_buoy, damping, acc);
+            const uint64_t m = __ballot(cut);
+            if (cut) cutq[wl][ncut + (uint32_t)__popcll(m & below)] = (uint16_t)(64u * j + lane);
+            ncut += (uint32_t)__popcll(m);
             p1 = q1; p2 = q2; p3 = q3;
             d0 = e0; d1 = e1; d2 = e2;
             b0 = c0; b1 = c1; b2 = c2;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t qb = 0; qb < ncut; qb += 64) {
+            if (qb + lane < ncut) {
+                const uint32_t fo = cutq[wl][qb + lane];
+                const uint32_t i0 = faces[3 * fo], i1 = faces[3 * fo + 1], i2 = faces[3 * fo + 2];
+                const Wet wt = face_clip(fb, vert(verts, i0), vert(verts, i1), vert(verts, i2), dep[i0], dep[i1],
+                                         dep[i2], acc);
+                wet_contrib(wt, fb, D, oc, wave_buoy, damping, acc);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();   // the queue is rewritten by the next chunk
         // chunk partial layout: Fdq Mdq Fdf Mdf Fb Mb CB (3 each), V6, Swet
         double vals[24];
 #pragma unroll
