@@ -386,7 +386,7 @@ def main() -> int:
         "config": {"workload": f"fleet of {args.bodies} bodies (ellipsoids/boxes, half-axes U[0.1,2] m, "
                                f"faces log-U[500,50000]), depth U[-1,3] m, 1024x1024 PM wave grid",
                    "bodies": args.bodies, "faces": int(nfaces), "vertices": int(nv.sum()), "chunks": nchunks,
-                   "graph": bool(args.graph), "faces_per_lane": args.faces_per_lane or 8},
+                   "graph": bool(args.graph), "faces_per_lane": args.faces_per_lane or 16},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_faces", "kernel_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),

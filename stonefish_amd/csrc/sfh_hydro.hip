@@ -128,6 +128,8 @@ struct OceanK {
     const float4* quad[kMaxLayers];   // per layer: (h00, h10, h01, h11) per texel
     int32_t nlayers;                  // 0 = flat surface
     int32_t n;
+    int32_t nmask;                    // n - 1 when n is a power of two, else -1
+    int32_t pad_;
 };
 
 // ---------------------------------------------------------------------------
@@ -158,8 +160,14 @@ __device__ __forceinline__ double depth64(const BodyDyn& D, const OceanK& oc, do
         double v = L.fv + L.rv[0] * x + L.rv[1] * y + L.rv[2] * z;
         double fi = floor(u), fj = floor(v);
         double a = u - fi, b = v - fj;
-        int i = wrap_n(L.ku + (int)fi, oc.n);
-        int j = wrap_n(L.kv + (int)fj, oc.n);
+        int i, j;
+        if (oc.nmask >= 0) {   // power-of-two grid: the wrap is a mask (two's complement)
+            i = (L.ku + (int)fi) & oc.nmask;
+            j = (L.kv + (int)fj) & oc.nmask;
+        } else {
+            i = wrap_n(L.ku + (int)fi, oc.n);
+            j = wrap_n(L.kv + (int)fj, oc.n);
+        }
         float4 t = oc.quad[l][(size_t)j * oc.n + i];
         d += (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
     }
@@ -1200,6 +1208,7 @@ static OceanK make_oceank(const sfh_ocean_t* o)
     for (int i = 0; i < 3; ++i) k.g[i] = o->g[i];
     k.nlayers = o->nlayers;
     k.n = (int32_t)o->n;
+    k.nmask = (o->n && (o->n & (o->n - 1)) == 0) ? (int32_t)o->n - 1 : -1;
     for (int l = 0; l < kMaxLayers; ++l) {
         k.len[l] = o->len[l];
         k.quad[l] = o->d_quad[l];
@@ -1425,7 +1434,7 @@ static uint32_t default_waves()
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
     }
-    return (uint32_t)cus * 32u;
+    return (uint32_t)cus * 64u;
 }
 
 // (Re)builds the work lists: every body's faces cut into chunks of 64*K
@@ -1509,7 +1518,7 @@ static int batch_build_chunks(sfh_batch_t* b)
 
 static int apply_opts(sfh_batch_t* b, const sfh_batch_opts* opts)
 {
-    uint32_t K = opts && opts->faces_per_lane ? opts->faces_per_lane : 8u;
+    uint32_t K = opts && opts->faces_per_lane ? opts->faces_per_lane : 16u;
     if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16)
         return fail(SFH_ERR_INVALID, "faces_per_lane must be 1, 2, 4, 8 or 16");
     b->waves = opts && opts->waves ? opts->waves : default_waves();
