@@ -101,6 +101,19 @@ def parity_sample(w, states, grid, out_gpu: np.ndarray, n: int = 96) -> dict:
             "norm_rel_err_max": {k: float(f"{v:.3e}") for k, v in plain.items()}}
 
 
+def measured_traffic():
+    """HBM bytes per step of the face passes from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (profiles/), or None."""
+    p = ROOT / "profiles" / "k_faces_fleet4096.txt"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_step"):
+                return int(line.split()[1])
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
+
+
 def _time_replays(fn, k: int) -> float:
     """Seconds per call of fn over k calls (after 3 untimed), bracketed by syncs."""
     import torch
@@ -368,6 +381,7 @@ def main() -> int:
     batch.timing(False)
 
     face_bytes = int((12 * nf + 12 * nv).sum()) + n * n * 16   # indices + vertices + wave quad texture
+    traffic = measured_traffic()
     achieved = face_bytes / (face_ms * 1e-3) / 1e9
     nb, nchunks, nfaces = batch.info()
     res = {
@@ -388,7 +402,7 @@ def main() -> int:
                    "bodies": args.bodies, "faces": int(nfaces), "vertices": int(nv.sum()), "chunks": nchunks,
                    "graph": bool(args.graph), "faces_per_lane": args.faces_per_lane or 16},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_faces", "kernel_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),
                      "algorithmic_bytes": face_bytes, "timed_launches": count},
     }
