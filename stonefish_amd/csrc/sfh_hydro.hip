@@ -124,6 +124,7 @@ struct OceanK {
     double rho;
     double g[3];
     double len[kMaxLayers];
+    double scale[kMaxLayers];         // n / len, texels per metre
     double u[3];                      // sum of enabled uniform currents
     const float4* quad[kMaxLayers];   // per layer: (h00, h10, h01, h11) per texel
     int32_t nlayers;                  // 0 = flat surface
@@ -179,18 +180,26 @@ __device__ __forceinline__ float depth_vertex(const BodyDyn& D, const OceanK& oc
     return (float)depth64(D, oc, p.x, p.y, p.z);
 }
 
+// floor(u) mod n in [0, n) for a texel coordinate floor fu.
+__device__ __forceinline__ int wrap_texel(double fu, const OceanK& oc)
+{
+    if (oc.nmask >= 0) return (int)((long long)fu & oc.nmask);   // power-of-two grid (two's complement)
+    long long k = (long long)fu % oc.n;
+    if (k < 0) k += oc.n;
+    return (int)k;
+}
+
 // FP64 wave height sum at world (x, y), for the classification corners.
+
 __device__ double wave_sum64(const OceanK& oc, double x, double y)
 {
     double h = 0.0;
     for (int l = 0; l < oc.nlayers; ++l) {
-        double s = (double)oc.n / oc.len[l];
+        const double s = oc.scale[l];   // the same IEEE quotient n / len the host formed
         double u = x * s - 0.5, v = y * s - 0.5;
         double fi = floor(u), fj = floor(v);
         double a = u - fi, b = v - fj;
-        long long ii = (long long)fi % oc.n, jj = (long long)fj % oc.n;
-        if (ii < 0) ii += oc.n;
-        if (jj < 0) jj += oc.n;
+        const int ii = wrap_texel(fi, oc), jj = wrap_texel(fj, oc);
         float4 t = oc.quad[l][(size_t)jj * oc.n + (size_t)ii];
         h += (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
     }
@@ -222,10 +231,23 @@ __device__ __forceinline__ void cross64(const double* a, const double* b, double
 
 __device__ __forceinline__ f3 to_f3(const double* x) { return mk3((float)x[0], (float)x[1], (float)x[2]); }
 
+// Sum over aligned groups of G lanes.
+template <int G, typename T>
+__device__ __forceinline__ T gsum(T x)
+{
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
 // CheckBodyFluidPosition (SolidEntity.cpp:1239-1261) on the world AABB
 // btCompoundShape::getAabb(T_CG) of the CG-frame AABB (centre c, half extents
-// h): 0 outside, 1 inside, 2 crossing.
-__device__ int classify64(const double* Rcg, const double* tcg, const double* ac, const double* ah, const OceanK& oc)
+// h): 0 outside, 1 inside, 2 crossing.  The eight wave queries are spread
+// over a group of G lanes (lane `sub` of the group takes corners sub,
+// sub + G, ...); every lane of the group returns the result.
+template <int G>
+__device__ int classify64_g(const double* Rcg, const double* tcg, const double* ac, const double* ah, const OceanK& oc,
+                            int sub)
 {
     double c[3], e[3], mn[3], mx[3], d[3];
     for (int i = 0; i < 3; ++i) {
@@ -236,7 +258,7 @@ __device__ int classify64(const double* Rcg, const double* tcg, const double* ac
         d[i] = mx[i] - mn[i];
     }
     int under = 0;
-    for (int k = 0; k < 8; ++k) {
+    for (int k = sub; k < 8; k += G) {
         double q[3];
         if (k == 1) { q[0] = mx[0]; q[1] = mx[1]; q[2] = mx[2]; }
         else {
@@ -252,7 +274,14 @@ __device__ int classify64(const double* Rcg, const double* tcg, const double* ac
         if (oc.nlayers) depth += wave_sum64(oc, q[0], q[1]);
         if (depth > 0.0) ++under;
     }
+    under = gsum<G>(under);
     return under == 0 ? 0 : (under == 8 ? 1 : 2);
+}
+
+__device__ __forceinline__ int classify64(const double* Rcg, const double* tcg, const double* ac, const double* ah,
+                                          const OceanK& oc)
+{
+    return classify64_g<1>(Rcg, tcg, ac, ah, oc, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -345,15 +374,13 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
         L.fu = L.fv = 0.0; L.ku = L.kv = 0;
         for (int k = 0; k < 3; ++k) L.ru[k] = L.rv[k] = 0.0;
         if (l < oc.nlayers) {
-            double s = (double)oc.n / oc.len[l];
+            const double s = oc.scale[l];
             double bu = tc[0] * s - 0.5, bv = tc[1] * s - 0.5;
             double fu = floor(bu), fv = floor(bv);
             L.fu = bu - fu;
             L.fv = bv - fv;
-            long long ku = (long long)fu % oc.n, kv = (long long)fv % oc.n;
-            if (ku < 0) ku += oc.n;
-            if (kv < 0) kv += oc.n;
-            L.ku = (int)ku; L.kv = (int)kv;
+            L.ku = wrap_texel(fu, oc);
+            L.kv = wrap_texel(fv, oc);
             for (int k = 0; k < 3; ++k) { L.ru[k] = R[k] * s; L.rv[k] = R[3 + k] * s; }
         }
         D.layer[l] = L;
@@ -1211,6 +1238,7 @@ static OceanK make_oceank(const sfh_ocean_t* o)
     k.nmask = (o->n && (o->n & (o->n - 1)) == 0) ? (int32_t)o->n - 1 : -1;
     for (int l = 0; l < kMaxLayers; ++l) {
         k.len[l] = o->len[l];
+        k.scale[l] = o->len[l] > 0.0 ? (double)o->n / o->len[l] : 0.0;
         k.quad[l] = o->d_quad[l];
     }
     if (o->currents_enabled)
@@ -2171,8 +2199,9 @@ struct SurCtx {
 // Entries: 6 floats, F / (rho g V) and T / (rho g V Lc), CG frame.
 __device__ void hst_lookup(const UnitStatic& U, double sv, const double* gh, double* out)
 {
-    const double th = acos(fmin(1.0, fmax(-1.0, gh[2])));
-    const double ph = atan2(gh[1], gh[0]);
+    // angles in FP32: the table's node spacing is several degrees
+    const double th = (double)acosf((float)fmin(1.0, fmax(-1.0, gh[2])));
+    const double ph = (double)atan2f((float)gh[1], (float)gh[0]);
     double fs = (fmin(1.0, fmax(-1.0, sv)) + 1.0) * 0.5 * (double)(U.hs - 1);
     double ft = th * (1.0 / 3.141592653589793) * (double)(U.hth - 1);
     double fp = (ph + 3.141592653589793) * (0.5 / 3.141592653589793) * (double)U.hph;
@@ -2190,7 +2219,11 @@ __device__ void hst_lookup(const UnitStatic& U, double sv, const double* gh, dou
     }
 }
 
-__device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc, SurCtx& C)
+// The context of one unit computed by a group of G lanes (G = 1: one lane);
+// every lane of the group ends with the full context.  The wave queries of
+// the classification and of the footprint plane are spread over the group.
+template <int G>
+__device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc, SurCtx& C, int sub)
 {
     gl_basis(s, C.R);
     for (int k = 0; k < 3; ++k) C.t[k] = s[12 + k];
@@ -2200,7 +2233,10 @@ __device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, c
     mat_t_vec(C.R, vr, f);
     mat_t_vec(C.R, s + 19, f + 3);
     rot_to_quat(C.R, f + 6);
-    C.pos = U.hydro ? classify64(C.R, C.t, U.aabb_c, U.aabb_h, oc) : -1;
+    {
+        const int p = classify64_g<G>(C.R, C.t, U.aabb_c, U.aabb_h, oc, sub);
+        C.pos = U.hydro ? p : -1;
+    }
     double cb[3];
     mat_vec(C.R, U.P_CB, cb);
     for (int k = 0; k < 3; ++k) cb[k] += C.t[k];
@@ -2221,13 +2257,16 @@ __device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, c
         ey = fmax(ey, d);
         const double w1[3] = {0.25, 0.5, 0.25};
         double hm = 0.0, sx = 0.0, sy = 0.0;
-        for (int a = 0; a < 3; ++a)
-            for (int c = 0; c < 3; ++c) {
-                const double h = wave_sum64(oc, cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey);
-                hm += w1[a] * w1[c] * h;
-                sx += w1[a] * w1[c] * (a - 1) * h;
-                sy += w1[a] * w1[c] * (c - 1) * h;
-            }
+        for (int k = sub; k < 9; k += G) {
+            const int a = k / 3, c = k - 3 * (k / 3);
+            const double h = wave_sum64(oc, cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey);
+            hm += w1[a] * w1[c] * h;
+            sx += w1[a] * w1[c] * (a - 1) * h;
+            sy += w1[a] * w1[c] * (c - 1) * h;
+        }
+        hm = gsum<G>(hm);
+        sx = gsum<G>(sx);
+        sy = gsum<G>(sy);
         f[10] += hm;
         f[11] = sx / (0.5 * ex);   // sum of w (a-1)^2 = 1/2
         f[12] = sy / (0.5 * ey);
@@ -2265,6 +2304,12 @@ __device__ void sur_context(const UnitStatic& U, const double* __restrict__ s, c
     }
     for (int k = 0; k < 3; ++k) C.base[k] = -oc.rho * U.volume * frac * gb[k];
     cross64(U.P_CB, C.base, C.base + 3);
+}
+
+__device__ __forceinline__ void sur_context(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc,
+                                            SurCtx& C)
+{
+    sur_context_g<1>(U, s, oc, C, 0);
 }
 
 // Network input row (BF16 bits, 32 per row) of a body's features.
@@ -2519,11 +2564,13 @@ struct SurRowCtx {
     int32_t pos, pad_;
 };
 
-// One workgroup per tile of TR units of one group (one network): lanes
-// 0..TR-1 build the features of their unit (the FP64 context of
-// k_sur_features) into the network's input rows in LDS, the four wavefronts
-// run the fused network (sfm_device.h), and lanes 0..TR-1 decode their
-// unit's outputs to world-frame F, T.
+// One workgroup per tile of TR units of one group (one network), tiles in
+// XCD-major order: groups of 256 / TR lanes build the features of one unit
+// each (the FP64 context of k_sur_features, wave queries spread over the
+// group) into the network's input rows in LDS while the first layer's
+// weights are in flight, the four wavefronts run the fused network
+// (sfm_device.h), and lanes 0..TR-1 decode their unit's outputs to
+// world-frame F, T.
 template <int TR>
 __global__ __launch_bounds__(256) void k_sur_step(const SurPlanDev* __restrict__ P, const SurTile* __restrict__ tiles,
                                                   const UnitStatic* __restrict__ units,
@@ -2532,39 +2579,51 @@ __global__ __launch_bounds__(256) void k_sur_step(const SurPlanDev* __restrict__
                                                   double* __restrict__ ft)
 {
     __shared__ __attribute__((aligned(16))) uint16_t act[2][TR * sfm::kActStride];
+    __shared__ __attribute__((aligned(16))) float bias[SFM_LAYERS - 1][SFM_HIDDEN];
     __shared__ SurRowCtx ctx[TR];
     __shared__ float yo[TR][SFM_OUT];
-    const SurTile tl = tiles[blockIdx.x];
+    const SurTile tl = tiles[sfm::xcd_tile(blockIdx.x, gridDim.x)];
     const int g = (int)tl.group;
     const int t = threadIdx.x;
-    if (t < TR) {
-        uint4* xr = reinterpret_cast<uint4*>(act[1] + t * sfm::kActStride);
-        if ((uint32_t)t < tl.nvalid) {
-            const uint32_t u = tl.unit0 + (uint32_t)t;
-            SurCtx C;
-            sur_context(units[u], states + (size_t)u * SFH_STATE_DOUBLES, oc, C);
-            const SurNorm& nm = P->norm[g];
-            uint32_t w[SFH_SUR_ROW / 2];
-            sur_input_row(C.f, nm, w);
-            for (int k = 0; k < SFH_SUR_ROW / 8; ++k) xr[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
-            if (ood) ood[u] = sur_ood(C.f, C.pos, nm);
-            if (posout) posout[u] = (int8_t)C.pos;
-            SurRowCtx& X = ctx[t];
-            for (int k = 0; k < 9; ++k) X.R[k] = C.R[k];
-            for (int k = 0; k < 6; ++k) X.base[k] = C.base[k];
-            X.sF = C.sF;
-            X.sT = C.sT;
-            X.pos = C.pos;
-        } else {
-            for (int k = 0; k < SFH_SUR_ROW / 8; ++k) xr[k] = make_uint4(0, 0, 0, 0);
+    sfm::LayerW<SFM_IN_PAD> w0;   // requested now, used after the features
+    sfm::load_layer<SFM_IN_PAD>(P->W[g][0], w0);
+    sfm::load_bias(P->B[g], bias);
+    {
+        // G lanes per unit build its context together
+        constexpr int G = 256 / TR;
+        const int r = t / G, sub = t % G;
+        const bool valid = (uint32_t)r < tl.nvalid;
+        const uint32_t u = tl.unit0 + (valid ? (uint32_t)r : 0u);
+        SurCtx C;
+        sur_context_g<G>(units[u], states + (size_t)u * SFH_STATE_DOUBLES, oc, C, sub);
+        if (sub == 0) {
+            uint4* xr = reinterpret_cast<uint4*>(act[1] + r * sfm::kActStride);
+            if (valid) {
+                const SurNorm& nm = P->norm[g];
+                uint32_t w[SFH_SUR_ROW / 2];
+                sur_input_row(C.f, nm, w);
+                for (int k = 0; k < SFH_SUR_ROW / 8; ++k)
+                    xr[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+                if (ood) ood[u] = sur_ood(C.f, C.pos, nm);
+                if (posout) posout[u] = (int8_t)C.pos;
+                SurRowCtx& X = ctx[r];
+                for (int k = 0; k < 9; ++k) X.R[k] = C.R[k];
+                for (int k = 0; k < 6; ++k) X.base[k] = C.base[k];
+                X.sF = C.sF;
+                X.sT = C.sT;
+                X.pos = C.pos;
+            } else {
+                for (int k = 0; k < SFH_SUR_ROW / 8; ++k) xr[k] = make_uint4(0, 0, 0, 0);
+            }
         }
     }
     __syncthreads();
-    sfm::mlp_hidden<TR>(P->W[g], P->B[g], act, nullptr);
+    sfm::OutW ow;
+    sfm::mlp_hidden<TR>(P->W[g], bias, act, nullptr, w0, ow);
     const int lane = t & 63, wave = t >> 6;
     if (wave < TR / 32) {
         float o[4];
-        sfm::mlp_output(P->W[g][SFM_LAYERS - 1], P->B[g][SFM_LAYERS - 1], act[1], o);
+        sfm::mlp_output(ow, P->B[g][SFM_LAYERS - 1], act[1], o);
         const int lq = (lane >> 5) * 4, row = 32 * wave + (lane & 31);
         for (int i = 0; i < 4; ++i)
             if (lq + i < SFM_OUT) yo[row][lq + i] = o[i];

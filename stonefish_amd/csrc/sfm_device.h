@@ -36,6 +36,12 @@ static constexpr int kActStride = SFM_HIDDEN + 8;   // LDS row stride (elements)
 
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
+// Loads through a global-address-space pointer (global_load, not flat_load:
+// a flat load also counts as an LDS access, and waiting for one waits for
+// every memory access issued after it).
+typedef const __attribute__((address_space(1))) bf16x8 gbf16x8;
+__device__ __forceinline__ bf16x8 ld8g(const uint16_t* p) { return *(gbf16x8*)(p); }
+
 // FP32 -> BF16 bits, round to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -55,42 +61,63 @@ __device__ __forceinline__ uint32_t f2bf2(float a, float b)
 // Row of the 32x32 accumulator tile held by register r of lane `lane`.
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
-// One hidden layer of a TR-row tile: out = gelu(in W^T + b) in BF16, in rows
-// at `in` with row stride in_stride (LDS, or global memory for the network
-// input), K inputs.  The weights of the next k-step are loaded while the
-// current one is multiplied.
+// Weight fragments of one wavefront for a hidden layer with K inputs: rows
+// 64 w + l % 32 and 64 w + 32 + l % 32 of W, columns 8 (l / 32) + 16 s.
+template <int K>
+struct LayerW {
+    bf16x8 a[2][K / 16];
+};
+
+// Requests a layer's weight fragments (2 K / 16 16-byte loads per lane, all
+// issued at once: one L2 round trip per layer instead of one per k-step).
+// The weights do not depend on the activations, so a caller issues them one
+// layer ahead, before the previous layer's epilogue.
+template <int K>
+__device__ __forceinline__ void load_layer(const uint16_t* __restrict__ w, LayerW<K>& L)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint16_t* wp0 = w + (size_t)(64 * wave + (lane & 31)) * K + (lane >> 5) * 8;
+    const uint16_t* wp1 = wp0 + (size_t)32 * K;
+#pragma unroll
+    for (int s = 0; s < K / 16; ++s) {
+        L.a[0][s] = ld8g(wp0 + 16 * s);
+        L.a[1][s] = ld8g(wp1 + 16 * s);
+    }
+}
+
+// acc = W^T-tile products of a TR-row tile: in rows at `in` with row stride
+// in_stride (LDS, or global memory for the network input), K inputs.
 template <int TR, int K>
-__device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ w, const float* __restrict__ bias,
-                                          const uint16_t* in, int in_stride, uint16_t* out)
+__device__ __forceinline__ void layer_mma(const LayerW<K>& L, const uint16_t* in, int in_stride,
+                                          f32x16 (&acc)[2][TR / 32])
 {
     constexpr int RT = TR / 32;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
-    f32x16 acc[2][RT];
+    const int lane = threadIdx.x & 63;
+    const uint16_t* bp = in + (lane & 31) * in_stride + (lane >> 5) * 8;
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int nt = 0; nt < RT; ++nt) acc[mt][nt] = f32x16{};
-    const uint16_t* wp0 = w + (size_t)(64 * wave + lr) * K + lk;
-    const uint16_t* wp1 = wp0 + (size_t)32 * K;
-    const uint16_t* bp = in + lr * in_stride + lk;
-    bf16x8 a0 = ld8(wp0), a1 = ld8(wp1);
 #pragma unroll
-    for (int ks = 0; ks < K; ks += 16) {
-        bf16x8 n0 = a0, n1 = a1;
-        if (ks + 16 < K) {
-            n0 = ld8(wp0 + ks + 16);
-            n1 = ld8(wp1 + ks + 16);
-        }
+    for (int s = 0; s < K / 16; ++s) {
 #pragma unroll
         for (int nt = 0; nt < RT; ++nt) {
-            const bf16x8 b = ld8(bp + 32 * nt * in_stride + ks);
-            acc[0][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][nt], 0, 0, 0);
-            acc[1][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][nt], 0, 0, 0);
+            const bf16x8 b = ld8(bp + 32 * nt * in_stride + 16 * s);
+            acc[0][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L.a[0][s], b, acc[0][nt], 0, 0, 0);
+            acc[1][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L.a[1][s], b, acc[1][nt], 0, 0, 0);
         }
-        a0 = n0;
-        a1 = n1;
     }
+}
+
+// out = gelu(acc + b) in BF16 (LDS rows, stride kActStride), then a barrier.
+// The bias is read from LDS: a global load here would have to wait for the
+// next layer's weight loads issued before it (loads complete in order).
+template <int TR>
+__device__ __forceinline__ void layer_store(const f32x16 (&acc)[2][TR / 32], const float* bias, uint16_t* out)
+{
+    constexpr int RT = TR / 32;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 31, lq = (lane >> 5) * 4;
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
@@ -110,38 +137,84 @@ __device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ w, const 
     __syncthreads();
 }
 
-// The four hidden layers of one TR-row tile.  Layer 0 reads its input rows
-// from global memory (xg, row-major [row][32], rows xg .. xg + TR) when xg is
-// not null, else from act[1] (row stride kActStride, columns 0..31).  On
-// return act[1] holds the last hidden layer.  Each layer ends with a barrier.
-template <int TR>
-__device__ __forceinline__ void mlp_hidden(const uint16_t* const* W, const float* const* B,
-                                           uint16_t (*act)[TR * kActStride], const uint16_t* __restrict__ xg)
+// Output-layer weight fragments of one lane: W[l % 32][8 (l / 32) + 16 s].
+struct OutW {
+    bf16x8 a[SFM_HIDDEN / 16];
+};
+
+__device__ __forceinline__ void load_output(const uint16_t* __restrict__ w, OutW& L)
 {
+    const int lane = threadIdx.x & 63;
+    const uint16_t* wp = w + (size_t)(lane & 31) * SFM_HIDDEN + (lane >> 5) * 8;
+#pragma unroll
+    for (int s = 0; s < SFM_HIDDEN / 16; ++s) L.a[s] = ld8g(wp + 16 * s);
+}
+
+// Hidden-layer biases of one network into LDS (bias[l][f], l < 4); the
+// caller puts a barrier between this and mlp_hidden.
+__device__ __forceinline__ void load_bias(const float* const* B, float (*bias)[SFM_HIDDEN])
+{
+    const int t = threadIdx.x;
+    for (int i = t; i < (SFM_LAYERS - 1) * SFM_HIDDEN; i += blockDim.x)
+        bias[i / SFM_HIDDEN][i % SFM_HIDDEN] = B[i / SFM_HIDDEN][i % SFM_HIDDEN];
+}
+
+// The four hidden layers of one TR-row tile, given layer 0's weight fragments
+// (requested by the caller, possibly long before) and the hidden biases in
+// LDS (load_bias).  Layer 0 reads its input
+// rows from global memory (xg, row-major [row][32], rows xg .. xg + TR) when
+// xg is not null, else from act[1] (row stride kActStride, columns 0..31).
+// On return act[1] holds the last hidden layer and wavefronts w < TR / 32
+// hold the output layer's fragments in ow.
+template <int TR>
+__device__ __forceinline__ void mlp_hidden(const uint16_t* const* W, const float (*B)[SFM_HIDDEN],
+                                           uint16_t (*act)[TR * kActStride], const uint16_t* __restrict__ xg,
+                                           const LayerW<SFM_IN_PAD>& w0, OutW& ow)
+{
+    f32x16 acc[2][TR / 32];
+    LayerW<SFM_HIDDEN> wn;
     if (xg)
-        mlp_layer<TR, SFM_IN_PAD>(W[0], B[0], xg, SFM_IN_PAD, act[0]);
+        layer_mma<TR, SFM_IN_PAD>(w0, xg, SFM_IN_PAD, acc);
     else
-        mlp_layer<TR, SFM_IN_PAD>(W[0], B[0], act[1], kActStride, act[0]);
-    mlp_layer<TR, SFM_HIDDEN>(W[1], B[1], act[0], kActStride, act[1]);
-    mlp_layer<TR, SFM_HIDDEN>(W[2], B[2], act[1], kActStride, act[0]);
-    mlp_layer<TR, SFM_HIDDEN>(W[3], B[3], act[0], kActStride, act[1]);
+        layer_mma<TR, SFM_IN_PAD>(w0, act[1], kActStride, acc);
+    load_layer<SFM_HIDDEN>(W[1], wn);
+    layer_store<TR>(acc, B[0], act[0]);
+    layer_mma<TR, SFM_HIDDEN>(wn, act[0], kActStride, acc);
+    load_layer<SFM_HIDDEN>(W[2], wn);
+    layer_store<TR>(acc, B[1], act[1]);
+    layer_mma<TR, SFM_HIDDEN>(wn, act[1], kActStride, acc);
+    load_layer<SFM_HIDDEN>(W[3], wn);
+    layer_store<TR>(acc, B[2], act[0]);
+    layer_mma<TR, SFM_HIDDEN>(wn, act[0], kActStride, acc);
+    if ((threadIdx.x >> 6) < TR / 32) load_output(W[SFM_LAYERS - 1], ow);
+    layer_store<TR>(acc, B[3], act[1]);
 }
 
 // Output layer of sample tile `wave` (< TR / 32) from act[1]: on return,
 // registers 0..3 of lane l hold outputs 4 (l / 32) + 0..3 (bias included) of
 // sample 32 wave + l % 32.
-__device__ __forceinline__ void mlp_output(const uint16_t* __restrict__ W, const float* __restrict__ bias,
+__device__ __forceinline__ void mlp_output(const OutW& L, const float* __restrict__ bias,
                                            const uint16_t* __restrict__ in, float o[4])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 31, lk = (lane >> 5) * 8, lq = (lane >> 5) * 4;
     f32x16 acc = {};
-    const uint16_t* wp = W + (size_t)lr * SFM_HIDDEN + lk;
-    for (int ks = 0; ks < SFM_HIDDEN; ks += 16)
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(wp + ks), ld8(in + (32 * wave + lr) * kActStride + ks + lk),
+#pragma unroll
+    for (int s = 0; s < SFM_HIDDEN / 16; ++s)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L.a[s], ld8(in + (32 * wave + lr) * kActStride + 16 * s + lk),
                                                       acc, 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) o[i] = acc[i] + (lq + i < SFM_OUT_PAD ? bias[lq + i] : 0.f);
+}
+
+// Workgroup b of a grid of n in XCD-major order: the hardware deals
+// workgroups to the eight XCDs round-robin (b % 8), so the workgroups that
+// share an XCD (and its L2) get consecutive tiles, which mostly share one
+// network's weights.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
+{
+    const uint32_t q = n / 8, r = n % 8, x = b % 8, s = b / 8;
+    return x * q + (x < r ? x : r) + s;
 }
 
 }  // namespace sfm
