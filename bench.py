@@ -157,8 +157,12 @@ def surrogate_section(args) -> dict:
     stream = torch.cuda.current_stream().cuda_stream
     t_exact = _time_replays(lambda: batch.replay(stream), 30)
     step = S.SurrogateStep(batch, ocean, nets, norms, [per] * 8, states)
+    # one kernel per step: a plain launch costs less than a one-node graph
+    # replay, so both are timed and the faster is reported
+    t_eager = _time_replays(step.run, 200)
     step.capture()
-    t_sur = _time_replays(step.replay, 200)
+    t_graph = _time_replays(step.replay, 200)
+    t_sur = min(t_eager, t_graph)
     # the surrogate's error on this sample against the exact outputs
     err = torch.zeros((nb, 2), dtype=torch.float32, device="cuda")
     step.run()
@@ -188,6 +192,7 @@ def surrogate_section(args) -> dict:
         "throughput_bodies": nb,
         "exact_body_steps_per_s": round(nb / t_exact, 1),
         "surrogate_body_steps_per_s": round(nb / t_sur, 1),
+        "surrogate_us_per_step": {"launch": round(t_eager * 1e6, 2), "graph": round(t_graph * 1e6, 2)},
         "speedup": round(t_exact / t_sur, 2),
         "throughput_sample_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in sample_err.items()},
         "mfma": {"inference_tflops": round(inf_tflops, 1), "inference_rows": int(x.shape[0]),

@@ -318,7 +318,8 @@ int sfh_surrogate_set_table(sfh_batch batch, uint32_t first, uint32_t count, con
  * sfm_weights (sfmlp.h).  The plan must not outlive the batch or the networks. */
 typedef struct sfh_sur_plan_t* sfh_sur_plan;
 typedef struct {
-    const uint16_t* weights[5];   /* BF16 [out x in] padded to {32,256,256,256,256,32} */
+    const uint16_t* weights[5];   /* BF16 [out x in], padded widths {32,256,256,256,256,32},
+                                     in the fragment order of sfm_weights */
     const float* bias[5];
 } sfh_net_ptrs;
 

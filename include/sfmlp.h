@@ -87,9 +87,12 @@ int sfm_train_step(sfm_net net, const uint16_t* x, const float* target, uint32_t
  * [out x in] padded and biases [out]; for checking against a reference. */
 int sfm_gradients(sfm_net net, int layer, float* dweights, float* dbias, void* stream);
 
-/* Device pointers of the network's BF16 weights (w[l]: [out x in] padded,
- * row-major) and FP32 biases (b[l]), l = 0..4, for a fused caller (the
- * surrogate step of sfhydro.h).  They stay valid while the network lives;
+/* Device pointers of the network's BF16 weights and FP32 biases (b[l]),
+ * l = 0..4, for a fused caller (the surrogate step of sfhydro.h).  The
+ * weights w[l] ([out x in] padded) are in fragment order: element (r, c) sits
+ * at ((((2 (r / 64) + (r / 32) % 2) (in / 16) + c / 16) 64 + r % 32
+ * + 32 ((c / 8) % 2)) 8 + c % 8, the order in which the fused kernels'
+ * lanes consume them.  They stay valid while the network lives;
  * sfm_train_step updates them in place. */
 int sfm_weights(sfm_net net, const uint16_t** w, const float** b);
 

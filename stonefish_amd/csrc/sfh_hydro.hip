@@ -189,6 +189,12 @@ __device__ __forceinline__ int wrap_texel(double fu, const OceanK& oc)
     return (int)k;
 }
 
+// Bilinear mix of a texel's four corner heights at fractions (a, b).
+__device__ __forceinline__ double texel_mix(double a, double b, float4 t)
+{
+    return (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
+}
+
 // FP64 wave height sum at world (x, y), for the classification corners.
 
 __device__ double wave_sum64(const OceanK& oc, double x, double y)
@@ -200,8 +206,7 @@ __device__ double wave_sum64(const OceanK& oc, double x, double y)
         double fi = floor(u), fj = floor(v);
         double a = u - fi, b = v - fj;
         const int ii = wrap_texel(fi, oc), jj = wrap_texel(fj, oc);
-        float4 t = oc.quad[l][(size_t)jj * oc.n + (size_t)ii];
-        h += (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
+        h += texel_mix(a, b, oc.quad[l][(size_t)jj * oc.n + (size_t)ii]);
     }
     return h;
 }
@@ -231,6 +236,35 @@ __device__ __forceinline__ void cross64(const double* a, const double* b, double
 
 __device__ __forceinline__ f3 to_f3(const double* x) { return mk3((float)x[0], (float)x[1], (float)x[2]); }
 
+// Corner k of a world AABB (min mn, max mx, extents d = mx - mn) in the
+// order of CheckBodyFluidPosition (SolidEntity.cpp:1246-1253): min, max,
+// min + (dx, 0, 0), (0, dy, 0), (dx, dy, 0), (0, 0, dz), (dx, 0, dz), (0, dy, dz).
+__device__ __forceinline__ void aabb_corner(int k, const double* mn, const double* mx, const double* d, double* q)
+{
+    if (k == 1) {
+        q[0] = mx[0]; q[1] = mx[1]; q[2] = mx[2];
+        return;
+    }
+    const bool ax = k == 2 || k == 4 || k == 6, ay = k == 3 || k == 4 || k == 7, az = k >= 5;
+    q[0] = mn[0] + (ax ? d[0] : 0.0);
+    q[1] = mn[1] + (ay ? d[1] : 0.0);
+    q[2] = mn[2] + (az ? d[2] : 0.0);
+}
+
+// btCompoundShape::getAabb(T_CG) of a CG-frame AABB (centre ac, half extents
+// ah): min, max and extents.
+__device__ __forceinline__ void world_aabb(const double* Rcg, const double* tcg, const double* ac, const double* ah,
+                                           double* mn, double* mx, double* d)
+{
+    for (int i = 0; i < 3; ++i) {
+        const double c = (Rcg[3 * i] * ac[0] + Rcg[3 * i + 1] * ac[1] + Rcg[3 * i + 2] * ac[2]) + tcg[i];
+        const double e = fabs(Rcg[3 * i]) * ah[0] + fabs(Rcg[3 * i + 1]) * ah[1] + fabs(Rcg[3 * i + 2]) * ah[2];
+        mn[i] = c - e;
+        mx[i] = c + e;
+        d[i] = mx[i] - mn[i];
+    }
+}
+
 // Sum over aligned groups of G lanes.
 template <int G, typename T>
 __device__ __forceinline__ T gsum(T x)
@@ -249,27 +283,12 @@ template <int G>
 __device__ int classify64_g(const double* Rcg, const double* tcg, const double* ac, const double* ah, const OceanK& oc,
                             int sub)
 {
-    double c[3], e[3], mn[3], mx[3], d[3];
-    for (int i = 0; i < 3; ++i) {
-        c[i] = (Rcg[3 * i] * ac[0] + Rcg[3 * i + 1] * ac[1] + Rcg[3 * i + 2] * ac[2]) + tcg[i];
-        e[i] = fabs(Rcg[3 * i]) * ah[0] + fabs(Rcg[3 * i + 1]) * ah[1] + fabs(Rcg[3 * i + 2]) * ah[2];
-        mn[i] = c[i] - e[i];
-        mx[i] = c[i] + e[i];
-        d[i] = mx[i] - mn[i];
-    }
+    double mn[3], mx[3], d[3];
+    world_aabb(Rcg, tcg, ac, ah, mn, mx, d);
     int under = 0;
     for (int k = sub; k < 8; k += G) {
         double q[3];
-        if (k == 1) { q[0] = mx[0]; q[1] = mx[1]; q[2] = mx[2]; }
-        else {
-            // corners min, +x, +y, +xy, +z, +xz, +yz in the order of the reference
-            const int sx[8] = {0, 0, 1, 0, 1, 0, 1, 0};
-            const int sy[8] = {0, 0, 0, 1, 1, 0, 0, 1};
-            const int sz[8] = {0, 0, 0, 0, 0, 1, 1, 1};
-            q[0] = mn[0] + (sx[k] ? d[0] : 0.0);
-            q[1] = mn[1] + (sy[k] ? d[1] : 0.0);
-            q[2] = mn[2] + (sz[k] ? d[2] : 0.0);
-        }
+        aabb_corner(k, mn, mx, d, q);
         double depth = q[2];
         if (oc.nlayers) depth += wave_sum64(oc, q[0], q[1]);
         if (depth > 0.0) ++under;
@@ -2150,26 +2169,28 @@ __device__ __forceinline__ void exact_totals(const double* e, double* F, double*
 }
 
 // Unit quaternion (x, y, z, w) of a rotation matrix (row-major), w >= 0.
-__device__ void rot_to_quat(const double* R, double* q)
+// Unit quaternion (x, y, z, w), w >= 0, of a rotation matrix (row-major):
+// the largest of the four diagonal combinations picks the component computed
+// from a square root, the other three follow from the off-diagonal sums and
+// differences.  FP32 and branch-free (selects), for the network's features.
+__device__ void rot_to_quat(const double* Rd, double* q)
 {
-    const double tr = R[0] + R[4] + R[8];
-    double x, y, z, w;
-    if (tr > 0.0) {
-        double s = 2.0 * sqrt(tr + 1.0);
-        w = 0.25 * s; x = (R[7] - R[5]) / s; y = (R[2] - R[6]) / s; z = (R[3] - R[1]) / s;
-    } else if (R[0] > R[4] && R[0] > R[8]) {
-        double s = 2.0 * sqrt(1.0 + R[0] - R[4] - R[8]);
-        w = (R[7] - R[5]) / s; x = 0.25 * s; y = (R[1] + R[3]) / s; z = (R[2] + R[6]) / s;
-    } else if (R[4] > R[8]) {
-        double s = 2.0 * sqrt(1.0 + R[4] - R[0] - R[8]);
-        w = (R[2] - R[6]) / s; x = (R[1] + R[3]) / s; y = 0.25 * s; z = (R[5] + R[7]) / s;
-    } else {
-        double s = 2.0 * sqrt(1.0 + R[8] - R[0] - R[4]);
-        w = (R[3] - R[1]) / s; x = (R[2] + R[6]) / s; y = (R[5] + R[7]) / s; z = 0.25 * s;
-    }
-    const double sg = (w < 0.0) ? -1.0 : 1.0;
-    const double inv = sg / sqrt(x * x + y * y + z * z + w * w);
-    q[0] = x * inv; q[1] = y * inv; q[2] = z * inv; q[3] = w * inv;
+    float R[9];
+    for (int k = 0; k < 9; ++k) R[k] = (float)Rd[k];
+    const float tr = R[0] + R[4] + R[8];
+    const int c = tr > 0.f ? 0 : ((R[0] > R[4] && R[0] > R[8]) ? 1 : (R[4] > R[8] ? 2 : 3));
+    const float dg = c == 0 ? tr : (c == 1 ? R[0] - R[4] - R[8] : (c == 2 ? R[4] - R[0] - R[8] : R[8] - R[0] - R[4]));
+    const float sq = 2.f * sqrtf(1.f + dg);
+    const float inv = 1.f / sq;
+    const float big = 0.25f * sq;
+    const float A = (R[7] - R[5]) * inv, B = (R[2] - R[6]) * inv, Cc = (R[3] - R[1]) * inv;
+    const float D = (R[1] + R[3]) * inv, E = (R[2] + R[6]) * inv, F = (R[5] + R[7]) * inv;
+    const float w = c == 0 ? big : (c == 1 ? A : (c == 2 ? B : Cc));
+    const float x = c == 0 ? A : (c == 1 ? big : (c == 2 ? D : E));
+    const float y = c == 0 ? B : (c == 1 ? D : (c == 2 ? big : F));
+    const float z = c == 0 ? Cc : (c == 1 ? E : (c == 2 ? F : big));
+    const float n = (w < 0.f ? -1.f : 1.f) / sqrtf(x * x + y * y + z * z + w * w);
+    q[0] = x * n; q[1] = y * n; q[2] = z * n; q[3] = w * n;
 }
 
 // Everything the surrogate derives from one body's state: the rotation, the
@@ -2196,32 +2217,38 @@ struct SurCtx {
 // AABB: -1 top at the surface, 1 bottom at the surface), gh the unit gravity
 // direction in the CG frame, as polar angle acos(gh.z) on hth nodes over
 // [0, pi] and azimuth atan2(gh.y, gh.x) on hph periodic nodes over [-pi, pi).
-// Entries: 6 floats, F / (rho g V) and T / (rho g V Lc), CG frame.
-__device__ void hst_lookup(const UnitStatic& U, double sv, const double* gh, double* out)
+// Entries: 6 floats, F / (rho g V) and T / (rho g V Lc), CG frame.  FP32: the
+// result is a baseline the network corrects.
+__device__ void hst_lookup(const UnitStatic& U, float sv, const float* gh, float* out)
 {
-    // angles in FP32: the table's node spacing is several degrees
-    const double th = (double)acosf((float)fmin(1.0, fmax(-1.0, gh[2])));
-    const double ph = (double)atan2f((float)gh[1], (float)gh[0]);
-    double fs = (fmin(1.0, fmax(-1.0, sv)) + 1.0) * 0.5 * (double)(U.hs - 1);
-    double ft = th * (1.0 / 3.141592653589793) * (double)(U.hth - 1);
-    double fp = (ph + 3.141592653589793) * (0.5 / 3.141592653589793) * (double)U.hph;
-    int is = min((int)fs, (int)U.hs - 2), it = min((int)ft, (int)U.hth - 2), ip = (int)floor(fp);
-    const double as = fs - is, at = ft - it, ap = fp - ip;
+    const float th = acosf(fminf(1.f, fmaxf(-1.f, gh[2])));
+    const float ph = atan2f(gh[1], gh[0]);
+    const float fs = (fminf(1.f, fmaxf(-1.f, sv)) + 1.f) * 0.5f * (float)(U.hs - 1);
+    const float ft = th * (float)(1.0 / 3.141592653589793) * (float)(U.hth - 1);
+    float fp = (ph + 3.14159265f) * (float)(0.5 / 3.141592653589793) * (float)U.hph;
+    if (!(fp >= 0.f && fp <= (float)U.hph)) fp = 0.f;   // non-finite direction
+    const int is = min((int)fs, (int)U.hs - 2), it = min((int)ft, (int)U.hth - 2);
+    int ip = (int)floorf(fp);
+    const float as = fs - is, at = ft - it, ap = fp - ip;
     ip %= (int)U.hph;
     if (ip < 0) ip += (int)U.hph;
     const int ip1 = (ip + 1 == (int)U.hph) ? 0 : ip + 1;
-    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    for (int k = 0; k < 6; ++k) out[k] = 0.f;
     for (int c = 0; c < 8; ++c) {
         const int ds = c & 1, dt = (c >> 1) & 1, dp = (c >> 2) & 1;
-        const double wgt = (ds ? as : 1.0 - as) * (dt ? at : 1.0 - at) * (dp ? ap : 1.0 - ap);
-        const size_t idx = (((size_t)(is + ds) * U.hth + (size_t)(it + dt)) * U.hph + (size_t)(dp ? ip1 : ip)) * 6;
-        for (int k = 0; k < 6; ++k) out[k] += wgt * (double)U.hst[idx + k];
+        const float wgt = (ds ? as : 1.f - as) * (dt ? at : 1.f - at) * (dp ? ap : 1.f - ap);
+        const float* e = U.hst + (((size_t)(is + ds) * U.hth + (size_t)(it + dt)) * U.hph + (size_t)(dp ? ip1 : ip)) * 6;
+        for (int k = 0; k < 6; ++k) out[k] += wgt * e[k];
     }
 }
 
 // The context of one unit computed by a group of G lanes (G = 1: one lane);
-// every lane of the group ends with the full context.  The wave queries of
-// the classification and of the footprint plane are spread over the group.
+// every lane of the group ends with the full context.  The 17 wave queries
+// (the classification's 8 AABB corners and the footprint plane's 3 x 3
+// stencil) are spread over the group and issued together, one memory round
+// trip.  Positions, wave sums and the classification are FP64 (the
+// classification is the exact path's, bit for bit); the rest is FP32, far
+// finer than the BF16 network input.
 template <int G>
 __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc, SurCtx& C, int sub)
 {
@@ -2233,77 +2260,123 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
     mat_t_vec(C.R, vr, f);
     mat_t_vec(C.R, s + 19, f + 3);
     rot_to_quat(C.R, f + 6);
-    {
-        const int p = classify64_g<G>(C.R, C.t, U.aabb_c, U.aabb_h, oc, sub);
-        C.pos = U.hydro ? p : -1;
-    }
-    double cb[3];
+    double mn[3], mx[3], dd[3], cb[3];
+    world_aabb(C.R, C.t, U.aabb_c, U.aabb_h, mn, mx, dd);
     mat_vec(C.R, U.P_CB, cb);
     for (int k = 0; k < 3; ++k) cb[k] += C.t[k];
+    // the local surface is the plane fitted to the wave field over the body's
+    // horizontal footprint (half extents ex, ey of its world AABB, at least
+    // one texel): weights (1/4, 1/2, 1/4) on a 3 x 3 stencil give its height
+    // at the CoB and least squares its slope
+    const double tex = oc.nlayers ? oc.len[0] / (double)oc.n : 0.0;
+    const double ex = fmax(0.5 * dd[0], tex), ey = fmax(0.5 * dd[1], tex);
+    constexpr int NQ = (17 + G - 1) / G;
+    double qx[NQ], qy[NQ], qz[NQ], h[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int k = sub + i * G;
+        double q[3];
+        aabb_corner(k < 8 ? k : 0, mn, mx, dd, q);
+        const int j = k < 8 ? 4 : (k < 17 ? k - 8 : 4), a = j / 3, c = j - 3 * (j / 3);
+        qx[i] = k < 8 ? q[0] : cb[0] + (a - 1) * ex;
+        qy[i] = k < 8 ? q[1] : cb[1] + (c - 1) * ey;
+        qz[i] = q[2];
+        h[i] = 0.0;
+    }
+    for (int l = 0; l < oc.nlayers; ++l) {
+        const double sc = oc.scale[l];
+        float4 tq[NQ];
+        double fa[NQ], fb[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const double u = qx[i] * sc - 0.5, v = qy[i] * sc - 0.5;
+            const double fi = floor(u), fj = floor(v);
+            fa[i] = u - fi;
+            fb[i] = v - fj;
+            tq[i] = oc.quad[l][(size_t)wrap_texel(fj, oc) * oc.n + (size_t)wrap_texel(fi, oc)];
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) h[i] += texel_mix(fa[i], fb[i], tq[i]);
+    }
+    int under = 0;
+    double hm = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int k = sub + i * G;
+        if (k < 8) {
+            if (qz[i] + h[i] > 0.0) ++under;   // wave_sum64 order: depth = z, then += the layer sum
+        } else if (k < 17) {
+            const int a = (k - 8) / 3, c = (k - 8) - 3 * ((k - 8) / 3);
+            const double w = (a == 1 ? 0.5 : 0.25) * (c == 1 ? 0.5 : 0.25);
+            hm += w * h[i];
+            sx += w * (a - 1) * h[i];
+            sy += w * (c - 1) * h[i];
+        }
+    }
+    under = gsum<G>(under);
+    C.pos = U.hydro ? (under == 0 ? 0 : (under == 8 ? 1 : 2)) : -1;
     f[10] = cb[2];
     f[11] = f[12] = 0.0;
     if (oc.nlayers) {
-        // the local surface is the plane fitted to the wave field over the
-        // body's horizontal footprint (half extents ex, ey of its world AABB,
-        // at least one texel): weights (1/4, 1/2, 1/4) on a 3 x 3 stencil give
-        // its height at the CoB and least squares its slope
-        const double d = oc.len[0] / (double)oc.n;
-        double ex = 0.0, ey = 0.0;
-        for (int j = 0; j < 3; ++j) {
-            ex += fabs(C.R[j]) * U.aabb_h[j];
-            ey += fabs(C.R[3 + j]) * U.aabb_h[j];
-        }
-        ex = fmax(ex, d);
-        ey = fmax(ey, d);
-        const double w1[3] = {0.25, 0.5, 0.25};
-        double hm = 0.0, sx = 0.0, sy = 0.0;
-        for (int k = sub; k < 9; k += G) {
-            const int a = k / 3, c = k - 3 * (k / 3);
-            const double h = wave_sum64(oc, cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey);
-            hm += w1[a] * w1[c] * h;
-            sx += w1[a] * w1[c] * (a - 1) * h;
-            sy += w1[a] * w1[c] * (c - 1) * h;
-        }
         hm = gsum<G>(hm);
         sx = gsum<G>(sx);
         sy = gsum<G>(sy);
         f[10] += hm;
-        f[11] = sx / (0.5 * ex);   // sum of w (a-1)^2 = 1/2
-        f[12] = sy / (0.5 * ey);
+        f[11] = (double)((float)sx * (2.f / (float)ex));   // sum of w (a-1)^2 = 1/2
+        f[12] = (double)((float)sy * (2.f / (float)ey));
     }
     // scales
-    const double Lc = fmax(fmax(U.aabb_h[0], U.aabb_h[1]), U.aabb_h[2]);
-    const double u = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) + sqrt(f[3] * f[3] + f[4] * f[4] + f[5] * f[5]) * Lc;
-    const double gn = sqrt(oc.g[0] * oc.g[0] + oc.g[1] * oc.g[1] + oc.g[2] * oc.g[2]);
-    C.sF = oc.rho * gn * U.volume + oc.rho * 0.25 * U.surface * (u * u * u + u);
-    if (!(C.sF > 0.0)) C.sF = 1.0;
-    C.sT = C.sF * fmax(Lc, 1e-6);
+    const float Lc = (float)fmax(fmax(U.aabb_h[0], U.aabb_h[1]), U.aabb_h[2]);
+    const float v0 = (float)f[0], v1 = (float)f[1], v2 = (float)f[2], w0 = (float)f[3], w1 = (float)f[4],
+                w2 = (float)f[5];
+    const float u = sqrtf(v0 * v0 + v1 * v1 + v2 * v2) + sqrtf(w0 * w0 + w1 * w1 + w2 * w2) * Lc;
+    const float g0 = (float)oc.g[0], g1 = (float)oc.g[1], g2 = (float)oc.g[2];
+    const float gn = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
+    const float rho = (float)oc.rho, vol = (float)U.volume;
+    float sF = rho * gn * vol + rho * 0.25f * (float)U.surface * (u * u * u + u);
+    if (!(sF > 0.f)) sF = 1.f;
+    C.sF = sF;
+    C.sT = (double)(sF * fmaxf(Lc, 1e-6f));
     // hydrostatic estimate
-    double gb[3];
-    mat_t_vec(C.R, oc.g, gb);
-    if (U.buoyant && C.pos == 2 && U.hst && gn > 0.0) {
+    float gb[3];
+    {
+        double g64[3];
+        mat_t_vec(C.R, oc.g, g64);
+        for (int k = 0; k < 3; ++k) gb[k] = (float)g64[k];
+    }
+    const float ah0 = (float)U.aabb_h[0], ah1 = (float)U.aabb_h[1], ah2 = (float)U.aabb_h[2];
+    if (U.buoyant && C.pos == 2 && U.hst && gn > 0.f) {
         // the hull's table: static buoyancy on a flat surface against the
         // immersion of its AABB and the direction of gravity (CG frame)
-        double gh[3] = {gb[0] / gn, gb[1] / gn, gb[2] / gn};
-        const double hz = fabs(gh[0]) * U.aabb_h[0] + fabs(gh[1]) * U.aabb_h[1] + fabs(gh[2]) * U.aabb_h[2];
-        double sv = 0.0;
-        for (int k = 0; k < 3; ++k) sv += gh[k] * (U.aabb_c[k] - U.P_CB[k]);
-        sv = hz > 0.0 ? (f[10] + sv) / hz : 0.0;
-        double tab[6];
+        const float ig = 1.f / gn;
+        const float gh[3] = {gb[0] * ig, gb[1] * ig, gb[2] * ig};
+        const float hz = fabsf(gh[0]) * ah0 + fabsf(gh[1]) * ah1 + fabsf(gh[2]) * ah2;
+        float sv = 0.f;
+        for (int k = 0; k < 3; ++k) sv += gh[k] * (float)(U.aabb_c[k] - U.P_CB[k]);
+        sv = hz > 0.f ? ((float)f[10] + sv) / hz : 0.f;
+        float tab[6];
         hst_lookup(U, sv, gh, tab);
-        const double sb = oc.rho * gn * U.volume;
-        for (int k = 0; k < 3; ++k) { C.base[k] = sb * tab[k]; C.base[3 + k] = sb * Lc * tab[3 + k]; }
+        const float sb = rho * gn * vol;
+        for (int k = 0; k < 3; ++k) {
+            C.base[k] = (double)(sb * tab[k]);
+            C.base[3 + k] = (double)(sb * Lc * tab[3 + k]);
+        }
         return;
     }
-    double frac = 0.0;
-    if (U.buoyant && C.pos == 1) frac = 1.0;
+    float frac = 0.f;
+    if (U.buoyant && C.pos == 1) frac = 1.f;
     if (U.buoyant && C.pos == 2) {
-        const double hz = gn > 0.0 ? (fabs(gb[0]) * U.aabb_h[0] + fabs(gb[1]) * U.aabb_h[1] + fabs(gb[2]) * U.aabb_h[2]) / gn
-                                   : 0.0;
-        frac = hz > 0.0 ? fmin(1.0, fmax(0.0, (f[10] + hz) / (2.0 * hz))) : 0.0;
+        const float hz = gn > 0.f ? (fabsf(gb[0]) * ah0 + fabsf(gb[1]) * ah1 + fabsf(gb[2]) * ah2) / gn : 0.f;
+        frac = hz > 0.f ? fminf(1.f, fmaxf(0.f, ((float)f[10] + hz) / (2.f * hz))) : 0.f;
     }
-    for (int k = 0; k < 3; ++k) C.base[k] = -oc.rho * U.volume * frac * gb[k];
-    cross64(U.P_CB, C.base, C.base + 3);
+    const float pc0 = (float)U.P_CB[0], pc1 = (float)U.P_CB[1], pc2 = (float)U.P_CB[2];
+    const float b0 = -rho * vol * frac * gb[0], b1 = -rho * vol * frac * gb[1], b2 = -rho * vol * frac * gb[2];
+    C.base[0] = b0;
+    C.base[1] = b1;
+    C.base[2] = b2;
+    C.base[3] = pc1 * b2 - pc2 * b1;
+    C.base[4] = pc2 * b0 - pc0 * b2;
+    C.base[5] = pc0 * b1 - pc1 * b0;
 }
 
 __device__ __forceinline__ void sur_context(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc,

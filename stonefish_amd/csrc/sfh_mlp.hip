@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_rowsum(const uint16_t* __restrict__ xt,
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ mo, float* __restrict__ ve,
                        const float* __restrict__ part, int nsplit, int rows, int cols, float lr, float b1, float b2,
                        float eps, float bc1, float bc2, float* __restrict__ gsave, uint16_t* __restrict__ wb,
-                       uint16_t* __restrict__ wbt)
+                       uint16_t* __restrict__ wbt, uint16_t* __restrict__ wp)
 {
     const int n = rows * cols;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,6 +302,7 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ mo, float* __r
         wb[i] = h;
         const int r = i / cols, c = i - r * cols;
         wbt[(size_t)c * rows + r] = h;
+        wp[sfm::frag_index(r, c, cols)] = h;
     }
 }
 
@@ -337,6 +338,7 @@ struct Layer {
     float *W = nullptr, *b = nullptr, *mW = nullptr, *vW = nullptr, *mb = nullptr, *vb = nullptr;
     float *gW = nullptr, *gb = nullptr;
     uint16_t *Wb = nullptr, *Wbt = nullptr;
+    uint16_t* Wp = nullptr;   // BF16 in fragment order (sfm::frag_index) for the fused inference
 };
 
 struct sfm_net_t {
@@ -370,6 +372,7 @@ static void net_free(sfm_net_t* n)
         for (float* p : {l.W, l.b, l.mW, l.vW, l.mb, l.vb, l.gW, l.gb}) hipFree(p);
         hipFree(l.Wb);
         hipFree(l.Wbt);
+        hipFree(l.Wp);
     }
     for (int i = 0; i < SFM_LAYERS; ++i) { hipFree(n->H[i]); hipFree(n->Ht[i]); hipFree(n->Z[i]); }
     for (int i = 0; i < 2; ++i) { hipFree(n->dZ[i]); hipFree(n->dZt[i]); }
@@ -382,7 +385,7 @@ static void net_free(sfm_net_t* n)
 static int upload_layer(sfm_net_t* n, int l, const float* w, const float* b)
 {
     const int rows = kDims[l + 1], cols = kDims[l];
-    std::vector<uint16_t> wb((size_t)rows * cols), wbt((size_t)rows * cols);
+    std::vector<uint16_t> wb((size_t)rows * cols), wbt((size_t)rows * cols), wp((size_t)rows * cols);
     for (int r = 0; r < rows; ++r)
         for (int c = 0; c < cols; ++c) {
             float f = w[(size_t)r * cols + c];
@@ -392,12 +395,14 @@ static int upload_layer(sfm_net_t* n, int l, const float* w, const float* b)
             uint16_t h = (uint16_t)(u >> 16);
             wb[(size_t)r * cols + c] = h;
             wbt[(size_t)c * rows + r] = h;
+            wp[sfm::frag_index(r, c, cols)] = h;
         }
     Layer& L = n->L[l];
     HIPCHK(hipMemcpy(L.W, w, sizeof(float) * rows * cols, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L.b, b, sizeof(float) * rows, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L.Wb, wb.data(), 2 * wb.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L.Wbt, wbt.data(), 2 * wbt.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L.Wp, wp.data(), 2 * wp.size(), hipMemcpyHostToDevice));
     return SFM_OK;
 }
 
@@ -419,7 +424,7 @@ extern "C" int sfm_create(uint32_t max_batch, uint64_t seed, sfm_net* out)
         ok = alloc((void**)&L.W, 4 * nw) && alloc((void**)&L.b, 4 * nb) && alloc((void**)&L.mW, 4 * nw) &&
              alloc((void**)&L.vW, 4 * nw) && alloc((void**)&L.mb, 4 * nb) && alloc((void**)&L.vb, 4 * nb) &&
              alloc((void**)&L.gW, 4 * nw) && alloc((void**)&L.gb, 4 * nb) && alloc((void**)&L.Wb, 2 * nw) &&
-             alloc((void**)&L.Wbt, 2 * nw);
+             alloc((void**)&L.Wbt, 2 * nw) && alloc((void**)&L.Wp, 2 * nw);
         if (ok && l > 0) ok = alloc((void**)&n->H[l], 2 * (size_t)max_batch * kDims[l]) &&
                                alloc((void**)&n->Z[l], 4 * (size_t)max_batch * kDims[l]);
         if (ok) ok = alloc((void**)&n->Ht[l], 2 * (size_t)max_batch * kDims[l]);
@@ -565,7 +570,7 @@ static int forward_group(const sfm_net* nets, uint32_t nnets, const uint32_t* co
         if (total > (1ull << 31)) return fail(SFM_ERR_SIZE, "too many rows");
         ns.row_end[i] = (uint32_t)total;
         for (int l = 0; l < SFM_LAYERS; ++l) {
-            ns.W[i][l] = nets[i]->L[l].Wb;
+            ns.W[i][l] = nets[i]->L[l].Wp;
             ns.b[i][l] = nets[i]->L[l].b;
         }
     }
@@ -668,9 +673,10 @@ extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target,
             }
             const int nw = dout * din;
             hipLaunchKernelGGL(k_adam, dim3((nw + 255) / 256), dim3(256), 0, s, L.W, L.mW, L.vW, n->part, splits, dout,
-                               din, lr, beta1, beta2, eps, bc1, bc2, L.gW, L.Wb, L.Wbt);
+                               din, lr, beta1, beta2, eps, bc1, bc2, L.gW, L.Wb, L.Wbt, L.Wp);
             hipLaunchKernelGGL(k_adam, dim3((dout + 255) / 256), dim3(256), 0, s, L.b, L.mb, L.vb, n->dbias, 1, dout, 1,
-                               lr, beta1, beta2, eps, bc1, bc2, L.gb, (uint16_t*)nullptr, (uint16_t*)nullptr);
+                               lr, beta1, beta2, eps, bc1, bc2, L.gb, (uint16_t*)nullptr, (uint16_t*)nullptr,
+                               (uint16_t*)nullptr);
         }
         cur ^= 1;
     }
@@ -709,7 +715,7 @@ extern "C" int sfm_weights(sfm_net n, const uint16_t** w, const float** b)
 {
     if (!n || !w || !b) return fail(SFM_ERR_INVALID, "null argument");
     for (int l = 0; l < SFM_LAYERS; ++l) {
-        w[l] = n->L[l].Wb;
+        w[l] = n->L[l].Wp;
         b[l] = n->L[l].b;
     }
     return SFM_OK;

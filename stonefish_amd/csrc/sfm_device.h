@@ -61,6 +61,19 @@ __device__ __forceinline__ uint32_t f2bf2(float a, float b)
 // Row of the 32x32 accumulator tile held by register r of lane `lane`.
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
+// Fragment order of a layer's BF16 weights W [out x K] (row-major indices r,
+// c) for the fused inference: the 8 consecutive columns 16 s + 8 h .. + 7 of
+// row r = 64 w + 32 m + i are the 16 bytes that lane i + 32 h of wavefront w
+// feeds to k-step s of its m-th 32-row tile, and each (w, m, s) owns 1 KB of
+// contiguous memory, read by one fully coalesced load instruction.  (Row-major
+// rows are 512 bytes apart, so the same fragments would touch 32 cache lines
+// per instruction.)
+__host__ __device__ __forceinline__ size_t frag_index(int r, int c, int K)
+{
+    const int w = r >> 6, m = (r >> 5) & 1, i = r & 31, s = c >> 4, h = (c >> 3) & 1, e = c & 7;
+    return ((((size_t)(2 * w + m) * (K / 16) + s) * 64) + i + 32 * h) * 8 + e;
+}
+
 // Weight fragments of one wavefront for a hidden layer with K inputs: rows
 // 64 w + l % 32 and 64 w + 32 + l % 32 of W, columns 8 (l / 32) + 16 s.
 template <int K>
@@ -68,7 +81,7 @@ struct LayerW {
     bf16x8 a[2][K / 16];
 };
 
-// Requests a layer's weight fragments (2 K / 16 16-byte loads per lane, all
+// Requests a layer's weight fragments (fragment order, frag_index) (2 K / 16 16-byte loads per lane, all
 // issued at once: one L2 round trip per layer instead of one per k-step).
 // The weights do not depend on the activations, so a caller issues them one
 // layer ahead, before the previous layer's epilogue.
@@ -76,12 +89,12 @@ template <int K>
 __device__ __forceinline__ void load_layer(const uint16_t* __restrict__ w, LayerW<K>& L)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint16_t* wp0 = w + (size_t)(64 * wave + (lane & 31)) * K + (lane >> 5) * 8;
-    const uint16_t* wp1 = wp0 + (size_t)32 * K;
+    const uint16_t* wp0 = w + ((size_t)(2 * wave) * (K / 16) * 64 + lane) * 8;   // fragment order
+    const uint16_t* wp1 = wp0 + (size_t)(K / 16) * 512;
 #pragma unroll
     for (int s = 0; s < K / 16; ++s) {
-        L.a[0][s] = ld8g(wp0 + 16 * s);
-        L.a[1][s] = ld8g(wp1 + 16 * s);
+        L.a[0][s] = ld8g(wp0 + 512 * s);
+        L.a[1][s] = ld8g(wp1 + 512 * s);
     }
 }
 
@@ -144,10 +157,9 @@ struct OutW {
 
 __device__ __forceinline__ void load_output(const uint16_t* __restrict__ w, OutW& L)
 {
-    const int lane = threadIdx.x & 63;
-    const uint16_t* wp = w + (size_t)(lane & 31) * SFM_HIDDEN + (lane >> 5) * 8;
+    const uint16_t* wp = w + (size_t)(threadIdx.x & 63) * 8;   // fragment order
 #pragma unroll
-    for (int s = 0; s < SFM_HIDDEN / 16; ++s) L.a[s] = ld8g(wp + 16 * s);
+    for (int s = 0; s < SFM_HIDDEN / 16; ++s) L.a[s] = ld8g(wp + 512 * s);
 }
 
 // Hidden-layer biases of one network into LDS (bias[l][f], l < 4); the
