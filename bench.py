@@ -134,7 +134,7 @@ def surrogate_section(args) -> dict:
     import torch
     from stonefish_amd import surrogate as S
     t0 = time.time()
-    cfg = S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=3e-3, warmup=200)
+    cfg = S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=args.sur_lr, warmup=200)
     r = S.run_training_workload(8, args.sur_per_hull, args.sur_steps, args.sur_holdout, cfg, seed=17, log=log)
     hulls, nets, norms, w, ocean = r["hulls"], r["nets"], r["norms"], r["workload"], r["ocean"]
     ds = r["dataset"]
@@ -180,15 +180,16 @@ def surrogate_section(args) -> dict:
     ms, flops, cnt = nets[0].kernel_ms()
     nets[0].timing(False)
     inf_tflops = flops / (ms * 1e-3) / 1e12
-    train_tflops = float(np.mean([h["train_tflops"] for h in hulls]))
+    train_tflops = float(r["train_tflops"])
     res = {
         "workload": f"8 hulls x {args.sur_per_hull} bodies x {args.sur_steps} steps of exact-kernel samples "
                     f"({8 * args.sur_per_hull * args.sur_steps} total, {args.sur_holdout} steps held out), "
                     f"{args.sur_train_steps} Adam steps of {args.sur_batch} per hull",
         "held_out_error": agg,
         "per_hull": [{k: h[k] for k in ("hull", "faces", "train_samples", "F_median", "F_p90", "T_median",
-                                        "T_p90", "n", "n_ood", "n_out_of_water", "final_loss", "train_seconds")}
+                                        "T_p90", "n", "n_ood", "n_out_of_water", "final_loss")}
                      for h in hulls],
+        "train_seconds": round(float(r["train_seconds"]), 1),
         "throughput_bodies": nb,
         "exact_body_steps_per_s": round(nb / t_exact, 1),
         "surrogate_body_steps_per_s": round(nb / t_sur, 1),
@@ -296,6 +297,7 @@ def main() -> int:
     ap.add_argument("--sur-steps", type=int, default=31)
     ap.add_argument("--sur-holdout", type=int, default=3)
     ap.add_argument("--sur-train-steps", type=int, default=20000)
+    ap.add_argument("--sur-lr", type=float, default=3e-3, help="peak Adam learning rate (cosine schedule)")
     ap.add_argument("--sur-batch", type=int, default=16384)
     ap.add_argument("--rollout-bodies", type=int, default=16384)
     ap.add_argument("--rollout-steps", type=int, default=300)

@@ -2198,13 +2198,15 @@ __device__ void rot_to_quat(const double* Rd, double* q)
 // the network's outputs are relative to.  The network predicts
 //   y_F = (F - Fb0) / sF,  y_T = (T - Tb0) / sT   (CG frame)
 // where Fb0, Tb0 is a hydrostatic estimate (the analytic buoyancy inside the
-// water; the same scaled by the immersed fraction of the body's vertical
-// extent, from the CoB depth, when crossing) and
+// water; when crossing, the hull's hydrostatic table, or without one the
+// analytic buoyancy scaled by the immersed fraction of the body's vertical
+// extent) and
 //   sF = rho g V + rho (A/4) (u^3 + u),  sT = sF Lc,
 // u = |v_rel| + |omega| Lc, Lc the largest half extent of the body's AABB, A
-// its surface: the sizes of the buoyancy, quadratic-drag and skin-friction
-// terms.  Outputs stay O(1) whatever the speed and immersion, so the error of
-// the network is a relative error per body.
+// its surface.  Outputs stay O(1) whatever the speed and immersion.  (Scales
+// that follow the drag law more closely, rho (A/4) (u^2 + u), or that shrink
+// with the submerged volume of a crossing body, gave larger held-out errors:
+// the latter lets the crossing bodies dominate the loss.)
 struct SurCtx {
     double R[9], t[3];
     double f[SFH_SUR_FEATURES];
@@ -2325,19 +2327,11 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
         f[11] = (double)((float)sx * (2.f / (float)ex));   // sum of w (a-1)^2 = 1/2
         f[12] = (double)((float)sy * (2.f / (float)ey));
     }
-    // scales
+    // hydrostatic estimate
     const float Lc = (float)fmax(fmax(U.aabb_h[0], U.aabb_h[1]), U.aabb_h[2]);
-    const float v0 = (float)f[0], v1 = (float)f[1], v2 = (float)f[2], w0 = (float)f[3], w1 = (float)f[4],
-                w2 = (float)f[5];
-    const float u = sqrtf(v0 * v0 + v1 * v1 + v2 * v2) + sqrtf(w0 * w0 + w1 * w1 + w2 * w2) * Lc;
     const float g0 = (float)oc.g[0], g1 = (float)oc.g[1], g2 = (float)oc.g[2];
     const float gn = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
     const float rho = (float)oc.rho, vol = (float)U.volume;
-    float sF = rho * gn * vol + rho * 0.25f * (float)U.surface * (u * u * u + u);
-    if (!(sF > 0.f)) sF = 1.f;
-    C.sF = sF;
-    C.sT = (double)(sF * fmaxf(Lc, 1e-6f));
-    // hydrostatic estimate
     float gb[3];
     {
         double g64[3];
@@ -2345,6 +2339,12 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
         for (int k = 0; k < 3; ++k) gb[k] = (float)g64[k];
     }
     const float ah0 = (float)U.aabb_h[0], ah1 = (float)U.aabb_h[1], ah2 = (float)U.aabb_h[2];
+    // immersed fraction of the AABB's vertical extent (1 inside, 0 outside)
+    float frac = C.pos == 1 ? 1.f : 0.f;
+    if (C.pos == 2 && U.buoyant && !(U.hst && gn > 0.f)) {
+        const float hz = gn > 0.f ? (fabsf(gb[0]) * ah0 + fabsf(gb[1]) * ah1 + fabsf(gb[2]) * ah2) / gn : 0.f;
+        frac = hz > 0.f ? fminf(1.f, fmaxf(0.f, ((float)f[10] + hz) / (2.f * hz))) : 0.f;
+    }
     if (U.buoyant && C.pos == 2 && U.hst && gn > 0.f) {
         // the hull's table: static buoyancy on a flat surface against the
         // immersion of its AABB and the direction of gravity (CG frame)
@@ -2361,22 +2361,25 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
             C.base[k] = (double)(sb * tab[k]);
             C.base[3 + k] = (double)(sb * Lc * tab[3 + k]);
         }
-        return;
+    } else {
+        const float fb = U.buoyant ? frac : 0.f;   // analytic buoyancy times the immersed fraction
+        const float pc0 = (float)U.P_CB[0], pc1 = (float)U.P_CB[1], pc2 = (float)U.P_CB[2];
+        const float b0 = -rho * vol * fb * gb[0], b1 = -rho * vol * fb * gb[1], b2 = -rho * vol * fb * gb[2];
+        C.base[0] = b0;
+        C.base[1] = b1;
+        C.base[2] = b2;
+        C.base[3] = pc1 * b2 - pc2 * b1;
+        C.base[4] = pc2 * b0 - pc0 * b2;
+        C.base[5] = pc0 * b1 - pc1 * b0;
     }
-    float frac = 0.f;
-    if (U.buoyant && C.pos == 1) frac = 1.f;
-    if (U.buoyant && C.pos == 2) {
-        const float hz = gn > 0.f ? (fabsf(gb[0]) * ah0 + fabsf(gb[1]) * ah1 + fabsf(gb[2]) * ah2) / gn : 0.f;
-        frac = hz > 0.f ? fminf(1.f, fmaxf(0.f, ((float)f[10] + hz) / (2.f * hz))) : 0.f;
-    }
-    const float pc0 = (float)U.P_CB[0], pc1 = (float)U.P_CB[1], pc2 = (float)U.P_CB[2];
-    const float b0 = -rho * vol * frac * gb[0], b1 = -rho * vol * frac * gb[1], b2 = -rho * vol * frac * gb[2];
-    C.base[0] = b0;
-    C.base[1] = b1;
-    C.base[2] = b2;
-    C.base[3] = pc1 * b2 - pc2 * b1;
-    C.base[4] = pc2 * b0 - pc0 * b2;
-    C.base[5] = pc0 * b1 - pc1 * b0;
+    // scales: the sizes of the buoyancy and drag terms
+    const float v0 = (float)f[0], v1 = (float)f[1], v2 = (float)f[2], w0 = (float)f[3], w1 = (float)f[4],
+                w2 = (float)f[5];
+    const float u = sqrtf(v0 * v0 + v1 * v1 + v2 * v2) + sqrtf(w0 * w0 + w1 * w1 + w2 * w2) * Lc;
+    float sF = rho * gn * vol + rho * 0.25f * (float)U.surface * (u * u * u + u);
+    if (!(sF > 0.f)) sF = 1.f;
+    C.sF = sF;
+    C.sT = (double)(sF * fmaxf(Lc, 1e-6f));
 }
 
 __device__ __forceinline__ void sur_context(const UnitStatic& U, const double* __restrict__ s, const OceanK& oc,
