@@ -78,10 +78,20 @@ int sfh_ocean_destroy(sfh_ocean ocean);
 int sfh_ocean_set_waves(sfh_ocean ocean, const float* data, int host_data, uint32_t n,
                         uint32_t channels, uint32_t layers, const float* lengths);
 
-/* Velocity fields summed by Ocean::GetFluidVelocity (Ocean.cpp:183-201).
- * type 0 = Uniform (params[0..2] = v, Uniform.cpp:51-54).  Jet and Pipe
- * (types 1, 2) are accepted by the oracle only; here they return
- * SFH_ERR_UNSUPPORTED. */
+/* Velocity fields summed by Ocean::GetFluidVelocity (Ocean.cpp:183-201),
+ * in the order added:
+ *   type 0 Uniform (Uniform.cpp:51-54): params[0..2] = v;
+ *   type 1 Jet (Jet.cpp:54-75): params[0..2] = outlet centre c,
+ *          params[3..5] = unit axis n, params[6] = r, params[7] = vout;
+ *   type 2 Pipe (Pipe.cpp:57-78): params[0..2] = p1, params[3..5] = unit
+ *          axis (p2 - p1) / l, params[6] = l, params[7] = r1, params[8] = r2,
+ *          params[9] = vin, params[10] = gamma.
+ * Jets and pipes (at most 4) are evaluated at every face centroid; uniform
+ * currents fold into the body's constant velocity.  Errors: SFH_ERR_UNSUPPORTED
+ * for another type, SFH_ERR_SIZE for a fifth jet or pipe, SFH_ERR_INVALID for
+ * an axis that is not a unit vector or a pipe length <= 0.  With jets or pipes
+ * enabled the surrogate flags every unit out of distribution (its features
+ * carry the uniform current only). */
 int sfh_ocean_add_current(sfh_ocean ocean, int type, int enabled, const double params[12]);
 /* Ocean::EnableCurrents / DisableCurrents (Ocean.cpp:203-211). */
 int sfh_ocean_enable_currents(sfh_ocean ocean, int enabled);

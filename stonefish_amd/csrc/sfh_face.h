@@ -94,15 +94,25 @@ SFH_HD void add3(T* acc, f3 x)
     acc[0] += (T)x.x; acc[1] += (T)x.y; acc[2] += (T)x.z;
 }
 
+// Fluid velocity that varies over the body (jets and pipes, Ocean.cpp:183-201)
+// at a mesh-frame point given in FP64, in the mesh frame: a functor with `on`
+// false when there is none (the uniform part is folded into FaceBody::vb).
+struct NoFlow {
+    static constexpr bool on = false;
+    SFH_HD f3 operator()(double, double, double) const { return mk3(0.f, 0.f, 0.f); }
+};
+
 // Drag integrand of one (possibly clipped) face: centroid c (mesh frame), unit
 // normal n1, area A (SolidEntity.cpp:1660-1683, 1767-1790).  With vm = -vc:
 // quadratic q = vc |vc| (-vc.n1) A = vm (-|vm| (vm.n1) A), applied when
 // vc.n1 < -1e-12; tangential vt = vc - n1 (vc.n1) = n1 (vm.n1) - vm, skin
-// s = vt A when |vt|^2 > 1e-9.
-template <class AccT>
-SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, AccT& acc)
+// s = vt A when |vt|^2 > 1e-9.  A varying fluid velocity uf at the centroid
+// (when Flow::on) enters as vm -= uf.
+template <bool ON, class AccT>
+SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, f3 uf, AccT& acc)
 {
     f3 vm = B.vb + cross(B.w, c);
+    if (ON) vm = vm - uf;
     float vn = dot(vm, n1);
     if (vn > -SFH_APPROACH_VN) {
         float m = fsqrt(dot(vm, vm));
@@ -118,9 +128,17 @@ SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, AccT& acc)
     }
 }
 
-// Submerged face (no clipping, no buoyancy): SolidEntity.cpp:1749-1791.
 template <class AccT>
-SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, AccT& acc)
+SFH_HD void face_drag(const FaceBody& B, f3 c, f3 n1, float A, AccT& acc)
+{
+    face_drag<false>(B, c, n1, A, mk3(0.f, 0.f, 0.f), acc);
+}
+
+// Submerged face (no clipping, no buoyancy): SolidEntity.cpp:1749-1791.  A
+// varying current is sampled at the centroid formed in FP64 (its gradients
+// can be steep: a jet's Gaussian, a pipe's wall profile).
+template <class Flow, class AccT>
+SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, const Flow& flow, AccT& acc)
 {
     f3 e1 = p2 - p1, e2 = p3 - p1;
     f3 n = cross(e1, e2);
@@ -128,7 +146,17 @@ SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, AccT& acc)
     if (n2 < SFH_DEGENERATE_N2) return;
     float inv = frsq(n2);
     f3 c = p1 + (e1 + e2) * (1.0f / 3.0f);
-    face_drag(B, c, n * inv, 0.5f * (n2 * inv), acc);
+    f3 uf = mk3(0.f, 0.f, 0.f);
+    if (Flow::on)
+        uf = flow(((double)p1.x + (double)p2.x + (double)p3.x) / 3.0, ((double)p1.y + (double)p2.y + (double)p3.y) / 3.0,
+                  ((double)p1.z + (double)p2.z + (double)p3.z) / 3.0);
+    face_drag<Flow::on>(B, c, n * inv, 0.5f * (n2 * inv), uf, acc);
+}
+
+template <class AccT>
+SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, AccT& acc)
+{
+    face_submerged(B, p1, p2, p3, NoFlow(), acc);
 }
 
 // Result of clipping one face against the waterline: the wet polygon as a

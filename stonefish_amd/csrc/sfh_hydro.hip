@@ -73,6 +73,17 @@ enum : uint32_t {
 
 static constexpr int kMaxLayers = 4;
 static constexpr int kChunkDoubles = 24;
+static constexpr int kMaxCur = 4;   // jets and pipes evaluated per face
+
+// The jets and pipes of the ocean in one body's mesh frame: anchor (jet
+// outlet c, pipe inlet p1) and unit axis, with the scalar parameters: jet r,
+// vout; pipe l, r1, r2, vin, gamma.
+struct CurBody {
+    int32_t n, pad_;
+    int32_t type[kMaxCur];     // 1 jet, 2 pipe
+    double c[kMaxCur][3], a[kMaxCur][3];
+    double p[kMaxCur][5];
+};
 
 struct BodyStatic {
     const float* verts;
@@ -113,6 +124,7 @@ struct BodyDyn {
     double rz[3];       // row 2 of R: world z of a mesh-frame vector
     double tz;          // world z of the mesh origin
     LayerK layer[kMaxLayers];
+    CurBody cur;        // varying currents (cur.n == 0: none)
     // FP64 epilogue constants
     double R[9];        // basis of T_C, row-major
     double dt[3];       // R^T (t_C - t_CG)
@@ -130,7 +142,10 @@ struct OceanK {
     int32_t nlayers;                  // 0 = flat surface
     int32_t n;
     int32_t nmask;                    // n - 1 when n is a power of two, else -1
-    int32_t pad_;
+    int32_t ncur;                     // enabled jets and pipes
+    int32_t cur_type[kMaxCur];
+    double cur_c[kMaxCur][3], cur_a[kMaxCur][3];   // world anchor and unit axis
+    double cur_p[kMaxCur][5];
 };
 
 // ---------------------------------------------------------------------------
@@ -404,10 +419,61 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
         }
         D.layer[l] = L;
     }
+    D.cur.n = oc.ncur;
+    D.cur.pad_ = 0;
+    for (int i = 0; i < kMaxCur; ++i) {
+        D.cur.type[i] = 0;
+        for (int k = 0; k < 3; ++k) D.cur.c[i][k] = D.cur.a[i][k] = 0.0;
+        for (int k = 0; k < 5; ++k) D.cur.p[i][k] = 0.0;
+        if (i >= oc.ncur) continue;
+        double d[3];
+        for (int k = 0; k < 3; ++k) d[k] = oc.cur_c[i][k] - tc[k];
+        mat_t_vec(R, d, D.cur.c[i]);
+        mat_t_vec(R, oc.cur_a[i], D.cur.a[i]);
+        D.cur.type[i] = oc.cur_type[i];
+        for (int k = 0; k < 5; ++k) D.cur.p[i][k] = oc.cur_p[i][k];
+    }
     for (int i = 0; i < 9; ++i) { D.R[i] = R[i]; D.Rcg[i] = Rcg[i]; }
     for (int i = 0; i < 3; ++i) D.tcg[i] = tcg[i];
     dyn[b] = D;
 }
+
+// Jets and pipes at a mesh-frame point (Jet.cpp:54-75, Pipe.cpp:57-78): the
+// reference's formulas, in FP64 as there (Scalar), on the body-relative
+// frame of CurBody; the result is the face loop's FP32.
+struct CurFlow {
+    static constexpr bool on = true;
+    const CurBody* C;
+    __device__ f3 operator()(double x, double y, double z) const
+    {
+        double u[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < C->n; ++i) {
+            const double* n = C->a[i];
+            const double cp[3] = {x - C->c[i][0], y - C->c[i][1], z - C->c[i][2]};
+            double xc[3];
+            cross64(cp, n, xc);
+            const double d = sqrt(xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
+            const double t = cp[0] * n[0] + cp[1] * n[1] + cp[2] * n[2];
+            const double* p = C->p[i];
+            double f, vm;
+            if (C->type[i] == 1) {   // jet: r = p0, vout = p1
+                if (t < 0.0) continue;
+                const double rr = 1.0 / 5.0 * (t + 5.0 * p[0]);
+                if (d >= rr) continue;
+                vm = 10.0 * p[0] / (t + 5.0 * p[0]) * p[1];
+                f = exp(-50.0 * d * d / (t * t));
+            } else {                 // pipe: l, r1, r2, vin, gamma = p0..p4
+                if (t < 0.0 || t > p[0]) continue;
+                const double r = p[1] + (p[2] - p[1]) * t / p[0];
+                if (d >= r) continue;
+                vm = p[1] / r * p[3];
+                f = pow(1.0 - d / r, p[4]);
+            }
+            for (int k = 0; k < 3; ++k) u[k] += f * (vm * n[k]);
+        }
+        return mk3((float)u[0], (float)u[1], (float)u[2]);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // k_depths: FP64 depth of every vertex of the bodies crossing the surface
@@ -709,7 +775,7 @@ __device__ __forceinline__ int xreduce12_slot(int lane)
 // with FP32 per-lane partial sums over the chunk's K faces per lane, reduced
 // across the wavefront and across chunks in FP64.  `list` holds the ids of the
 // chunks of this step's inside bodies.
-template <int K>
+template <int K, bool NU>
 __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ count,
                                                    const Chunk* __restrict__ chunks,
@@ -730,6 +796,7 @@ __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ 
         const float* __restrict__ verts = S.verts;
         const uint32_t* __restrict__ faces = S.faces + 3 * (size_t)f0;
         const FaceBody fb = dyn[b].fb;
+        const CurFlow flow{&dyn[b].cur};
         AccDrag<float> acc;
         acc_zero(acc);
         // Two-stage software pipeline over the lane's K faces: the indices of
@@ -745,7 +812,12 @@ __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ 
             const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
             f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
             uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
-            if (64u * j + lane < nfc) face_submerged(fb, p1, p2, p3, acc);
+            if (64u * j + lane < nfc) {
+                if (NU)
+                    face_submerged(fb, p1, p2, p3, flow, acc);
+                else
+                    face_submerged(fb, p1, p2, p3, acc);
+            }
             p1 = q1; p2 = q2; p3 = q3;
             b0 = c0; b1 = c1; b2 = c2;
         }
@@ -763,6 +835,7 @@ __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ 
 
 // Contributions of a wet (possibly clipped) face: pressure buoyancy at the
 // wet centroid when the surface has waves (1649-1658), drag, wetted area.
+template <bool NU>
 __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, const BodyDyn& D, const OceanK& oc,
                                             bool wave_buoy, bool damping, Acc<float>& acc)
 {
@@ -775,7 +848,15 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
         add3(acc.Fb, fbi);
         add3(acc.Mb, cross(cpt, fbi));
     }
-    if (damping) face_drag(fb, cpt, wt.n1, wt.A, acc);
+    if (damping) {
+        if (NU)
+            face_drag<true>(fb, cpt, wt.n1, wt.A,
+                            CurFlow{&D.cur}((double)wt.base.x + (double)wt.off.x, (double)wt.base.y + (double)wt.off.y,
+                                            (double)wt.base.z + (double)wt.off.z),
+                            acc);
+        else
+            face_drag(fb, cpt, wt.n1, wt.A, acc);
+    }
     acc.Swet += wt.A;
 }
 
@@ -786,7 +867,7 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
 // cuts (clip cases A-F) are queued in LDS by each wavefront and processed
 // together afterwards, 64 at a time, so the clip code runs once per 64 cut
 // faces rather than on every pass over the chunk.
-template <int K>
+template <int K, bool NU>
 __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
                                                      const Chunk* __restrict__ chunks,
@@ -833,7 +914,8 @@ __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict_
             const bool valid = 64u * j + lane < nfc;
             const bool allwet = !(d0 < 0.f) && !(d1 < 0.f) && !(d2 < 0.f);
             const bool cut = valid && !allwet && !(d0 < 0.f && d1 < 0.f && d2 < 0.f);
-            if (valid && allwet) wet_contrib(face_allwet(fb, p1, p2, p3, acc), fb, D, oc, wave_buoy, damping, acc);
+            if (valid && allwet)
+                wet_contrib<NU>(face_allwet(fb, p1, p2, p3, acc), fb, D, oc, wave_buoy, damping, acc);
             const uint64_t m = __ballot(cut);
             if (cut) cutq[wl][ncut + (uint32_t)__popcll(m & below)] = (uint16_t)(64u * j + lane);
             ncut += (uint32_t)__popcll(m);
@@ -850,7 +932,7 @@ __global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict_
                 const uint32_t i0 = faces[3 * fo], i1 = faces[3 * fo + 1], i2 = faces[3 * fo + 2];
                 const Wet wt = face_clip(fb, vert(verts, i0), vert(verts, i1), vert(verts, i2), dep[i0], dep[i1],
                                          dep[i2], acc);
-                wet_contrib(wt, fb, D, oc, wave_buoy, damping, acc);
+                wet_contrib<NU>(wt, fb, D, oc, wave_buoy, damping, acc);
             }
         }
         __builtin_amdgcn_wave_barrier();   // the queue is rewritten by the next chunk
@@ -1261,9 +1343,20 @@ static OceanK make_oceank(const sfh_ocean_t* o)
         k.quad[l] = o->d_quad[l];
     }
     if (o->currents_enabled)
-        for (const Current& c : o->currents)
-            if (c.enabled && c.type == 0)
+        for (const Current& c : o->currents) {
+            if (!c.enabled) continue;
+            if (c.type == 0) {
                 for (int i = 0; i < 3; ++i) k.u[i] += c.p[i];
+            } else if (k.ncur < kMaxCur) {   // sfh_ocean_add_current keeps at most kMaxCur
+                const int j = k.ncur++;
+                k.cur_type[j] = c.type;
+                for (int i = 0; i < 3; ++i) {
+                    k.cur_c[j][i] = c.p[i];
+                    k.cur_a[j][i] = c.p[3 + i];
+                }
+                for (int i = 0; i < 5; ++i) k.cur_p[j][i] = c.p[6 + i];
+            }
+        }
     return k;
 }
 
@@ -1424,7 +1517,15 @@ extern "C" int sfh_ocean_set_waves(sfh_ocean o, const float* data, int host_data
 extern "C" int sfh_ocean_add_current(sfh_ocean o, int type, int enabled, const double params[12])
 {
     if (!o || !params) return fail(SFH_ERR_INVALID, "null argument");
-    if (type != 0) return fail(SFH_ERR_UNSUPPORTED, "only uniform currents are implemented on the device");
+    if (type < 0 || type > 2) return fail(SFH_ERR_UNSUPPORTED, "current type %d (0 uniform, 1 jet, 2 pipe)", type);
+    if (type != 0) {
+        int nvar = 0;
+        for (const Current& x : o->currents) nvar += x.type != 0;
+        if (nvar >= kMaxCur) return fail(SFH_ERR_SIZE, "at most %d jets and pipes", kMaxCur);
+        const double an = params[3] * params[3] + params[4] * params[4] + params[5] * params[5];
+        if (!(fabs(an - 1.0) < 1e-9)) return fail(SFH_ERR_INVALID, "the axis params[3..5] must be a unit vector");
+        if (type == 2 && !(params[6] > 0.0)) return fail(SFH_ERR_INVALID, "pipe length must be positive");
+    }
     Current c;
     c.type = type;
     c.enabled = enabled;
@@ -1794,18 +1895,25 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
     if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
     uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
     uint32_t blocks = (waves + 3) / 4;
-#define LAUNCH_K(KK)                                                                                      \
-    case KK:                                                                                              \
-        hipLaunchKernelGGL(k_faces_sub<KK>, dim3(blocks), dim3(256), 0, s, list_sub, counts, b->d_chunks, \
-                           b->d_static, b->d_dyn, b->d_partials);                                       \
-        hipLaunchKernelGGL(k_faces_cross<KK>, dim3(blocks), dim3(256), 0, s, list_cross, counts + 1,     \
-                           b->d_chunks, b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);          \
+#define LAUNCH_KN(KK, NU)                                                                                 \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, NU>), dim3(blocks), dim3(256), 0, s, list_sub, counts,  \
+                       b->d_chunks, b->d_static, b->d_dyn, b->d_partials);                                    \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_cross<KK, NU>), dim3(blocks), dim3(256), 0, s, list_cross,      \
+                       counts + 1, b->d_chunks, b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);
+#define LAUNCH_K(KK)                  \
+    case KK:                          \
+        if (oc.ncur) {                \
+            LAUNCH_KN(KK, true)       \
+        } else {                      \
+            LAUNCH_KN(KK, false)      \
+        }                             \
         break;
     switch (b->K) {
         LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16)
         default: return fail(SFH_ERR_INVALID, "bad K");
     }
 #undef LAUNCH_K
+#undef LAUNCH_KN
     return SFH_OK;
 }
 
@@ -1834,9 +1942,6 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
 static int check_ocean(const sfh_ocean_t* o)
 {
     if (!o) return fail(SFH_ERR_INVALID, "null ocean");
-    if (o->currents_enabled)
-        for (const Current& c : o->currents)
-            if (c.enabled && c.type != 0) return fail(SFH_ERR_UNSUPPORTED, "non-uniform current");
     return SFH_OK;
 }
 
@@ -2398,9 +2503,11 @@ __device__ __forceinline__ void sur_input_row(const double* f, const SurNorm& nm
     }
 }
 
-__device__ __forceinline__ uint8_t sur_ood(const double* f, int pos, const SurNorm& nm)
+// Out of distribution: a feature outside the training range, or jets and
+// pipes in the ocean (the features carry only the uniform current).
+__device__ __forceinline__ uint8_t sur_ood(const double* f, int pos, const SurNorm& nm, const OceanK& oc)
 {
-    uint8_t o = 0;
+    uint8_t o = oc.ncur > 0 ? 1 : 0;
     for (int k = 0; k < SFH_SUR_FEATURES; ++k) {
         const float v = (float)f[k];
         if (!(v >= nm.in_lo[k] && v <= nm.in_hi[k])) o = 1;   // also catches NaN
@@ -2431,7 +2538,7 @@ __global__ void k_sur_features(const UnitStatic* __restrict__ units, const doubl
     if (posout) posout[i] = (int8_t)pos;
     if (raw)
         for (int k = 0; k < SFH_SUR_FEATURES; ++k) raw[(size_t)i * SFH_SUR_FEATURES + k] = (float)f[k];
-    if (ood) ood[i] = sur_ood(f, pos, nm);
+    if (ood) ood[i] = sur_ood(f, pos, nm, oc);
     if (x) {
         uint32_t w[SFH_SUR_ROW / 2];
         sur_input_row(f, nm, w);
@@ -2680,7 +2787,7 @@ __global__ __launch_bounds__(256) void k_sur_step(const SurPlanDev* __restrict__
                 sur_input_row(C.f, nm, w);
                 for (int k = 0; k < SFH_SUR_ROW / 8; ++k)
                     xr[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
-                if (ood) ood[u] = sur_ood(C.f, C.pos, nm);
+                if (ood) ood[u] = sur_ood(C.f, C.pos, nm, oc);
                 if (posout) posout[u] = (int8_t)C.pos;
                 SurRowCtx& X = ctx[r];
                 for (int k = 0; k < 9; ++k) X.R[k] = C.R[k];

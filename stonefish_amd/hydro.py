@@ -113,6 +113,27 @@ class Ocean:
         p[:params.size] = params
         N.check(N.load().sfh_ocean_add_current(self.handle, int(kind), int(enabled), _dp(p)))
 
+    def add_jet(self, point, direction, radius: float, outlet_velocity: float, enabled: bool = True) -> None:
+        """Jet(point, direction, radius, outletVelocity) (Jet.cpp:31-37): axis normalised."""
+        d = _vec(direction)
+        p = np.zeros(12)
+        p[:3] = _vec(point)
+        p[3:6] = d / np.linalg.norm(d)
+        p[6], p[7] = radius, outlet_velocity
+        self.add_current(1, p, enabled)
+
+    def add_pipe(self, point1, point2, radius1: float, radius2: float, inlet_velocity: float, exponent: float,
+                 enabled: bool = True) -> None:
+        """Pipe(point1, point2, radius1, radius2, inletVelocity, exponent) (Pipe.cpp:31-41):
+        length |p2 - p1|, axis (p2 - p1) / length."""
+        p1, p2 = _vec(point1), _vec(point2)
+        ln = float(np.linalg.norm(p2 - p1))
+        p = np.zeros(12)
+        p[:3] = p1
+        p[3:6] = (p2 - p1) / ln
+        p[6], p[7], p[8], p[9], p[10] = ln, radius1, radius2, inlet_velocity, exponent
+        self.add_current(2, p, enabled)
+
     def enable_currents(self, on: bool = True) -> None:
         N.check(N.load().sfh_ocean_enable_currents(self.handle, int(on)))
 

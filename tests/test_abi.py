@@ -5,6 +5,8 @@ import ctypes as C
 import re
 from pathlib import Path
 
+import pytest
+
 from stonefish_amd import _native as N
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -52,3 +54,24 @@ def test_mlp_argument_checks_without_device():
     assert rc == -2
     assert b"multiple" in lib.sfm_last_error()
     assert lib.sfm_destroy(None) == -1
+
+
+def test_current_argument_checks_without_device():
+    """sfh_ocean_add_current: jets and pipes in the reference's parameter
+    layout; bad types, non-unit axes, non-positive pipe lengths and a fifth
+    jet or pipe are refused (host-side checks, no GPU needed)."""
+    from stonefish_amd import hydro as H
+    o = H.Ocean()
+    o.add_uniform_current([0.2, 0.0, 0.0])
+    o.add_jet([0, 0, 1], [2, 0, 0], 0.5, 3.0)   # the axis is normalised as in Jet::Jet
+    o.add_pipe([0, 0, 0], [0, 4, 0], 1.0, 0.5, 1.0, 1 / 7)
+    lib = N.load()
+    for kind, params in ((1, [0, 0, 0, 1, 1, 0, 0.5, 1]), (3, [0] * 12), (2, [0, 0, 0, 1, 0, 0, 0.0, 1, 1, 1, 1])):
+        with pytest.raises(N.SfhError):
+            o.add_current(kind, params)
+    o.add_jet([1, 0, 1], [0, 0, 1], 0.2, 1.0)
+    o.add_pipe([0, 1, 0], [0, 1, 3], 0.4, 0.4, 1.0, 0.5)
+    with pytest.raises(N.SfhError):
+        o.add_jet([2, 0, 1], [0, 1, 0], 0.2, 1.0)
+    assert b"at most 4" in lib.sfh_last_error()
+    o.close()

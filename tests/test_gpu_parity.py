@@ -126,6 +126,93 @@ def test_batch_fleet(gpu, waves):
         np.testing.assert_allclose(g[m, 19], r[m, 19], rtol=RTOL)
 
 
+def _jet_pipe_oceans(centre, waves):
+    """The same jets, pipe and uniform current on the device ocean and the
+    oracle's, laid across a body at `centre`; optionally waves.  The pipe's
+    exponent is 1.5: with an exponent below 1 the profile (1 - d/r)^gamma has
+    an unbounded slope at the wall, where a 1e-7 m change of a face centroid
+    (the reference's own FP32 centroids) moves the velocity by ~1e-3 m/s, so
+    no FP32 evaluation, the reference's included, meets the FP64 bound there."""
+    c = np.asarray(centre, dtype=float)
+    jet = (c + [-2.0, 0.1, 0.05], [1.0, 0.05, 0.0], 0.4, 3.0)
+    jet2 = (c + [0.2, -2.5, 0.1], [0.0, 1.0, -0.1], 0.3, 2.0)
+    pipe = (c + [0.3, -3.0, -0.2], c + [0.3, 3.0, 0.2], 0.8, 0.5, 1.5, 1.5)
+    go = H.Ocean()
+    go.add_uniform_current([0.2, -0.1, 0.0])
+    go.add_jet(*jet)
+    go.add_pipe(*pipe)
+    go.add_jet(*jet2)
+    go.enable_currents(True)
+    cur = [(0, 1, [0.2, -0.1, 0.0])]
+    for pt, d, r, v in (jet, jet2):
+        d = np.asarray(d) / np.linalg.norm(d)
+        cur.append((1, 1, list(pt) + list(d) + [r, v]))
+    p1, p2, r1, r2, vin, gam = pipe
+    ln = float(np.linalg.norm(p2 - p1))
+    cur.insert(2, (2, 1, list(p1) + list((p2 - p1) / ln) + [ln, r1, r2, vin, gam]))
+    kw = {}
+    if waves:
+        comps = W.pm_components(16, 6.0, 64.0, seed=3)
+        grid = W.grid_reference(comps, 64).astype(np.float32)
+        go.set_waves(grid, 64, 1, [64.0])
+        kw = dict(wave_data=grid, wave_n=64, channels=1, lengths=[64.0])
+    oo = O.Ocean(currents=cur, currents_enabled=True, **kw)
+    return go, oo
+
+
+def test_jets_and_pipes_submerged(gpu):
+    """Jets and a pipe (evaluated per face centroid) over a submerged capsule."""
+    log("case: submerged capsule in two jets, a pipe and a uniform current")
+    w = WL.capsule_submerged()
+    tm = w.bodies[0].mesh
+    gm, om = H.Mesh(tm), O.Mesh.from_tri(tm)
+    go, oo = _jet_pipe_oceans([0.0, 0.0, 5.0], False)
+    rng = np.random.default_rng(17)
+    n = 16
+    g, r, x = np.zeros((n, 24)), np.zeros((n, 24)), np.zeros((n, 22))
+    for s in range(n):
+        q = T.random_unit_quaternions(rng, 1)[0]
+        Tcg = T.quat_to_gl(q, [rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), 5.0 + rng.uniform(-0.3, 0.3)])
+        v, om_ = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        Fdq, Tdq, Fdf, Tdf = (np.zeros(3) for _ in range(4))
+        H.ComputeHydrodynamicForcesSubmerged(gm, go, Tcg, Tcg, v, om_, Fdq, Tdq, Fdf, Tdf)
+        g[s, 6:9], g[s, 9:12], g[s, 12:15], g[s, 15:18] = Fdq, Tdq, Fdf, Tdf
+        r[s], x[s] = O.submerged(64, om, oo, Tcg, Tcg, v, om_, full=True)
+        r[s, 20] = 60
+    # the currents change the result (they are not ignored)
+    go0, oo0 = H.Ocean(), O.Ocean()
+    r0, _ = O.submerged(64, om, oo0, Tcg, Tcg, v, om_, full=True)
+    assert np.linalg.norm(r0[6:9] - r[n - 1, 6:9]) > 1e-3 * np.linalg.norm(r[n - 1, 6:9])
+    assert_bound(g, r, x, "submerged capsule, jets and pipe", names=("Fdq", "Tdq", "Fdf", "Tdf"))
+
+
+@pytest.mark.parametrize("waves", [False, True])
+def test_jets_and_pipes_surface(gpu, waves):
+    """Jets and a pipe over an ellipsoid crossing the surface (clipped faces
+    take the current at their wet centroid)."""
+    log(f"case: surface ellipsoid in two jets, a pipe and a uniform current, waves={waves}")
+    w = WL.ellipsoid_surface()
+    tm = w.bodies[0].mesh
+    gm, om = H.Mesh(tm), O.Mesh.from_tri(tm)
+    go, oo = _jet_pipe_oceans([0.0, 0.0, 0.1], waves)
+    rng = np.random.default_rng(23 + int(waves))
+    n = 12
+    g, r, x = np.zeros((n, 24)), np.zeros((n, 24)), np.zeros((n, 22))
+    settings = H.HydrodynamicsSettings(True, True)
+    for s in range(n):
+        q = T.random_unit_quaternions(rng, 1)[0]
+        Tcg = T.quat_to_gl(q, [rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-0.2, 0.2)])
+        v, om_ = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        outs = [np.zeros(3) for _ in range(6)]
+        sv = np.zeros(2)
+        H.ComputeHydrodynamicForcesSurface(settings, gm, go, Tcg, Tcg, v, om_, *outs, sv)
+        g[s, :18] = np.concatenate(outs)
+        g[s, 18:20] = sv
+        r[s], x[s] = O.surface(64, 1, 1, om, oo, Tcg, Tcg, v, om_, full=True)
+    assert_bound(g, r, x, f"surface ellipsoid, jets and pipe, waves={waves}")
+    np.testing.assert_allclose(g[:, 18], r[:, 18], rtol=RTOL)
+
+
 def test_reference_wave_layout(gpu):
     """Two-layer RGBA texel layout of the reference's fftData (channels 0 and 1,
     sizes 893 m and 101 m, OpenGLOcean.cpp:70)."""

@@ -267,6 +267,14 @@ def test_normalised_inputs_and_ood(gpu):
     expect = torch.from_numpy(((r[ok] - norm.in_shift) * norm.in_scale).astype(np.float32)).to(torch.bfloat16).float()
     assert np.array_equal(xo[:batch.n][ok][:, :13], expect.numpy())
     assert np.all(xo[:, 13:] == 0) and np.all(xo[batch.n:] == 0)
+    # a jet makes every body in the water out of distribution (the features
+    # carry the uniform current only)
+    ocean.add_jet([5.0, 5.0, 0.2], [1.0, 0.0, 0.0], 0.5, 1.0)
+    pos = torch.zeros(batch.n, dtype=torch.int8, device="cuda")
+    S.features(batch, ocean, ds, 0, batch.n, norm=norm, rows=rows, x=x, ood=ood, pos=pos)
+    torch.cuda.synchronize()
+    wet = pos.cpu().numpy() > 0
+    assert wet.sum() > 0 and np.all(ood.cpu().numpy()[wet] == 1)
 
 
 def test_masked_compute_and_apply(gpu):

@@ -467,3 +467,69 @@ def test_ref32_close_to_ref64():
         for k in (0, 2, 4):
             np.testing.assert_allclose(b[3 * k:3 * k + 3], a[3 * k:3 * k + 3], rtol=1e-3,
                                        atol=1e-3 * np.abs(a[3 * k:3 * k + 3]).max())
+
+
+# ---------------------------------------------------------------- velocity fields
+
+def _velocity(ocn, p):
+    out = arr(3)
+    O.load().orc_fluid_velocity(C.byref(ocn.s), d(p), out)
+    return np.array(out[:])
+
+
+def test_jet_and_pipe_velocity_closed_forms():
+    """Jet (Jet.cpp:54-75): on the axis u = 10 r / (t + 5 r) vout n (vout at
+    t = 5 r); off the axis the Gaussian exp(-50 d^2 / t^2); nothing behind the
+    outlet or outside the cone of radius (t + 5 r) / 5.  Pipe (Pipe.cpp:57-78):
+    on the axis r1 / r(t) vin n with r(t) linear from r1 to r2; at half the
+    radius the factor 0.5^gamma; nothing past either end or outside r(t).
+    Fields add (Ocean.cpp:183-196) and are ignored when disabled."""
+    n = np.array([0.0, 0.6, 0.8])
+    c = np.array([1.0, -2.0, 3.0])
+    r, vout = 0.4, 2.5
+    e = np.cross(n, [1.0, 0.0, 0.0])
+    e /= np.linalg.norm(e)
+    jet = (1, 1, list(c) + list(n) + [r, vout])
+    ocn = O.Ocean(currents=[jet], currents_enabled=True)
+    np.testing.assert_allclose(_velocity(ocn, c + 5 * r * n), vout * n, rtol=1e-14)
+    t = 0.7
+    np.testing.assert_allclose(_velocity(ocn, c + t * n), 10 * r / (t + 5 * r) * vout * n, rtol=1e-14)
+    dd = 0.05
+    np.testing.assert_allclose(_velocity(ocn, c + t * n + dd * e),
+                               np.exp(-50 * dd * dd / (t * t)) * 10 * r / (t + 5 * r) * vout * n, rtol=1e-12)
+    assert np.all(_velocity(ocn, c - 0.1 * n) == 0)
+    assert np.all(_velocity(ocn, c + t * n + 1.001 * (t + 5 * r) / 5 * e) == 0)
+    p1, ln, r1, r2, vin, gam = np.array([0.0, 0.0, 0.0]), 4.0, 1.0, 0.5, 1.5, 1 / 7
+    a = np.array([1.0, 0.0, 0.0])
+    pipe = (2, 1, list(p1) + list(a) + [ln, r1, r2, vin, gam])
+    ocn = O.Ocean(currents=[pipe], currents_enabled=True)
+    rt = r1 + (r2 - r1) * 0.5
+    np.testing.assert_allclose(_velocity(ocn, p1 + 2.0 * a), r1 / rt * vin * a, rtol=1e-14)
+    side = np.array([0.0, 0.0, 1.0])
+    np.testing.assert_allclose(_velocity(ocn, p1 + 2.0 * a + 0.5 * rt * side), 0.5 ** gam * r1 / rt * vin * a,
+                               rtol=1e-12)
+    assert np.all(_velocity(ocn, p1 + 4.01 * a) == 0) and np.all(_velocity(ocn, p1 - 0.01 * a) == 0)
+    assert np.all(_velocity(ocn, p1 + 2.0 * a + 1.01 * rt * side) == 0)
+    both = O.Ocean(currents=[pipe, (0, 1, [0.1, 0.2, 0.3]), jet], currents_enabled=True)
+    q = p1 + 2.0 * a
+    np.testing.assert_allclose(_velocity(both, q), r1 / rt * vin * a + [0.1, 0.2, 0.3], rtol=1e-14)
+    assert np.all(_velocity(O.Ocean(currents=[pipe], currents_enabled=False), q) == 0)
+    assert np.all(_velocity(O.Ocean(currents=[(2, 0, pipe[2])], currents_enabled=True), q) == 0)
+
+
+def test_body_in_uniform_pipe_core_equals_moving_body():
+    """A pipe with r1 = r2 and gamma = 0 is uniform (vin n) inside its radius:
+    a submerged box at rest inside it feels the drag of the same box moving
+    at -vin n through still water (the per-face velocity path against the
+    relative-velocity closed form)."""
+    hx, hy, hz = (float(np.float32(x)) for x in (0.7, 0.4, 0.2))
+    m = O.Mesh.from_tri(M.box(hx, hy, hz, 5))
+    rng = np.random.default_rng(3)
+    Tcg = T.quat_to_gl(T.random_unit_quaternions(rng, 1)[0], [0.0, 0.0, 5.0])
+    vin = 1.3
+    a = np.array([0.6, 0.0, 0.8])
+    pipe = (2, 1, list(np.array([0.0, 0.0, 5.0]) - 10 * a) + list(a) + [20.0, 5.0, 5.0, vin, 0.0])
+    r_pipe = O.submerged(64, m, O.Ocean(currents=[pipe], currents_enabled=True), Tcg, Tcg, [0, 0, 0], [0, 0, 0])
+    r_move = O.submerged(64, m, O.Ocean(), Tcg, Tcg, -vin * a, [0, 0, 0])
+    np.testing.assert_allclose(r_pipe[6:18], r_move[6:18], rtol=1e-12, atol=1e-12)
+    assert np.linalg.norm(r_move[6:9]) > 0.1
