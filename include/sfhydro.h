@@ -352,6 +352,55 @@ int sfh_surrogate_run(sfh_sur_plan plan, sfh_ocean ocean, const double* d_states
 int sfh_surrogate_merge(sfh_sur_plan plan, const uint8_t* d_ood, const double* d_exact, double* d_ft, float* d_err,
                         void* stream);
 
+/* ---- hull preprocessing (host, once per hull) -------------------------------- */
+
+/* ComputePhysicalProperties (GeometryFileUtil.cpp:335-563): mass, volume,
+ * surface, CG, principal moments Ipri and principal axes Irot (row-major,
+ * axes as columns) of a closed triangle mesh (thickness <= 0: solid) or of a
+ * shell of the given thickness.  xyz: nv x 3 floats (mesh frame), idx:
+ * nf x 3 vertex indices. */
+typedef struct {
+    double mass, volume, surface;
+    double CG[3];
+    double Ipri[3];
+    double Irot[9];
+} sfh_mesh_properties;
+
+int sfh_compute_physical_properties(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nf,
+                                    double thickness, double density, sfh_mesh_properties* out);
+
+/* SolidEntity::ComputeFluidDynamicsApprox (SolidEntity.cpp:698-1039): the
+ * body's fluid-dynamics proxy from its physics-mesh vertices, as the
+ * reference computes it for a rigid body.
+ *   type: SFH_APPROX_AUTO / _ELLIPSOID (the Kumar-Yildirim minimum-volume
+ *         axis-aligned ellipsoid, at most 10 iterations, 20 % volume
+ *         tolerance), _SPHERE, _CYLINDER (GeometryApproxType order);
+ *   T_CG2C, T_CG2O: OpenGL column-major 4x4 transforms (CG -> collision
+ *         geometry, CG -> origin); P_CB: centre of buoyancy, CG frame;
+ *   rho: the liquid's density (the reference uses 1000 without an ocean).
+ * Outputs the proxy type and parameters (sphere r; cylinder r, length;
+ * ellipsoid semi-axes), its frame T_CG2H, added mass and inertia, and the
+ * drag and skin-friction coefficients Cd, Cf (origin frame, Cf = 0.1 Cd) of
+ * SetHydrodynamicCoefficients.  The reference's quirks are kept: the sphere's
+ * added mass uses the squared radius in place of the radius, and Lamb's k
+ * factor takes e = 1 - r2^2 / r1. */
+#define SFH_APPROX_AUTO 0
+#define SFH_APPROX_SPHERE 1
+#define SFH_APPROX_CYLINDER 2
+#define SFH_APPROX_ELLIPSOID 3
+typedef struct {
+    int32_t type;          /* SFH_APPROX_SPHERE / _CYLINDER / _ELLIPSOID */
+    int32_t iterations;    /* ellipsoid: iterations run */
+    double params[3];
+    double T_CG2H[16];
+    double added_mass[3], added_inertia[3];
+    double Cd[3], Cf[3];
+} sfh_fluid_approx;
+
+int sfh_compute_fluid_dynamics_approx(const float* xyz, uint32_t nv, int type, const double T_CG2C[16],
+                                      const double P_CB[3], const double T_CG2O[16], double rho,
+                                      sfh_fluid_approx* out);
+
 /* ---- rollout harness (no counterpart in the hydrodynamics path) ------------ */
 
 /* Rigid-body data of identical bodies: mass and principal moments of inertia

@@ -94,6 +94,20 @@ class sfh_rigid(C.Structure):
     _fields_ = [("mass", C.c_double), ("inertia", C.c_double * 3)]
 
 
+class sfh_mesh_properties(C.Structure):
+    _fields_ = [("mass", C.c_double), ("volume", C.c_double), ("surface", C.c_double), ("CG", C.c_double * 3),
+                ("Ipri", C.c_double * 3), ("Irot", C.c_double * 9)]
+
+
+SFH_APPROX_AUTO, SFH_APPROX_SPHERE, SFH_APPROX_CYLINDER, SFH_APPROX_ELLIPSOID = 0, 1, 2, 3
+
+
+class sfh_fluid_approx(C.Structure):
+    _fields_ = [("type", C.c_int32), ("iterations", C.c_int32), ("params", C.c_double * 3),
+                ("T_CG2H", C.c_double * 16), ("added_mass", C.c_double * 3), ("added_inertia", C.c_double * 3),
+                ("Cd", C.c_double * 3), ("Cf", C.c_double * 3)]
+
+
 # name -> (restype, argtypes)
 _PROTOS = {
     "sfh_last_error": (C.c_char_p, []),
@@ -138,6 +152,11 @@ _PROTOS = {
     "sfh_surrogate_plan_destroy": (C.c_int, [C.c_void_p]),
     "sfh_surrogate_run": (C.c_int, [C.c_void_p] * 7),
     "sfh_surrogate_merge": (C.c_int, [C.c_void_p] * 6),
+    "sfh_compute_physical_properties": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_double,
+                                                  C.c_double, C.POINTER(sfh_mesh_properties)]),
+    "sfh_compute_fluid_dynamics_approx": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_double),
+                                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                                                    C.POINTER(sfh_fluid_approx)]),
     "sfh_rollout_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(sfh_rigid),
                                         C.c_double, C.c_void_p]),
     "sfh_surrogate_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,

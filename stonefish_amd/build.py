@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 
 LIBS = {
-    "libsfhydro.so": ["sfh_hydro.hip"],
+    "libsfhydro.so": ["sfh_hydro.hip", "sfh_hull.cpp"],
     "libsfmlp.so": ["sfh_mlp.hip"],
 }
 

@@ -55,6 +55,9 @@ static int fail(int code, const char* fmt, ...)
 
 extern "C" const char* sfh_last_error(void) { return g_err; }
 
+// The error path of the host-only translation units of the library (sfh_hull.cpp).
+int sfh_internal_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+
 // ---------------------------------------------------------------------------
 // device-side data
 // ---------------------------------------------------------------------------

@@ -364,3 +364,36 @@ class CompoundBatch(BodyBatch):
         self.handle = h
         self.n = len(comps)
         self.bodies = comps
+
+
+# ---- hull preprocessing (host) -------------------------------------------------
+
+def compute_physical_properties(verts, faces, thickness: float = 0.0, density: float = 1000.0) -> dict:
+    """ComputePhysicalProperties (GeometryFileUtil.cpp:335-563) through
+    sfh_compute_physical_properties: mass, volume, surface, CG, principal
+    moments Ipri and axes Irot (columns) of a closed mesh (thickness <= 0) or
+    a shell."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+    out = N.sfh_mesh_properties()
+    N.check(N.load().sfh_compute_physical_properties(v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0],
+                                                     float(thickness), float(density), C.byref(out)))
+    return {"mass": out.mass, "volume": out.volume, "surface": out.surface, "CG": np.array(out.CG[:]),
+            "Ipri": np.array(out.Ipri[:]), "Irot": np.array(out.Irot[:]).reshape(3, 3)}
+
+
+def compute_fluid_dynamics_approx(verts, kind: int = N.SFH_APPROX_AUTO, T_CG2C=None, P_CB=(0.0, 0.0, 0.0),
+                                  T_CG2O=None, rho: float = 1000.0) -> dict:
+    """SolidEntity::ComputeFluidDynamicsApprox (SolidEntity.cpp:698-1039)
+    through sfh_compute_fluid_dynamics_approx: the proxy (sphere, cylinder or
+    ellipsoid), its frame, added mass and inertia, and Cd, Cf (origin frame)."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    eye = np.eye(4).reshape(-1)
+    a = np.ascontiguousarray(T_CG2C if T_CG2C is not None else eye, dtype=np.float64)
+    o = np.ascontiguousarray(T_CG2O if T_CG2O is not None else eye, dtype=np.float64)
+    out = N.sfh_fluid_approx()
+    N.check(N.load().sfh_compute_fluid_dynamics_approx(v.ctypes.data, v.shape[0], int(kind), _dp(a),
+                                                       _dp(_vec(P_CB)), _dp(o), float(rho), C.byref(out)))
+    return {"type": out.type, "iterations": out.iterations, "params": np.array(out.params[:]),
+            "T_CG2H": np.array(out.T_CG2H[:]), "added_mass": np.array(out.added_mass[:]),
+            "added_inertia": np.array(out.added_inertia[:]), "Cd": np.array(out.Cd[:]), "Cf": np.array(out.Cf[:])}

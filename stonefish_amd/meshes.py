@@ -68,6 +68,15 @@ def _revolve(profile_r: np.ndarray, profile_z: np.ndarray, n_around: int) -> Tri
     return TriMesh(V, F)
 
 
+def cylinder(radius: float, length: float, n_around: int = 32, bands: int = 4) -> TriMesh:
+    """Closed cylinder about z, `length` end to end, flat caps (fans from the
+    axis)."""
+    zs = np.linspace(-0.5 * length, 0.5 * length, bands + 1)
+    pr = np.concatenate([[0.0], np.full(bands + 1, radius), [0.0]])
+    pz = np.concatenate([[zs[0]], zs, [zs[-1]]])
+    return _revolve(pr, pz, n_around)
+
+
 def capsule(radius: float, length: float, n_around: int = 32, cap_bands: int = 8, body_bands: int = 17,
             axis: int = 0) -> TriMesh:
     """Capsule, `length` tip to tip along `axis`.  Faces: 2*n_around*(2*cap_bands-1+body_bands).
