@@ -233,7 +233,8 @@ def rollout_section(args, sea) -> dict:
     norm = ds.fit_norm(0)
     x, t, _ = ds.inputs(0, norm, range(ds.train_steps))
     net = S.Net(65536, seed=7)
-    hist = S.train(net, x, t, S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=3e-3, warmup=200))
+    hist = S.train(net, x, t, S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=args.sur_lr,
+                                            warmup=200))
     held = S.evaluate(ds, 0, net, norm)
     log(f"rollout surrogate trained ({ds.gen_seconds:.1f}s data, {hist['seconds']:.1f}s train): "
         f"F median {held['F_median']:.4f}")
@@ -296,8 +297,8 @@ def main() -> int:
     ap.add_argument("--sur-per-hull", type=int, default=8192)
     ap.add_argument("--sur-steps", type=int, default=31)
     ap.add_argument("--sur-holdout", type=int, default=3)
-    ap.add_argument("--sur-train-steps", type=int, default=20000)
-    ap.add_argument("--sur-lr", type=float, default=3e-3, help="peak Adam learning rate (cosine schedule)")
+    ap.add_argument("--sur-train-steps", type=int, default=40000)
+    ap.add_argument("--sur-lr", type=float, default=5e-3, help="peak Adam learning rate (cosine schedule)")
     ap.add_argument("--sur-batch", type=int, default=16384)
     ap.add_argument("--rollout-bodies", type=int, default=16384)
     ap.add_argument("--rollout-steps", type=int, default=300)

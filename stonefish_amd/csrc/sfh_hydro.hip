@@ -116,6 +116,7 @@ struct LayerK {
     double fu, fv;      // fractional texel coordinate of the mesh origin
     double ru[3], rv[3];// rows 0/1 of R scaled by n / length: texels per mesh unit
     int ku, kv;         // integer texel coordinate of the mesh origin, mod n
+    float fu32, fv32, ru32[3], rv32[3];   // the same in FP32 (depth32)
 };
 
 struct BodyDyn {
@@ -165,6 +166,12 @@ __device__ __forceinline__ int wrap_n(int i, int n)
     return min(max(i, 0), n - 1);
 }
 
+// Bilinear mix of a texel's four corner heights at fractions (a, b).
+__device__ __forceinline__ double texel_mix(double a, double b, float4 t)
+{
+    return (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
+}
+
 // Depth of the mesh-frame point (x, y, z), FP64: z - H(x, y) with
 // H = -sum of layer heights (Ocean::GetDepth, OpenGLRealOcean.cpp:280-331).
 // The texel coordinate is u = ku + fu + ru.p, the integer part of the body
@@ -198,6 +205,34 @@ __device__ __forceinline__ float depth_vertex(const BodyDyn& D, const OceanK& oc
     return (float)depth64(D, oc, p.x, p.y, p.z);
 }
 
+// depth64 with the texel coordinates in FP32, for the wet centroids of the
+// pressure buoyancy (SolidEntity.cpp:1650-1658).  The coordinates are
+// relative to the body's texel (ku + fu), so FP32 places the centroid to
+// ~1e-7 texel, a height error of ~1e-8 m; the world z and the bilinear blend
+// stay FP64 (in FP32 their rounding of metre-sized heights, ~1e-7 m per
+// face, does not average out over the few wet faces of a small mesh).  The
+// vertex depths that decide and place the waterline clip are depth64.
+__device__ __forceinline__ double depth_centroid(const BodyDyn& D, const OceanK& oc, f3 p)
+{
+    double d = D.tz + D.rz[0] * (double)p.x + D.rz[1] * (double)p.y + D.rz[2] * (double)p.z;
+    for (int l = 0; l < oc.nlayers; ++l) {
+        const LayerK& L = D.layer[l];
+        const float u = L.fu32 + L.ru32[0] * p.x + L.ru32[1] * p.y + L.ru32[2] * p.z;
+        const float v = L.fv32 + L.rv32[0] * p.x + L.rv32[1] * p.y + L.rv32[2] * p.z;
+        const float fi = floorf(u), fj = floorf(v);
+        int i, j;
+        if (oc.nmask >= 0) {
+            i = (L.ku + (int)fi) & oc.nmask;
+            j = (L.kv + (int)fj) & oc.nmask;
+        } else {
+            i = wrap_n(L.ku + (int)fi, oc.n);
+            j = wrap_n(L.kv + (int)fj, oc.n);
+        }
+        d += texel_mix((double)(u - fi), (double)(v - fj), oc.quad[l][(size_t)j * oc.n + i]);
+    }
+    return d;
+}
+
 // floor(u) mod n in [0, n) for a texel coordinate floor fu.
 __device__ __forceinline__ int wrap_texel(double fu, const OceanK& oc)
 {
@@ -205,12 +240,6 @@ __device__ __forceinline__ int wrap_texel(double fu, const OceanK& oc)
     long long k = (long long)fu % oc.n;
     if (k < 0) k += oc.n;
     return (int)k;
-}
-
-// Bilinear mix of a texel's four corner heights at fractions (a, b).
-__device__ __forceinline__ double texel_mix(double a, double b, float4 t)
-{
-    return (1.0 - a) * (1.0 - b) * t.x + a * (1.0 - b) * t.y + (1.0 - a) * b * t.z + a * b * t.w;
 }
 
 // FP64 wave height sum at world (x, y), for the classification corners.
@@ -406,6 +435,7 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     D.fb.o = to_f3(y);
     D.rz[0] = R[6]; D.rz[1] = R[7]; D.rz[2] = R[8];
     D.tz = tc[2];
+
     for (int l = 0; l < kMaxLayers; ++l) {
         LayerK L;
         L.fu = L.fv = 0.0; L.ku = L.kv = 0;
@@ -420,6 +450,9 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
             L.kv = wrap_texel(fv, oc);
             for (int k = 0; k < 3; ++k) { L.ru[k] = R[k] * s; L.rv[k] = R[3 + k] * s; }
         }
+        L.fu32 = (float)L.fu;
+        L.fv32 = (float)L.fv;
+        for (int k = 0; k < 3; ++k) { L.ru32[k] = (float)L.ru[k]; L.rv32[k] = (float)L.rv[k]; }
         D.layer[l] = L;
     }
     D.cur.n = oc.ncur;
@@ -845,8 +878,7 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
     if (!wt.ok) return;
     f3 cpt = wt.base + wt.off;
     if (wave_buoy) {
-        float dc = (float)depth64(D, oc, (double)wt.base.x + (double)wt.off.x, (double)wt.base.y + (double)wt.off.y,
-                                  (double)wt.base.z + (double)wt.off.z);
+        const float dc = (float)depth_centroid(D, oc, cpt);
         f3 fbi = wt.n1 * (-wt.A * dc);
         add3(acc.Fb, fbi);
         add3(acc.Mb, cross(cpt, fbi));
