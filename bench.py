@@ -292,6 +292,7 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate and rollout sections")
     ap.add_argument("--rollout-only", action="store_true", help="run only the mixed rollout section (profiling)")
+    ap.add_argument("--surrogate-only", action="store_true", help="run only the surrogate section (profiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB device copy (reference dispatch for HBM byte counters)")
     ap.add_argument("--sur-per-hull", type=int, default=8192)
@@ -325,6 +326,9 @@ def main() -> int:
         del a, b
     if args.rollout_only:
         print(json.dumps({"rollout": rollout_section(args, WL.fleet(1, seed=args.seed).sea)}), flush=True)
+        return 0
+    if args.surrogate_only:
+        print(json.dumps({"surrogate": surrogate_section(args)}), flush=True)
         return 0
     w = build_fleet(args.bodies, args.seed)
     nf = np.array([b.mesh.nf for b in w.bodies], dtype=np.int64)
