@@ -247,17 +247,18 @@ int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_
 
 /* ---- surrogate glue (no counterpart in the reference) -------------------- */
 
-/* The learned surrogate (sfmlp.h) sees 13 features of a body's state:
- *   0-2   linear velocity relative to the current at the CG, CG frame (R^T (v - u))
- *   3-5   angular velocity, CG frame (R^T omega)
- *   6-9   orientation of the CG frame, unit quaternion (x, y, z, w), w >= 0
- *   10    depth of the centre of buoyancy below the local surface
- *   11-12 slope of the local surface's depth offset along world x and y
+/* The learned surrogate (sfmlp.h) sees 13 features of a body's state, all in
+ * the CG frame:
+ *   0-2   linear velocity relative to the current at the CG (R^T (v - u))
+ *   3-5   angular velocity (R^T omega)
+ *   6-8   direction of gravity (R^T g / |g|)
+ *   9     depth of the centre of buoyancy below the local surface
+ *   10-12 the local surface's unit normal, towards increasing depth
  * where the local surface is the plane fitted to the wave field (the heights
  * of Ocean::GetDepth) over the body's horizontal footprint: a 3 x 3 stencil
  * at offsets of the world AABB's half extents (at least one texel of the
  * first layer), weights (1/4, 1/2, 1/4) per axis; a flat sea gives the
- * CoB's depth and zero slopes.
+ * CoB's depth and the normal of gravity's direction.
  * and predicts the total force and torque about the CG, CG frame,
  *   F = R^T (Fb + Fdq + Fdf), T = R^T (Tb + Tdq + Tdf) after CorrectHydrodynamicForces,
  * relative to a hydrostatic estimate (Fb0, Tb0: the analytic buoyancy of the

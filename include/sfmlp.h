@@ -3,8 +3,8 @@
  *
  * No counterpart in the reference (Stonefish computes hydrodynamics only with
  * the per-face loops).  A small MLP per hull shape maps body-frame features of
- * a body's state (relative linear and angular velocity, orientation
- * quaternion, depth of the centre below the local surface, local wave slope)
+ * a body's state (relative linear and angular velocity, direction of gravity,
+ * depth of the centre below the local surface, normal of the local surface)
  * to the body-frame force and torque that SolidEntity::ComputeHydrodynamicForces
  * produces (sum of buoyancy, quadratic drag and skin friction after
  * CorrectHydrodynamicForces).  The exact kernels of sfhydro.h generate its

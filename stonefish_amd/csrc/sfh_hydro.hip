@@ -2308,34 +2308,16 @@ __device__ __forceinline__ void exact_totals(const double* e, double* F, double*
     }
 }
 
-// Unit quaternion (x, y, z, w) of a rotation matrix (row-major), w >= 0.
-// Unit quaternion (x, y, z, w), w >= 0, of a rotation matrix (row-major):
-// the largest of the four diagonal combinations picks the component computed
-// from a square root, the other three follow from the off-diagonal sums and
-// differences.  FP32 and branch-free (selects), for the network's features.
-__device__ void rot_to_quat(const double* Rd, double* q)
-{
-    float R[9];
-    for (int k = 0; k < 9; ++k) R[k] = (float)Rd[k];
-    const float tr = R[0] + R[4] + R[8];
-    const int c = tr > 0.f ? 0 : ((R[0] > R[4] && R[0] > R[8]) ? 1 : (R[4] > R[8] ? 2 : 3));
-    const float dg = c == 0 ? tr : (c == 1 ? R[0] - R[4] - R[8] : (c == 2 ? R[4] - R[0] - R[8] : R[8] - R[0] - R[4]));
-    const float sq = 2.f * sqrtf(1.f + dg);
-    const float inv = 1.f / sq;
-    const float big = 0.25f * sq;
-    const float A = (R[7] - R[5]) * inv, B = (R[2] - R[6]) * inv, Cc = (R[3] - R[1]) * inv;
-    const float D = (R[1] + R[3]) * inv, E = (R[2] + R[6]) * inv, F = (R[5] + R[7]) * inv;
-    const float w = c == 0 ? big : (c == 1 ? A : (c == 2 ? B : Cc));
-    const float x = c == 0 ? A : (c == 1 ? big : (c == 2 ? D : E));
-    const float y = c == 0 ? B : (c == 1 ? D : (c == 2 ? big : F));
-    const float z = c == 0 ? Cc : (c == 1 ? E : (c == 2 ? F : big));
-    const float n = (w < 0.f ? -1.f : 1.f) / sqrtf(x * x + y * y + z * z + w * w);
-    q[0] = x * n; q[1] = y * n; q[2] = z * n; q[3] = w * n;
-}
-
 // Everything the surrogate derives from one body's state: the rotation, the
 // 13 features, the body/fluid position, and the physical baseline and scales
-// the network's outputs are relative to.  The network predicts
+// the network's outputs are relative to.  Features, all in the body (CG)
+// frame: relative linear velocity R^T (v - u) (0-2), angular velocity (3-5),
+// the direction of gravity (6-8), the depth of the centre of buoyancy below
+// the local surface (9) and that surface's unit normal (10-12; the plane
+// fitted to the wave heights over the body's footprint).  The forces depend
+// on the orientation only through these directions, so the network does not
+// have to learn that a yaw about gravity changes nothing on a flat sea.  The
+// network predicts
 //   y_F = (F - Fb0) / sF,  y_T = (T - Tb0) / sT   (CG frame)
 // where Fb0, Tb0 is a hydrostatic estimate (the analytic buoyancy inside the
 // water; when crossing, the hull's hydrostatic table, or without one the
@@ -2401,7 +2383,13 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
     double* f = C.f;
     mat_t_vec(C.R, vr, f);
     mat_t_vec(C.R, s + 19, f + 3);
-    rot_to_quat(C.R, f + 6);
+    {   // the direction of gravity in the body frame
+        const double gnn = sqrt(oc.g[0] * oc.g[0] + oc.g[1] * oc.g[1] + oc.g[2] * oc.g[2]);
+        double gh[3] = {0.0, 0.0, 1.0};
+        if (gnn > 0.0)
+            for (int k = 0; k < 3; ++k) gh[k] = oc.g[k] / gnn;
+        mat_t_vec(C.R, gh, f + 6);
+    }
     double mn[3], mx[3], dd[3], cb[3];
     world_aabb(C.R, C.t, U.aabb_c, U.aabb_h, mn, mx, dd);
     mat_vec(C.R, U.P_CB, cb);
@@ -2457,15 +2445,20 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
     }
     under = gsum<G>(under);
     C.pos = U.hydro ? (under == 0 ? 0 : (under == 8 ? 1 : 2)) : -1;
-    f[10] = cb[2];
-    f[11] = f[12] = 0.0;
+    double dep = cb[2], gx = 0.0, gy = 0.0;
     if (oc.nlayers) {
         hm = gsum<G>(hm);
         sx = gsum<G>(sx);
         sy = gsum<G>(sy);
-        f[10] += hm;
-        f[11] = (double)((float)sx * (2.f / (float)ex));   // sum of w (a-1)^2 = 1/2
-        f[12] = (double)((float)sy * (2.f / (float)ey));
+        dep += hm;
+        gx = (double)((float)sx * (2.f / (float)ex));   // sum of w (a-1)^2 = 1/2
+        gy = (double)((float)sy * (2.f / (float)ey));
+    }
+    f[9] = dep;
+    {   // the local plane's unit normal (towards increasing depth) in the body frame
+        const double in = 1.0 / sqrt(1.0 + gx * gx + gy * gy);
+        const double nw[3] = {gx * in, gy * in, in};
+        mat_t_vec(C.R, nw, f + 10);
     }
     // hydrostatic estimate
     const float Lc = (float)fmax(fmax(U.aabb_h[0], U.aabb_h[1]), U.aabb_h[2]);
@@ -2483,7 +2476,7 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
     float frac = C.pos == 1 ? 1.f : 0.f;
     if (C.pos == 2 && U.buoyant && !(U.hst && gn > 0.f)) {
         const float hz = gn > 0.f ? (fabsf(gb[0]) * ah0 + fabsf(gb[1]) * ah1 + fabsf(gb[2]) * ah2) / gn : 0.f;
-        frac = hz > 0.f ? fminf(1.f, fmaxf(0.f, ((float)f[10] + hz) / (2.f * hz))) : 0.f;
+        frac = hz > 0.f ? fminf(1.f, fmaxf(0.f, ((float)dep + hz) / (2.f * hz))) : 0.f;
     }
     if (U.buoyant && C.pos == 2 && U.hst && gn > 0.f) {
         // the hull's table: static buoyancy on a flat surface against the
@@ -2493,7 +2486,7 @@ __device__ void sur_context_g(const UnitStatic& U, const double* __restrict__ s,
         const float hz = fabsf(gh[0]) * ah0 + fabsf(gh[1]) * ah1 + fabsf(gh[2]) * ah2;
         float sv = 0.f;
         for (int k = 0; k < 3; ++k) sv += gh[k] * (float)(U.aabb_c[k] - U.P_CB[k]);
-        sv = hz > 0.f ? ((float)f[10] + sv) / hz : 0.f;
+        sv = hz > 0.f ? ((float)dep + sv) / hz : 0.f;
         float tab[6];
         hst_lookup(U, sv, gh, tab);
         const float sb = rho * gn * vol;

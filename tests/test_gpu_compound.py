@@ -124,4 +124,4 @@ def test_compound_masked_and_features(gpu):
         Hs = np.array([[-O.load().orc_wave_height_64(ctypes.byref(oo.s), cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey)
                         for c in range(3)] for a in range(3)])
         d = cb[2] + (wts * Hs).sum()
-        assert abs(r[i, 10] - d) <= 2e-6 * (1 + abs(d))
+        assert abs(r[i, 9] - d) <= 2e-6 * (1 + abs(d))

@@ -210,8 +210,7 @@ def test_features_match_oracle(gpu):
         exp = np.zeros(13)
         exp[0:3] = Rm.T @ (s[16:19] - u)
         exp[3:6] = Rm.T @ s[19:22]
-        q = _quat(Rm)
-        exp[6:10] = q
+        exp[6:9] = Rm.T @ [0.0, 0.0, 1.0]          # gravity (NED, +z) in the body frame
         cb = Rm @ np.array([0.05, -0.02, 0.01]) + t
         # local surface: plane fitted over the footprint of the body's world AABB
         hx = lambda x, y: -lib.orc_wave_height_64(C_ocean(ocn), x, y)   # noqa: E731
@@ -219,9 +218,11 @@ def test_features_match_oracle(gpu):
         ex, ey = max(np.abs(Rm[0]) @ half, 1.0), max(np.abs(Rm[1]) @ half, 1.0)
         wts = np.outer([0.25, 0.5, 0.25], [0.25, 0.5, 0.25])
         H = np.array([[hx(cb[0] + (a - 1) * ex, cb[1] + (c - 1) * ey) for c in range(3)] for a in range(3)])
-        exp[10] = cb[2] + (wts * H).sum()
-        exp[11] = (wts * H * (np.arange(3) - 1)[:, None]).sum() / (0.5 * ex)
-        exp[12] = (wts * H * (np.arange(3) - 1)[None, :]).sum() / (0.5 * ey)
+        exp[9] = cb[2] + (wts * H).sum()
+        gx = (wts * H * (np.arange(3) - 1)[:, None]).sum() / (0.5 * ex)
+        gy = (wts * H * (np.arange(3) - 1)[None, :]).sum() / (0.5 * ey)
+        nw = np.array([gx, gy, 1.0]) / np.sqrt(1 + gx * gx + gy * gy)   # grad(depth) direction
+        exp[10:13] = Rm.T @ nw
         assert np.allclose(raw[i], exp, rtol=2e-6, atol=2e-6), (i, raw[i], exp)
     # the targets are checked by their round trip through the decode (test_masked_compute_and_apply)
 
@@ -229,18 +230,6 @@ def test_features_match_oracle(gpu):
 def C_ocean(o):
     import ctypes as C
     return C.byref(o.s)
-
-
-def _quat(Rm):
-    # reference quaternion from the rotation matrix via its eigenvector (w >= 0)
-    K = np.array([
-        [Rm[0, 0] - Rm[1, 1] - Rm[2, 2], Rm[1, 0] + Rm[0, 1], Rm[2, 0] + Rm[0, 2], Rm[2, 1] - Rm[1, 2]],
-        [Rm[1, 0] + Rm[0, 1], Rm[1, 1] - Rm[0, 0] - Rm[2, 2], Rm[2, 1] + Rm[1, 2], Rm[0, 2] - Rm[2, 0]],
-        [Rm[2, 0] + Rm[0, 2], Rm[2, 1] + Rm[1, 2], Rm[2, 2] - Rm[0, 0] - Rm[1, 1], Rm[1, 0] - Rm[0, 1]],
-        [Rm[2, 1] - Rm[1, 2], Rm[0, 2] - Rm[2, 0], Rm[1, 0] - Rm[0, 1], Rm[0, 0] + Rm[1, 1] + Rm[2, 2]]]) / 3.0
-    w, v = np.linalg.eigh(K)
-    q = v[:, np.argmax(w)]   # (x, y, z, w)
-    return q if q[3] >= 0 else -q
 
 
 def test_normalised_inputs_and_ood(gpu):
