@@ -414,11 +414,13 @@ typedef struct {
 /* Advances count body states (d_states, count x SFH_STATE_DOUBLES, device) by
  * dt under their world-frame total force and torque d_ft (count x 6) plus
  * gravity (the ocean's): semi-implicit Euler of the Newton-Euler equations,
- * orientation by the exact rotation of omega dt.  The reference integrates
- * bodies with Bullet (btRigidBody), outside this path; this step drives the
- * mixed exact/surrogate rollout. */
-int sfh_rollout_integrate(sfh_ocean ocean, double* d_states, uint32_t count, const double* d_ft,
-                          const sfh_rigid* rigid, double dt, void* stream);
+ * orientation by the exact rotation of omega dt.  With d_ood (optional,
+ * needs d_exact: n x SFH_OUT_DOUBLES exact outputs) the exact totals first
+ * replace d_ft for the flagged bodies, as sfh_surrogate_merge does, in the
+ * same launch.  The reference integrates bodies with Bullet (btRigidBody),
+ * outside this path; this step drives the mixed exact/surrogate rollout. */
+int sfh_rollout_integrate(sfh_ocean ocean, double* d_states, uint32_t count, double* d_ft, const uint8_t* d_ood,
+                          const double* d_exact, const sfh_rigid* rigid, double dt, void* stream);
 
 #ifdef __cplusplus
 }

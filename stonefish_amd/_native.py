@@ -157,7 +157,8 @@ _PROTOS = {
     "sfh_compute_fluid_dynamics_approx": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_double),
                                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
                                                     C.POINTER(sfh_fluid_approx)]),
-    "sfh_rollout_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(sfh_rigid),
+    "sfh_rollout_integrate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.POINTER(sfh_rigid),
                                         C.c_double, C.c_void_p]),
     "sfh_surrogate_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.POINTER(sfh_surrogate_norm)] + [C.c_void_p] * 7),

@@ -522,18 +522,17 @@ struct Chunk {
 };
 
 // Per-step work lists.  k_prologue writes every body's position to bpos; a
-// three-phase prefix sum in body order turns the positions into each body's
+// two-pass prefix sum in body order turns the positions into each body's
 // offsets in the three lists (chunks of inside bodies, chunks of crossing
 // bodies, vertex chunks of crossing bodies): k_scan_part sums blocks of
-// kScanBlock bodies, k_scan_top scans the block sums (one workgroup),
-// k_scan_off scans within each block; k_scatter (one wavefront per body,
+// kScanBlock bodies, k_scan_off adds the sums of the blocks before its own
+// and scans within the block; k_scatter (one wavefront per body,
 // grid-strided) writes the chunk ids.  The passes then take list entries
 // w, w + W, w + 2W, ... (W wavefronts), so every wavefront gets the same
 // number of chunks of its kind.
 static constexpr int kScanThreads = 256;
 static constexpr int kScanPer = 8;                              // consecutive bodies per thread
 static constexpr uint32_t kScanBlock = kScanThreads * kScanPer; // bodies per workgroup
-static constexpr int kTopThreads = 1024;
 
 struct BodyChunks {
     uint32_t chunk0, nchunks, vchunk0, nvchunks;
@@ -604,33 +603,30 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_part(const int8_t* __rest
     }
 }
 
-// One workgroup: exclusive scan of the block sums (in place) and the totals.
-__global__ __launch_bounds__(kTopThreads) void k_scan_top(uint4* __restrict__ part, uint32_t nblocks,
-                                                          uint32_t* __restrict__ counts)
-{
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (nblocks + kTopThreads - 1) / kTopThreads;
-    const uint32_t i0 = min(t * per, nblocks), i1 = min(i0 + per, nblocks);
-    uint32_t v[3] = {0, 0, 0};
-    for (uint32_t i = i0; i < i1; ++i) {
-        const uint4 p = part[i];
-        v[0] += p.x; v[1] += p.y; v[2] += p.z;
-    }
-    uint32_t tot[3] = {0, 0, 0};
-    block_scan3<kTopThreads>(v, tot);
-    for (uint32_t i = i0; i < i1; ++i) {
-        const uint4 p = part[i];
-        part[i] = make_uint4(v[0], v[1], v[2], 0u);
-        v[0] += p.x; v[1] += p.y; v[2] += p.z;
-    }
-    if (t == 0) { counts[0] = tot[0]; counts[1] = tot[1]; counts[2] = tot[2]; }
-}
-
+// The block's base offset is the sum of the block sums before it (each
+// workgroup adds them itself: a few hundred at most, so no third launch is
+// needed for a scan of the block sums); the last block writes the totals.
 __global__ __launch_bounds__(kScanThreads) void k_scan_off(const int8_t* __restrict__ bpos,
                                                            const BodyChunks* __restrict__ bch, uint32_t n,
-                                                           const uint4* __restrict__ part, uint4* __restrict__ boff)
+                                                           const uint4* __restrict__ part, uint4* __restrict__ boff,
+                                                           uint32_t* __restrict__ counts)
 {
+    __shared__ uint32_t red[3][kScanThreads / 64];
     const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    uint32_t pre[3] = {0, 0, 0};
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) {
+        const uint4 p = part[j];
+        pre[0] += p.x; pre[1] += p.y; pre[2] += p.z;
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int k = 0; k < 3; ++k) pre[k] += __shfl_xor(pre[k], off, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) red[k][threadIdx.x >> 6] = pre[k];
+    __syncthreads();
+    uint4 base = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = 0; i < kScanThreads / 64; ++i) {
+        base.x += red[0][i]; base.y += red[1][i]; base.z += red[2][i];
+    }
     uint32_t c[3] = {0, 0, 0};
     for (int j = 0; j < kScanPer; ++j) {
         const uint32_t b = b0 + j;
@@ -642,7 +638,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_off(const int8_t* __restr
     }
     uint32_t tot[3];
     block_scan3<kScanThreads>(c, tot);
-    const uint4 base = part[blockIdx.x];
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        counts[0] = base.x + tot[0];
+        counts[1] = base.y + tot[1];
+        counts[2] = base.z + tot[2];
+    }
     uint32_t o[3] = {base.x + c[0], base.y + c[1], base.z + c[2]};
     for (int j = 0; j < kScanPer; ++j) {
         const uint32_t b = b0 + j;
@@ -1917,9 +1917,8 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
     uint32_t* counts = list_vert + b->nvchunks;
     const uint32_t nblk = std::max<uint32_t>(1, (b->n + kScanBlock - 1) / kScanBlock);
     hipLaunchKernelGGL(k_scan_part, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kTopThreads), 0, s, b->d_scan, nblk, counts);
     hipLaunchKernelGGL(k_scan_off, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan,
-                       b->d_boff);
+                       b->d_boff, counts);
     const uint32_t swaves = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
     hipLaunchKernelGGL(k_scatter, dim3((swaves + 3) / 4), dim3(256), 0, s, b->d_bpos, b->d_bch, b->d_boff, b->n,
                        list_sub, list_cross, list_vert);
@@ -2705,13 +2704,19 @@ extern "C" int sfh_surrogate_apply(sfh_batch b, sfh_ocean o, const double* d_sta
 // velocity from F + m g, angular velocity in the CG frame from
 // T - w x (I w) with principal moments I, then position from the new
 // velocity and orientation by the exact rotation of w dt (Rodrigues).
-__global__ void k_rollout(double* __restrict__ states, uint32_t count, const double* __restrict__ ft, double mass,
+__global__ void k_rollout(double* __restrict__ states, uint32_t count, double* __restrict__ ft,
+                          const uint8_t* __restrict__ ood, const double* __restrict__ exact, double mass,
                           double I0, double I1, double I2, double g0, double g1, double g2, double dt)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     double* s = states + (size_t)i * SFH_STATE_DOUBLES;
-    const double* f = ft + (size_t)i * 6;
+    double* f = ft + (size_t)i * 6;
+    if (ood && ood[i]) {   // the exact totals of a flagged body (merged here rather than in a launch of its own)
+        double Fe[3], Te[3];
+        exact_totals(exact + (size_t)i * SFH_OUT_DOUBLES, Fe, Te);
+        for (int k = 0; k < 3; ++k) { f[k] = Fe[k]; f[3 + k] = Te[k]; }
+    }
     double R[9];
     gl_basis(s, R);
     const double g[3] = {g0, g1, g2};
@@ -2742,15 +2747,17 @@ __global__ void k_rollout(double* __restrict__ states, uint32_t count, const dou
         for (int r = 0; r < 3; ++r) s[4 * c + r] = Rn[3 * r + c];
 }
 
-extern "C" int sfh_rollout_integrate(sfh_ocean o, double* d_states, uint32_t count, const double* d_ft,
-                                     const sfh_rigid* rigid, double dt, void* stream)
+extern "C" int sfh_rollout_integrate(sfh_ocean o, double* d_states, uint32_t count, double* d_ft,
+                                     const uint8_t* d_ood, const double* d_exact, const sfh_rigid* rigid, double dt,
+                                     void* stream)
 {
     if (!o || !d_states || !d_ft || !rigid) return fail(SFH_ERR_INVALID, "null argument");
+    if (d_ood && !d_exact) return fail(SFH_ERR_INVALID, "d_ood needs d_exact");
     if (!(rigid->mass > 0.0) || !(rigid->inertia[0] > 0.0) || !(rigid->inertia[1] > 0.0) || !(rigid->inertia[2] > 0.0))
         return fail(SFH_ERR_INVALID, "mass and inertia must be positive");
     if (count == 0) return SFH_OK;
     hipLaunchKernelGGL(k_rollout, dim3((count + 127) / 128), dim3(128), 0, (hipStream_t)stream, d_states, count, d_ft,
-                       rigid->mass, rigid->inertia[0], rigid->inertia[1], rigid->inertia[2], o->g[0], o->g[1], o->g[2],
+                       d_ood, d_exact, rigid->mass, rigid->inertia[0], rigid->inertia[1], rigid->inertia[2], o->g[0], o->g[1], o->g[2],
                        dt);
     HIPCHK(hipGetLastError());
     return SFH_OK;

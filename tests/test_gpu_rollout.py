@@ -114,3 +114,40 @@ def test_mixed_rollout_runs_with_audits(gpu):
     assert np.array_equal(runs[0], runs[1])
     assert not np.array_equal(runs[0], init.cpu().numpy())
     net.close()
+
+
+def test_integrate_takes_the_exact_totals_of_flagged_bodies(gpu):
+    """sfh_rollout_integrate with d_ood / d_exact: the flagged bodies' exact
+    totals (fields with their written bit) replace their forces before the
+    step, the others keep theirs; the result equals merging first."""
+    from stonefish_amd import hydro as H
+    from stonefish_amd import surrogate as S
+    ocean = H.Ocean()
+    n = 64
+    rng = np.random.default_rng(5)
+    st = np.zeros((n, 22))
+    st[:, :16] = T.quat_to_gl(T.random_unit_quaternions(rng, n), rng.uniform(-1, 1, (n, 3)))
+    st[:, 16:22] = rng.uniform(-1, 1, (n, 6))
+    ft = rng.uniform(-100, 100, (n, 6))
+    exact = np.zeros((n, 24))
+    exact[:, :18] = rng.uniform(-100, 100, (n, 18))
+    exact[:, 20] = rng.integers(0, 64, n)        # written bits of Fb Tb Fdq Tdq Fdf Tdf
+    ood = (rng.random(n) < 0.3).astype(np.uint8)
+    merged = ft.copy()
+    for i in np.nonzero(ood)[0]:
+        wm = int(exact[i, 20])
+        F, Tq = np.zeros(3), np.zeros(3)
+        for q in range(3):
+            if wm & (1 << (2 * q)):
+                F += exact[i, 6 * q:6 * q + 3]
+            if wm & (1 << (2 * q + 1)):
+                Tq += exact[i, 6 * q + 3:6 * q + 6]
+        merged[i] = np.concatenate([F, Tq])
+    a, b = torch.from_numpy(st.copy()).cuda(), torch.from_numpy(st.copy()).cuda()
+    fa, fb = torch.from_numpy(ft.copy()).cuda(), torch.from_numpy(merged).cuda()
+    S.integrate(ocean, a, n, fa, 2.0, (1.0, 2.0, 3.0), 1e-3, ood=torch.from_numpy(ood).cuda(),
+                exact=torch.from_numpy(exact).cuda())
+    S.integrate(ocean, b, n, fb, 2.0, (1.0, 2.0, 3.0), 1e-3)
+    torch.cuda.synchronize()
+    assert np.array_equal(fa.cpu().numpy(), merged)
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
