@@ -344,12 +344,6 @@ __device__ int classify64_g(const double* Rcg, const double* tcg, const double* 
     return under == 0 ? 0 : (under == 8 ? 1 : 2);
 }
 
-__device__ __forceinline__ int classify64(const double* Rcg, const double* tcg, const double* ac, const double* ah,
-                                          const OceanK& oc)
-{
-    return classify64_g<1>(Rcg, tcg, ac, ah, oc, 0);
-}
-
 // ---------------------------------------------------------------------------
 // k_prologue: one lane per body
 // ---------------------------------------------------------------------------
@@ -358,16 +352,26 @@ __device__ __forceinline__ int classify64(const double* Rcg, const double* tcg, 
 //         kind 1 = explicit (T_CG[16], T_C[16], v, w).
 // active (optional): bodies with active[b] == 0 get position -2 and are left
 // out of the step entirely.
-__global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
+// kPrologueLanes lanes per body: the eight wave queries of the classification
+// are spread over them (one memory round trip instead of eight in turn, the
+// body's critical path); they compute the rest redundantly and the first
+// writes the results.
+static constexpr int kPrologueLanes = 8;
+
+__global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
                            uint32_t n, OceanK oc, const uint8_t* __restrict__ active, BodyDyn* __restrict__ dyn,
                            int8_t* __restrict__ bpos)
 {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = gt / kPrologueLanes;
+    const int sub = (int)(gt % kPrologueLanes);
+    if (b >= n) return;   // whole groups: blockDim is a multiple of kPrologueLanes
     const BodyStatic& S = st[b];
     if (active && !active[S.sidx]) {
-        dyn[b].pos = -2;
-        bpos[b] = -2;
+        if (sub == 0) {
+            dyn[b].pos = -2;
+            bpos[b] = -2;
+        }
         return;
     }
     BodyDyn D;
@@ -396,7 +400,7 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     else if (flags & BF_MODE_SUBM) pos = 1;
     else if (!(flags & BF_HYDRO)) pos = -1;
     else {
-        pos = classify64(Rcg, tcg, S.aabb_c, S.aabb_h, oc);
+        pos = classify64_g<kPrologueLanes>(Rcg, tcg, S.aabb_c, S.aabb_h, oc, sub);
         // body level: damping always on (Ocean::ApplyFluidForces, Ocean.cpp:248-252)
         flags |= BF_DAMPING;
     }
@@ -416,7 +420,7 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     }
     D.pos = pos;
     D.flags = flags;
-    bpos[b] = (int8_t)pos;
+    if (sub == 0) bpos[b] = (int8_t)pos;
 
     // mesh-frame FP32 constants
     double x[3], y[3], vl[3], wl[3], ul[3], wd[3];
@@ -471,7 +475,7 @@ __global__ void k_prologue(const BodyStatic* __restrict__ st, const double* __re
     }
     for (int i = 0; i < 9; ++i) { D.R[i] = R[i]; D.Rcg[i] = Rcg[i]; }
     for (int i = 0; i < 3; ++i) D.tcg[i] = tcg[i];
-    dyn[b] = D;
+    if (sub == 0) dyn[b] = D;
 }
 
 // Jets and pipes at a mesh-frame point (Jet.cpp:54-75, Pipe.cpp:57-78): the
@@ -1956,7 +1960,7 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
 {
     if (b->n == 0) return SFH_OK;
     OceanK oc = make_oceank(o);
-    uint32_t blocks = (b->n + 127) / 128;
+    const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + 127) / 128);
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
     hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, active,
                        b->d_dyn, b->d_bpos);
@@ -2542,7 +2546,7 @@ __device__ __forceinline__ uint8_t sur_ood(const double* f, int pos, const SurNo
     return pos > 0 ? o : (uint8_t)0;   // out of the water (or not hydrodynamic): the result is known
 }
 
-__global__ void k_sur_features(const UnitStatic* __restrict__ units, const double* __restrict__ states,
+__global__ __launch_bounds__(128) void k_sur_features(const UnitStatic* __restrict__ units, const double* __restrict__ states,
                                uint32_t first, uint32_t count, uint32_t rows, OceanK oc, SurNorm nm,
                                float* __restrict__ raw, uint16_t* __restrict__ x, uint8_t* __restrict__ ood,
                                int8_t* __restrict__ posout, const double* __restrict__ exact,
@@ -2605,7 +2609,7 @@ __device__ __forceinline__ void sur_decode(const SurCtx& C, const SurNorm& nm, c
 
 
 
-__global__ void k_sur_apply(const UnitStatic* __restrict__ units, const double* __restrict__ states, uint32_t first,
+__global__ __launch_bounds__(128) void k_sur_apply(const UnitStatic* __restrict__ units, const double* __restrict__ states, uint32_t first,
                             uint32_t count, OceanK oc, SurNorm nm, const float* __restrict__ y,
                             const uint8_t* __restrict__ ood, int8_t* __restrict__ posout,
                             const double* __restrict__ exact, double* __restrict__ ft, float* __restrict__ err)
