@@ -120,6 +120,23 @@ int sfh_forces_submerged(sfh_mesh mesh, sfh_ocean ocean,
                          const double v[3], const double omega[3],
                          double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3]);
 
+/* Replaces: SolidEntity::ComputeAerodynamicForces(mesh, atm, ...) (SolidEntity.h:180,
+ * SolidEntity.cpp:1865-1918): every face, quadratic drag of the normal relative
+ * velocity (no skin friction), times 0.5 rho_gas.  The atmosphere is an
+ * sfh_ocean made with the gas density, flat, with every wind field added
+ * enabled and sfh_ocean_enable_currents(h, 1): Atmosphere::GetFluidVelocity
+ * (Atmosphere.cpp:146-152) sums all of them.  mesh == NULL zeroes the outputs
+ * as the reference does. */
+int sfh_forces_aero(sfh_mesh mesh, sfh_ocean atmosphere, const double T_CG[16], const double T_C[16],
+                    const double v[3], const double omega[3], double Fda[3], double Tda[3]);
+
+/* Replaces: SolidEntity::CorrectAerodynamicForces (SolidEntity.h:167,
+ * SolidEntity.cpp:1920-1948), host FP64: scales Fda and Tda by
+ * |Cd . safeNormalize(inverse(basis of T_CG2H) Fda)| with Cd of the body's
+ * proxy (sfh_compute_fluid_dynamics_approx: type, params, T_CG2H); 1 for a
+ * sphere. */
+int sfh_correct_aero(int type, const double params[3], const double T_CG2H[16], double Fda[3], double Tda[3]);
+
 /* Replaces: SolidEntity::CorrectHydrodynamicForces (SolidEntity.h:113-114,
  * SolidEntity.cpp:1263-1287).  In/out drag vectors; T_O = getOTransform(). */
 int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],

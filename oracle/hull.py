@@ -13,6 +13,7 @@ Follows, operation by operation:
   ComputeEllipsoidalApprox    SolidEntity.cpp:849-1026
   LambKFactor                 SolidEntity.cpp:1028-1039
   SetHydrodynamicCoefficients SolidEntity.cpp:192-198
+  CorrectAerodynamicForces    SolidEntity.cpp:1920-1948
 Bullet conventions (BT_USE_DOUBLE_PRECISION, BT_EULER_DEFAULT_ZYX):
 btVector3 division multiplies by the reciprocal (btVector3.h), cofactor
 inverse (btMatrix3x3.h:1093-1103), setRotation (btMatrix3x3.h:215-282),
@@ -321,3 +322,20 @@ def fluid_dynamics_approx(V, kind=AUTO, T_CG2C=None, P_CB=(0.0, 0.0, 0.0), T_CG2
     res["T_CG2H"] = [H[0][0], H[1][0], H[2][0], 0.0, H[0][1], H[1][1], H[2][1], 0.0,
                      H[0][2], H[1][2], H[2][2], 0.0, pcb[0], pcb[1], pcb[2], 1.0]
     return res
+
+
+def correct_aero(kind, params, T_CG2H, Fda, Tda):
+    """CorrectAerodynamicForces: (Fda, Tda) scaled by |Cd . safeNormalize(inv(H) Fda)|."""
+    d = _mv(_inverse(_basis(T_CG2H)), [float(x) for x in Fda])
+    l2 = _dot(d, d)
+    eps = 2.220446049250313e-16
+    d = _div(d, math.sqrt(l2)) if l2 >= eps * eps else [1.0, 0.0, 0.0]
+    cor = 1.0
+    if kind == CYLINDER:
+        cor = _dot([0.5, 0.5, 1.0], d)
+    elif kind == ELLIPSOID:
+        Cd = [1.0 / params[0], 1.0 / params[1], 1.0 / params[2]]
+        Cd = _div(Cd, max(max(Cd[0], Cd[1]), Cd[2]))
+        cor = _dot(Cd, d)
+    cor = abs(cor)
+    return [f * cor for f in Fda], [t * cor for t in Tda]

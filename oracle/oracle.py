@@ -105,7 +105,7 @@ def load(path: str | None = None) -> C.CDLL:
         "orc_compound_forces_64": (None, [C.POINTER(orc_body), C.POINTER(orc_part), C.c_uint32,
                                           C.POINTER(orc_ocean), dp, dp, dp, C.POINTER(orc_result)]),
         "orc_aero_64": (None, [C.POINTER(orc_mesh), C.c_double, C.POINTER(orc_current), C.c_uint32, dp, dp, dp,
-                               dp, dp, dp]),
+                               dp, dp, dp, dp]),
         "orc_batch_forces_32": (C.c_double, [C.POINTER(orc_body), C.c_uint32, C.POINTER(orc_ocean), dp, C.c_uint32,
                                              C.c_int, C.POINTER(orc_result)]),
         "orc_batch_forces_64": (C.c_double, [C.POINTER(orc_body), C.c_uint32, C.POINTER(orc_ocean), dp, C.c_uint32,
@@ -266,3 +266,20 @@ def compound_forces(comp: orc_body, parts: list, ocn: Ocean, T_CG, v, w, init=No
     r = result(init)
     lib.orc_compound_forces_64(C.byref(comp), arr, len(parts), C.byref(ocn.s), _d(T_CG), _d(v), _d(w), C.byref(r))
     return (r.vec(), r.extras()) if full else r.vec()
+
+
+def aero(mesh: Mesh | None, gas_density: float, wind, T_CG, T_C, v, w, sums: bool = False):
+    """orc_aero_64 (SolidEntity::ComputeAerodynamicForces, SolidEntity.cpp:1865-1918):
+    (Fda, Tda); wind: [(kind, params)] summed at every face centroid.  With
+    sums, also (S_F, S_T): the sums of the norms of the per-face terms."""
+    cur = (orc_current * max(1, len(wind)))()
+    for i, (kind, params) in enumerate(wind):
+        cur[i].type = kind
+        cur[i].enabled = 1
+        cur[i].p[:] = list(params) + [0.0] * (12 - len(params))
+    F, T, S = (C.c_double * 3)(), (C.c_double * 3)(), (C.c_double * 2)()
+    load().orc_aero_64(C.byref(mesh.s) if mesh is not None else None, float(gas_density), cur, len(wind), _d(T_CG),
+                       _d(T_C), _d(v), _d(w), F, T, S)
+    if sums:
+        return np.array(F[:]), np.array(T[:]), (S[0], S[1])
+    return np.array(F[:]), np.array(T[:])

@@ -329,8 +329,9 @@ void orc_compound_forces_64(const orc_body* comp, const orc_part* parts, uint32_
 
 void orc_aero_64(const orc_mesh* mesh, double gas_density, const orc_current* wind, uint32_t nwind,
                  const double T_CG[16], const double T_C[16],
-                 const double v_[3], const double w_[3], double Fda[3], double Tda[3])
+                 const double v_[3], const double w_[3], double Fda[3], double Tda[3], double S[2])
 {
+    if (S) S[0] = S[1] = 0.0;
     if (mesh == NULL) {
         Fda[0] = Fda[1] = Fda[2] = Tda[0] = Tda[1] = Tda[2] = 0.0;
         return;
@@ -363,8 +364,13 @@ void orc_aero_64(const orc_mesh* mesh, double gas_density, const orc_current* wi
         v3_64 vn = mul_64(fn1, dot_64(vc, fn1));
         if (dot_64(fn1, vn) < (double)-1e-12f) {
             v3_64 q = mul_64(mul_64(vn, sqrt(len2_64(vn))), Ar);
+            v3_64 t = cross_64(r, q);
             F = add_64(F, q);
-            T = add_64(T, cross_64(r, q));
+            T = add_64(T, t);
+            if (S) {
+                S[0] += 0.5 * gas_density * sqrt(len2_64(q));
+                S[1] += 0.5 * gas_density * sqrt(len2_64(t));
+            }
         }
     }
     Fda[0] = 0.5 * gas_density * F.x; Fda[1] = 0.5 * gas_density * F.y; Fda[2] = 0.5 * gas_density * F.z;

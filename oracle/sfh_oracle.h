@@ -181,10 +181,11 @@ void orc_compound_forces_64(const orc_body* comp, const orc_part* parts, uint32_
 
 /* ---- aerodynamics (SolidEntity::ComputeAerodynamicForces static) ------ */
 /* wind: Atmosphere::GetFluidVelocity sums every wind field (no enabled flag);
- * Fda/Tda are the raw integrals times 0.5*rho_gas, before CorrectAerodynamicForces. */
+ * Fda/Tda are the raw integrals times 0.5*rho_gas, before CorrectAerodynamicForces.
+ * S (may be NULL): sums of the norms of the per-face force and torque terms. */
 void orc_aero_64(const orc_mesh* mesh, double gas_density, const orc_current* wind, uint32_t nwind,
                  const double T_CG[16], const double T_C[16],
-                 const double v[3], const double w[3], double Fda[3], double Tda[3]);
+                 const double v[3], const double w[3], double Fda[3], double Tda[3], double S[2]);
 
 /* ---- batched CPU baseline: one task per body on a thread pool -------- */
 /* states: per body 22 doubles: T_CG[16], v[3], w[3]; the C transform is

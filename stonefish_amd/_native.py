@@ -152,6 +152,8 @@ _PROTOS = {
     "sfh_surrogate_plan_destroy": (C.c_int, [C.c_void_p]),
     "sfh_surrogate_run": (C.c_int, [C.c_void_p] * 7),
     "sfh_surrogate_merge": (C.c_int, [C.c_void_p] * 6),
+    "sfh_forces_aero": (C.c_int, [C.c_void_p, C.c_void_p] + [C.POINTER(C.c_double)] * 6),
+    "sfh_correct_aero": (C.c_int, [C.c_int] + [C.POINTER(C.c_double)] * 4),
     "sfh_compute_physical_properties": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_double,
                                                   C.c_double, C.POINTER(sfh_mesh_properties)]),
     "sfh_compute_fluid_dynamics_approx": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_double),

@@ -159,6 +159,36 @@ SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, AccT& acc)
     face_submerged(B, p1, p2, p3, NoFlow(), acc);
 }
 
+// Aerodynamic face (SolidEntity.cpp:1884-1912): quadratic drag of the normal
+// velocity only, vn = (vc.n1) n1, applied when n1.vn < -1e-12:
+// q = vn |vn| A.  With vm = -vc and s = vm.n1: vn = -s n1, the condition is
+// s > 1e-12 and q = n1 (-s |s| A).  Accumulated in the quadratic-drag slot;
+// there is no skin friction.  The wind (the atmosphere's velocity fields) is
+// sampled at the centroid as for the currents.
+template <class Flow, class AccT>
+SFH_HD void face_aero(const FaceBody& B, f3 p1, f3 p2, f3 p3, const Flow& flow, AccT& acc)
+{
+    f3 e1 = p2 - p1, e2 = p3 - p1;
+    f3 n = cross(e1, e2);
+    float n2 = dot(n, n);
+    if (n2 < SFH_DEGENERATE_N2) return;
+    float inv = frsq(n2);
+    f3 n1 = n * inv;
+    float A = 0.5f * (n2 * inv);
+    f3 c = p1 + (e1 + e2) * (1.0f / 3.0f);
+    f3 vm = B.vb + cross(B.w, c);
+    if (Flow::on)
+        vm = vm - flow(((double)p1.x + (double)p2.x + (double)p3.x) / 3.0,
+                       ((double)p1.y + (double)p2.y + (double)p3.y) / 3.0,
+                       ((double)p1.z + (double)p2.z + (double)p3.z) / 3.0);
+    float s = dot(vm, n1);
+    if (s > -SFH_APPROACH_VN) {
+        f3 q = n1 * (-(s * fabsf(s) * A));
+        add3(acc.Fdq, q);
+        add3(acc.Mdq, cross(c, q));
+    }
+}
+
 // Result of clipping one face against the waterline: the wet polygon as a
 // base vertex plus an offset to its centroid, its unit normal and area, and
 // whether the face contributes.

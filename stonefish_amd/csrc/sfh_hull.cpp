@@ -1,7 +1,9 @@
-// Hull preprocessing on the host (once per hull): the mesh's physical
-// properties (ComputePhysicalProperties, GeometryFileUtil.cpp:335-563) and the
-// fluid-dynamics proxy with its drag coefficients
-// (SolidEntity::ComputeFluidDynamicsApprox, SolidEntity.cpp:698-1039).
+// Host-side per-body work: the hull preprocessing (once per hull) — the mesh's
+// physical properties (ComputePhysicalProperties, GeometryFileUtil.cpp:
+// 335-563) and the fluid-dynamics proxy with its drag coefficients
+// (SolidEntity::ComputeFluidDynamicsApprox, SolidEntity.cpp:698-1039) — and
+// the proxy's aerodynamic correction (SolidEntity::CorrectAerodynamicForces,
+// SolidEntity.cpp:1920-1948).
 //
 // Everything is FP64 with the reference's operation order, so the results
 // follow the reference's Scalar = double arithmetic: Bullet's conventions are
@@ -319,6 +321,32 @@ void ellipsoid_approx(const std::vector<V3>& x, V3 P_CB, const M3& B_CG2O, doubl
 }
 
 }  // namespace
+
+extern "C" int sfh_correct_aero(int type, const double params[3], const double T_CG2H[16], double Fda[3],
+                                double Tda[3])
+{
+    if (!T_CG2H || !Fda || !Tda || (type >= SFH_APPROX_CYLINDER && !params))
+        return sfh_internal_fail(SFH_ERR_INVALID, "null argument");
+    V3 F = {Fda[0], Fda[1], Fda[2]};
+    V3 d = basis_gl(T_CG2H).inverse() * F;
+    // btVector3::safeNormalize (btVector3.h:286-299)
+    const double l2 = len2(d);
+    d = l2 >= 2.220446049250313e-16 * 2.220446049250313e-16 ? d / std::sqrt(l2) : V3{1.0, 0.0, 0.0};
+    double cor = 1.0;
+    if (type == SFH_APPROX_CYLINDER) {
+        cor = dot(V3{0.5, 0.5, 1.0}, d);
+    } else if (type == SFH_APPROX_ELLIPSOID) {
+        V3 Cd = {1.0 / params[0], 1.0 / params[1], 1.0 / params[2]};
+        Cd = Cd / std::max(std::max(Cd.x, Cd.y), Cd.z);
+        cor = dot(Cd, d);
+    }
+    cor = std::fabs(cor);
+    for (int k = 0; k < 3; ++k) {
+        Fda[k] *= cor;
+        Tda[k] *= cor;
+    }
+    return SFH_OK;
+}
 
 extern "C" int sfh_compute_physical_properties(const float* xyz, uint32_t nv, const uint32_t* idx, uint32_t nf,
                                                double thickness, double density, sfh_mesh_properties* out)

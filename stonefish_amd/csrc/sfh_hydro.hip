@@ -71,7 +71,8 @@ enum : uint32_t {
     BF_EXPLICIT_TC = 32u,// state carries T_C explicitly (per-face calls)
     BF_PART = 64u,       // part of a compound (Compound::ComputeHydrodynamicForces)
     BF_EXTERNAL = 128u,  // part: CompoundPart::isExternal
-    BF_PHYDRO = 256u     // part: the part solid's mode is FLOATING or SUBMERGED
+    BF_PHYDRO = 256u,    // part: the part solid's mode is FLOATING or SUBMERGED
+    BF_MODE_AERO = 512u  // static aerodynamic call: every face, normal-velocity drag, no Correct
 };
 
 static constexpr int kMaxLayers = 4;
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__
     uint32_t flags = S.flags;
     int pos;
     if (flags & BF_MODE_SURF) pos = 2;
-    else if (flags & BF_MODE_SUBM) pos = 1;
+    else if (flags & (BF_MODE_SUBM | BF_MODE_AERO)) pos = 1;
     else if (!(flags & BF_HYDRO)) pos = -1;
     else {
         pos = classify64_g<kPrologueLanes>(Rcg, tcg, S.aabb_c, S.aabb_h, oc, sub);
@@ -815,7 +816,7 @@ __device__ __forceinline__ int xreduce12_slot(int lane)
 // with FP32 per-lane partial sums over the chunk's K faces per lane, reduced
 // across the wavefront and across chunks in FP64.  `list` holds the ids of the
 // chunks of this step's inside bodies.
-template <int K, bool NU>
+template <int K, bool NU, bool AERO = false>
 __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ count,
                                                    const Chunk* __restrict__ chunks,
@@ -853,10 +854,16 @@ __global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ 
             f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
             uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
             if (64u * j + lane < nfc) {
-                if (NU)
+                if (AERO) {
+                    if (NU)
+                        face_aero(fb, p1, p2, p3, flow, acc);
+                    else
+                        face_aero(fb, p1, p2, p3, NoFlow(), acc);
+                } else if (NU) {
                     face_submerged(fb, p1, p2, p3, flow, acc);
-                else
+                } else {
                     face_submerged(fb, p1, p2, p3, acc);
+                }
             }
             p1 = q1; p2 = q2; p3 = q3;
             b0 = c0; b1 = c1; b2 = c2;
@@ -1060,7 +1067,7 @@ __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* 
 #pragma unroll
     for (int k = 0; k < 23; ++k) P[k] = wave_sum_f64(P[k]);
     if (lane != 0) return;
-    const bool body_level = !(flags & (BF_MODE_SURF | BF_MODE_SUBM));
+    const bool body_level = !(flags & (BF_MODE_SURF | BF_MODE_SUBM | BF_MODE_AERO));
     const bool damping = flags & BF_DAMPING;
     const bool buoyant = flags & BF_BUOYANT;
     uint32_t written = 0;
@@ -1079,6 +1086,15 @@ __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* 
     double* Tdq = drag + 3;
     double* Fdf = drag + 6;
     double* Tdf = drag + 9;
+    if (flags & BF_MODE_AERO) {   // Fda, Tda = 0.5 rho_gas times the sums (SolidEntity.cpp:1915-1917)
+        for (int k = 0; k < 3; ++k) {
+            o[6 + k] = 0.5 * oc.rho * Fdq[k];
+            o[9 + k] = 0.5 * oc.rho * Tdq[k];
+        }
+        o[20] = (double)(4u | 8u);   // written: the quadratic-drag slots
+        o[21] = (double)pos;
+        return;
+    }
     double Fb[3] = {0, 0, 0}, Tb[3] = {0, 0, 0};
     double Swet = 0.0, Vsub = 0.0;
     if (pos == 1) {
@@ -1336,6 +1352,7 @@ struct sfh_batch_t {
     uint32_t n = 0;         // bodies of the face passes (the parts, for a compound batch)
     uint32_t nunits = 0;    // rows of the state and output arrays: bodies, or compounds
     bool compound = false;
+    bool aero = false;      // every body runs the aerodynamic face loop (sfh_forces_aero)
     CompStatic* d_comp = nullptr;
     double* d_part_out = nullptr;
     UnitStatic* d_units = nullptr;  // per unit: P_CB and AABB (CG frame), for the surrogate features
@@ -1938,13 +1955,20 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
                        b->d_chunks, b->d_static, b->d_dyn, b->d_partials);                                    \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_cross<KK, NU>), dim3(blocks), dim3(256), 0, s, list_cross,      \
                        counts + 1, b->d_chunks, b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);
-#define LAUNCH_K(KK)                  \
-    case KK:                          \
-        if (oc.ncur) {                \
-            LAUNCH_KN(KK, true)       \
-        } else {                      \
-            LAUNCH_KN(KK, false)      \
-        }                             \
+#define LAUNCH_K(KK)                                                                                 \
+    case KK:                                                                                         \
+        if (b->aero) {                                                                               \
+            if (oc.ncur)                                                                             \
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, true, true>), dim3(blocks), dim3(256), 0, s, \
+                                   list_sub, counts, b->d_chunks, b->d_static, b->d_dyn, b->d_partials);  \
+            else                                                                                     \
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, false, true>), dim3(blocks), dim3(256), 0, s, \
+                                   list_sub, counts, b->d_chunks, b->d_static, b->d_dyn, b->d_partials);  \
+        } else if (oc.ncur) {                                                                        \
+            LAUNCH_KN(KK, true)                                                                      \
+        } else {                                                                                     \
+            LAUNCH_KN(KK, false)                                                                     \
+        }                                                                                            \
         break;
     switch (b->K) {
         LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16)
@@ -2146,6 +2170,7 @@ static int single_prepare(sfh_mesh_t* m, uint32_t flags)
         b->meshes_by_body.push_back(m);
         b->h_static.push_back(make_static(m, flags));
         b->nfaces = b->h_static[0].nf;
+        b->aero = flags & BF_MODE_AERO;
         b->K = 0;
         int rc = hipMalloc(&b->d_dyn, sizeof(BodyDyn)) == hipSuccess ? SFH_OK : fail(SFH_ERR_HIP, "alloc failed");
         sfh_batch_opts opts = {8, 0};
@@ -2235,6 +2260,27 @@ extern "C" int sfh_forces_submerged(sfh_mesh mesh, sfh_ocean ocean, const double
     memcpy(Tdq, out + 9, 24);
     memcpy(Fdf, out + 12, 24);
     memcpy(Tdf, out + 15, 24);
+    return SFH_OK;
+}
+
+extern "C" int sfh_forces_aero(sfh_mesh mesh, sfh_ocean atmosphere, const double T_CG[16], const double T_C[16],
+                               const double v[3], const double omega[3], double Fda[3], double Tda[3])
+{
+    if (!T_CG || !T_C || !v || !omega || !Fda || !Tda) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(atmosphere);
+    if (rc != SFH_OK) return rc;
+    if (!mesh) {
+        for (int k = 0; k < 3; ++k) Fda[k] = Tda[k] = 0.0;
+        return SFH_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_single_mu);
+    rc = single_prepare(mesh, BF_MODE_AERO | BF_EXPLICIT_TC | BF_DAMPING);
+    if (rc != SFH_OK) return rc;
+    double out[SFH_OUT_DOUBLES];
+    rc = single_run(atmosphere, T_CG, T_C, v, omega, out);
+    if (rc != SFH_OK) return rc;
+    memcpy(Fda, out + 6, 24);
+    memcpy(Tda, out + 9, 24);
     return SFH_OK;
 }
 

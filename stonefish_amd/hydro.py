@@ -149,6 +149,26 @@ class Ocean:
             pass
 
 
+class Atmosphere(Ocean):
+    """Gas and wind fields (the reference's `Atmosphere*`) as seen by the
+    aerodynamic face loop: a flat Ocean with the gas density whose currents are
+    the wind fields, all of them summed (Atmosphere::GetFluidVelocity,
+    Atmosphere.cpp:146-152, checks no enabled flag)."""
+
+    def __init__(self, density: float = 1.225, gravity=(0.0, 0.0, 9.81)):
+        super().__init__(density, gravity)
+        self.enable_currents(True)
+
+    def add_current(self, kind: int, params, enabled: bool = True) -> None:
+        super().add_current(kind, params, True)
+
+    def add_uniform_current(self, v, enabled: bool = True) -> None:
+        super().add_uniform_current(v, True)
+
+    def enable_currents(self, on: bool = True) -> None:
+        super().enable_currents(True)
+
+
 def ComputeHydrodynamicForcesSurface(settings: HydrodynamicsSettings, mesh: Mesh | None, ocn: Ocean,
                                      T_CG, T_C, v, omega, Fb, Tb, Fdq, Tdq, Fdf, Tdf, SwetVsub) -> None:
     """SolidEntity::ComputeHydrodynamicForcesSurface (SolidEntity.cpp:1289-1721)."""
@@ -171,6 +191,22 @@ def ComputeHydrodynamicForcesSubmerged(mesh: Mesh | None, ocn: Ocean, T_CG, T_C,
     T_CG, T_C, v, omega = _vec(T_CG, 16), _vec(T_C, 16), _vec(v), _vec(omega)
     N.check(lib.sfh_forces_submerged(mesh.handle if mesh is not None else None, ocn.handle, _dp(T_CG), _dp(T_C),
                                      _dp(v), _dp(omega), _dp(Fdq), _dp(Tdq), _dp(Fdf), _dp(Tdf)))
+
+
+def ComputeAerodynamicForces(mesh: Mesh | None, atm: Ocean, T_CG, T_C, v, omega, Fda, Tda) -> None:
+    """SolidEntity::ComputeAerodynamicForces(mesh, atm, ...) (SolidEntity.cpp:1865-1918).
+    atm: an Atmosphere (or an Ocean made with the gas density whose enabled
+    currents are the wind fields)."""
+    lib = N.load()
+    T_CG, T_C, v, omega = _vec(T_CG, 16), _vec(T_C, 16), _vec(v), _vec(omega)
+    N.check(lib.sfh_forces_aero(mesh.handle if mesh is not None else None, atm.handle, _dp(T_CG), _dp(T_C), _dp(v),
+                                _dp(omega), _dp(Fda), _dp(Tda)))
+
+
+def CorrectAerodynamicForces(approx_type: int, params, T_CG2H, Fda, Tda) -> None:
+    """SolidEntity::CorrectAerodynamicForces (SolidEntity.cpp:1920-1948), in place."""
+    p = _vec(params)
+    N.check(N.load().sfh_correct_aero(int(approx_type), _dp(p), _dp(_vec(T_CG2H, 16)), _dp(Fda), _dp(Tda)))
 
 
 def CorrectHydrodynamicForces(ocn: Ocean, Fdq, Tdq, Fdf, Tdf, fdCd, fdCf, T_O) -> None:

@@ -191,3 +191,38 @@ def test_argument_checks():
         H.compute_fluid_dynamics_approx(np.zeros((4, 3), np.float32), kind=7)
     with pytest.raises(N.SfhError):
         H.compute_physical_properties(np.zeros((3, 3), np.float32), np.array([[0, 1, 5]]))
+
+
+# ---- CorrectAerodynamicForces ---------------------------------------------------------
+
+def test_correct_aero_closed_forms_and_oracle():
+    """Sphere: unchanged.  Cylinder with the drag along its axis: factor 1,
+    across it: 0.5.  Ellipsoid: |Cd . d| with Cd = (1/a, 1/b, 1/c) / max.
+    The library against the restatement on random frames and vectors."""
+    rng = np.random.default_rng(5)
+    Rq = T.random_unit_quaternions(rng, 1)[0]
+    H_ = T.quat_to_gl(Rq, [0.1, 0.2, 0.3])
+    R = np.array(H_).reshape(4, 4).T[:3, :3]
+    Tda0 = np.array([0.3, -0.2, 0.5])
+    for kind, params, F_local, factor in ((HO.SPHERE, [0.5, 0, 0], [1.0, 2.0, 3.0], 1.0),
+                                          (HO.CYLINDER, [0.3, 2.0, 0], [0, 0, -4.0], 1.0),
+                                          (HO.CYLINDER, [0.3, 2.0, 0], [0, 2.5, 0], 0.5),
+                                          (HO.ELLIPSOID, [1.0, 0.5, 0.25], [0, 0, 3.0], 1.0),
+                                          (HO.ELLIPSOID, [1.0, 0.5, 0.25], [-2.0, 0, 0], 0.25)):
+        F = R @ np.array(F_local, dtype=float)
+        Fd, Td = F.copy(), Tda0.copy()
+        H.CorrectAerodynamicForces(kind, params, H_, Fd, Td)
+        np.testing.assert_allclose(Fd, factor * F, rtol=1e-12)
+        np.testing.assert_allclose(Td, factor * Tda0, rtol=1e-12)
+    for _ in range(20):
+        kind = int(rng.integers(0, 4))
+        params = rng.uniform(0.1, 2.0, 3)
+        Hr = T.quat_to_gl(T.random_unit_quaternions(rng, 1)[0], rng.uniform(-1, 1, 3))
+        F, Tq = rng.normal(size=3), rng.normal(size=3)
+        if _ == 0:
+            F[:] = 0.0      # safeNormalize falls back to +x
+        Fo, To = HO.correct_aero(kind, params, Hr, F, Tq)
+        Fd, Td = F.copy(), Tq.copy()
+        H.CorrectAerodynamicForces(kind, params, Hr, Fd, Td)
+        _close(Fd, Fo, rtol=1e-13)
+        _close(Td, To, rtol=1e-13)

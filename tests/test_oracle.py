@@ -533,3 +533,49 @@ def test_body_in_uniform_pipe_core_equals_moving_body():
     r_move = O.submerged(64, m, O.Ocean(), Tcg, Tcg, -vin * a, [0, 0, 0])
     np.testing.assert_allclose(r_pipe[6:18], r_move[6:18], rtol=1e-12, atol=1e-12)
     assert np.linalg.norm(r_move[6:9]) > 0.1
+
+
+# ---------------------------------------------------------------- aerodynamics
+
+def test_aero_translating_box_closed_form():
+    """A box of half extents (hx, hy, hz) in any orientation translating at u
+    through still air: only the upwind faces (n.u_hat > 0) act, each with
+    q = vn |vn| A, vn = (vc.n) n, vc = -u: F = -0.5 rho |u|^2 sum A_i (n_i.u_hat)^2 n_i
+    and no torque about the centre (SolidEntity.cpp:1865-1918)."""
+    hx, hy, hz = (float(np.float32(x)) for x in (0.7, 0.4, 0.2))
+    m = O.Mesh.from_tri(M.box(hx, hy, hz, 4))
+    rng = np.random.default_rng(12)
+    rho = 1.2
+    for _ in range(4):
+        Tcg = random_pose(rng, depth=-30.0)
+        R = np.array(Tcg).reshape(4, 4).T[:3, :3]
+        u = rng.uniform(-3, 3, 3)
+        F, T_ = O.aero(m, rho, [], Tcg, Tcg, u, [0, 0, 0])
+        uh = u / np.linalg.norm(u)
+        areas = np.array([4 * hy * hz, 4 * hx * hz, 4 * hx * hy])
+        expect = np.zeros(3)
+        for k in range(3):
+            for s in (1.0, -1.0):
+                n = s * R[:, k]
+                c = n @ uh
+                if c > 0:
+                    expect += -0.5 * rho * (u @ u) * areas[k] * c * c * n
+        np.testing.assert_allclose(F, expect, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(T_, 0, atol=1e-9)
+
+
+def test_aero_uniform_wind_equals_moving_body():
+    """A body at rest in a uniform wind w feels what the body moving at -w in
+    still air feels; a spinning sphere about its centre feels ~nothing."""
+    m = O.Mesh.from_tri(M.ico_ellipsoid(1.0, 0.3, 0.25, 3))
+    rng = np.random.default_rng(2)
+    Tcg = random_pose(rng, depth=-10.0)
+    w = np.array([1.5, -0.5, 0.2])
+    F1, T1 = O.aero(m, 1.2, [(0, list(w))], Tcg, Tcg, [0, 0, 0], [0, 0, 0])
+    F2, T2 = O.aero(m, 1.2, [], Tcg, Tcg, -w, [0, 0, 0])
+    np.testing.assert_allclose(F1, F2, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(T1, T2, rtol=1e-12, atol=1e-12)
+    s = O.Mesh.from_tri(M.ico_ellipsoid(1.0, 1.0, 1.0, 4))
+    Fs, Ts = O.aero(s, 1.2, [], Tcg, Tcg, [0, 0, 0], [0.7, -0.3, 0.4])
+    Fr, _ = O.aero(s, 1.2, [], Tcg, Tcg, [0.7, 0, 0], [0, 0, 0])
+    assert np.linalg.norm(Fs) < 1e-3 * np.linalg.norm(Fr)
