@@ -215,6 +215,55 @@ typedef struct {
 int sfh_compound_batch_create(const sfh_compound_desc* comps, uint32_t n, const sfh_batch_opts* opts,
                               sfh_batch* out);
 
+/* ---- aerodynamic batches ------------------------------------------------ */
+
+/* One body of Atmosphere::ApplyFluidForces (Atmosphere.cpp:160-195): what
+ * SolidEntity::ComputeAerodynamicForces(atm) (SolidEntity.cpp:1852-1863) and
+ * CorrectAerodynamicForces (1925-1948) read from the entity. */
+typedef struct {
+    sfh_mesh mesh;            /* getPhysicsMesh() (NULL: zero drag) */
+    double T_CG2C[16];        /* getCTransform() = getCGTransform() * T_CG2C */
+    double T_CG2H[16];        /* the drag proxy's frame (its basis only is used) */
+    double approx_params[3];  /* fdApproxParams */
+    int32_t approx_type;      /* fdApproxType: SFH_APPROX_* */
+    int32_t aero;             /* phy.mode == AERODYNAMIC (else the row is untouched) */
+} sfh_aero_desc;
+
+/* One part of Compound::ComputeAerodynamicForces (Compound.cpp:439-464). */
+typedef struct {
+    sfh_mesh mesh;            /* part solid getPhysicsMesh() */
+    double origin[16];        /* CompoundPart::origin */
+    double O2C[16];           /* part solid getO2CTransform() */
+    double T_CG2H[16];        /* part solid's drag proxy frame */
+    double approx_params[3];  /* part solid fdApproxParams */
+    int32_t approx_type;      /* part solid fdApproxType */
+    int32_t external;         /* CompoundPart::isExternal (internal parts add nothing) */
+} sfh_aero_part_desc;
+
+typedef struct {
+    double T_CG2O[16];        /* the compound's getOTransform() = getCGTransform() * T_CG2O */
+    int32_t aero;             /* the compound's phy.mode == AERODYNAMIC */
+    int32_t pad_;
+    const sfh_aero_part_desc* parts;
+    uint32_t nparts;
+} sfh_aero_compound_desc;
+
+/* Replaces the per-body loop of Atmosphere::ApplyFluidForces (recompute
+ * step) + SolidEntity::ComputeAerodynamicForces(atm): a batch whose compute
+ * runs the aerodynamic face loop for every body with aero set, then
+ * CorrectAerodynamicForces.  Evaluate with sfh_batch_compute* / capture /
+ * replay on an atmosphere (an sfh_ocean as for sfh_forces_aero): each output
+ * row holds Fda in [6..8] and Tda in [9..11], written mask 12 and position 1;
+ * rows of bodies without aero keep their values (mask 0, position -1).  The
+ * surrogate entry points reject these batches. */
+int sfh_aero_batch_create(const sfh_aero_desc* bodies, uint32_t n, const sfh_batch_opts* opts, sfh_batch* out);
+
+/* Compound::ComputeAerodynamicForces for a batch of compounds: every external
+ * part runs the face loop with T_C = T_CG * T_CG2O * origin * O2C and its own
+ * CorrectAerodynamicForces; the compound's row is their sum (rows as above). */
+int sfh_aero_compound_batch_create(const sfh_aero_compound_desc* comps, uint32_t n, const sfh_batch_opts* opts,
+                                   sfh_batch* out);
+
 /* Evaluates every body (classification, per-face loop, FP64 reduction,
  * buoyancy, CorrectHydrodynamicForces).  d_states: n x SFH_STATE_DOUBLES,
  * d_out: n x SFH_OUT_DOUBLES, both device memory; `stream` is a hipStream_t

@@ -63,6 +63,22 @@ class sfh_compound_desc(C.Structure):
     _fields_ = [("body", sfh_body_desc), ("parts", C.POINTER(sfh_part_desc)), ("nparts", C.c_uint32)]
 
 
+class sfh_aero_desc(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("T_CG2C", C.c_double * 16), ("T_CG2H", C.c_double * 16),
+                ("approx_params", C.c_double * 3), ("approx_type", C.c_int32), ("aero", C.c_int32)]
+
+
+class sfh_aero_part_desc(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("origin", C.c_double * 16), ("O2C", C.c_double * 16),
+                ("T_CG2H", C.c_double * 16), ("approx_params", C.c_double * 3), ("approx_type", C.c_int32),
+                ("external", C.c_int32)]
+
+
+class sfh_aero_compound_desc(C.Structure):
+    _fields_ = [("T_CG2O", C.c_double * 16), ("aero", C.c_int32), ("pad_", C.c_int32),
+                ("parts", C.POINTER(sfh_aero_part_desc)), ("nparts", C.c_uint32)]
+
+
 class sfh_batch_opts(C.Structure):
     _fields_ = [("faces_per_lane", C.c_uint32), ("waves", C.c_uint32)]
 
@@ -131,6 +147,10 @@ _PROTOS = {
                                    C.POINTER(C.c_void_p)]),
     "sfh_compound_batch_create": (C.c_int, [C.POINTER(sfh_compound_desc), C.c_uint32, C.POINTER(sfh_batch_opts),
                                             C.POINTER(C.c_void_p)]),
+    "sfh_aero_batch_create": (C.c_int, [C.POINTER(sfh_aero_desc), C.c_uint32, C.POINTER(sfh_batch_opts),
+                                        C.POINTER(C.c_void_p)]),
+    "sfh_aero_compound_batch_create": (C.c_int, [C.POINTER(sfh_aero_compound_desc), C.c_uint32,
+                                                 C.POINTER(sfh_batch_opts), C.POINTER(C.c_void_p)]),
     "sfh_batch_destroy": (C.c_int, [C.c_void_p]),
     "sfh_batch_set_opts": (C.c_int, [C.c_void_p, C.POINTER(sfh_batch_opts)]),
     "sfh_batch_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint32_p, C.POINTER(C.c_uint64)]),

@@ -72,7 +72,9 @@ enum : uint32_t {
     BF_PART = 64u,       // part of a compound (Compound::ComputeHydrodynamicForces)
     BF_EXTERNAL = 128u,  // part: CompoundPart::isExternal
     BF_PHYDRO = 256u,    // part: the part solid's mode is FLOATING or SUBMERGED
-    BF_MODE_AERO = 512u  // static aerodynamic call: every face, normal-velocity drag, no Correct
+    BF_MODE_AERO = 512u, // aerodynamic loop: every face, normal-velocity drag
+    BF_AERO_CORRECT = 1024u,  // body level: CorrectAerodynamicForces with the proxy in bits 11-12
+    BF_APPROX_SHIFT = 11u
 };
 
 static constexpr int kMaxLayers = 4;
@@ -1036,6 +1038,33 @@ __device__ void correct64(double rho, const double* BO, const double* Cd, const 
     }
 }
 
+// CorrectAerodynamicForces (SolidEntity.cpp:1925-1948): d = safeNormalize(
+// inverse(basis of T_CG2H) * Fda); F and T scaled by |Cd . d| with the proxy's
+// Cd: sphere 1, cylinder (0.5, 0.5, 1), ellipsoid (1/a, 1/b, 1/c) / max.
+// Minv: the inverse basis, rows, from the host; params: fdApproxParams.
+__device__ void correct_aero64(uint32_t type, const double* params, const double* Minv, double* F, double* T)
+{
+    double d[3];
+    mat_vec(Minv, F, d);
+    const double l2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (l2 >= 2.220446049250313e-16 * 2.220446049250313e-16) {
+        const double inv = 1.0 / sqrt(l2);
+        d[0] *= inv; d[1] *= inv; d[2] *= inv;
+    } else {
+        d[0] = 1.0; d[1] = 0.0; d[2] = 0.0;
+    }
+    double cor = 1.0;
+    if (type == 2u) {
+        cor = 0.5 * d[0] + 0.5 * d[1] + d[2];
+    } else if (type == 3u) {
+        double c[3] = {1.0 / params[0], 1.0 / params[1], 1.0 / params[2]};
+        const double mx = fmax(fmax(c[0], c[1]), c[2]);
+        cor = c[0] / mx * d[0] + c[1] / mx * d[1] + c[2] / mx * d[2];
+    }
+    cor = fabs(cor);
+    for (int k = 0; k < 3; ++k) { F[k] *= cor; T[k] *= cor; }
+}
+
 __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t b,
                               const OceanK& oc, const double* __restrict__ partials, double* __restrict__ out)
 {
@@ -1087,9 +1116,15 @@ __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* 
     double* Fdf = drag + 6;
     double* Tdf = drag + 9;
     if (flags & BF_MODE_AERO) {   // Fda, Tda = 0.5 rho_gas times the sums (SolidEntity.cpp:1915-1917)
+        double Fa[3], Ta[3];
         for (int k = 0; k < 3; ++k) {
-            o[6 + k] = 0.5 * oc.rho * Fdq[k];
-            o[9 + k] = 0.5 * oc.rho * Tdq[k];
+            Fa[k] = 0.5 * oc.rho * Fdq[k];
+            Ta[k] = 0.5 * oc.rho * Tdq[k];
+        }
+        if (flags & BF_AERO_CORRECT) correct_aero64((flags >> BF_APPROX_SHIFT) & 3u, S.Cd, S.cg2o, Fa, Ta);
+        for (int k = 0; k < 3; ++k) {
+            o[6 + k] = Fa[k];
+            o[9 + k] = Ta[k];
         }
         o[20] = (double)(4u | 8u);   // written: the quadratic-drag slots
         o[21] = (double)pos;
@@ -1208,6 +1243,18 @@ __global__ void k_compound(const CompStatic* __restrict__ cs, const BodyStatic* 
     if (cpos == 0) {
         for (int k = 0; k < 20; ++k) o[k] = 0.0;
         o[20] = 255.0; o[21] = 0.0;
+        return;
+    }
+    if (C.flags & BF_MODE_AERO) {   // Compound::ComputeAerodynamicForces (Compound.cpp:439-464)
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t p = C.part0; p < C.part0 + C.nparts; ++p) {
+            if (dyn[p].pos != 1) continue;
+            const double* row = part_out + (size_t)p * SFH_OUT_DOUBLES;
+            for (int k = 0; k < 6; ++k) acc[k] += row[6 + k];
+        }
+        for (int k = 0; k < 6; ++k) o[6 + k] = acc[k];
+        o[20] = (double)(4u | 8u);
+        o[21] = 1.0;
         return;
     }
     if (cpos == 1) {
@@ -1890,6 +1937,137 @@ extern "C" int sfh_compound_batch_create(const sfh_compound_desc* comps, uint32_
     if (hipMalloc(&b->d_dyn, sizeof(BodyDyn) * std::max<uint64_t>(1, nparts)) != hipSuccess ||
         hipMalloc(&b->d_comp, sizeof(CompStatic) * std::max<uint32_t>(1, n)) != hipSuccess ||
         hipMalloc(&b->d_part_out, sizeof(double) * SFH_OUT_DOUBLES * std::max<uint64_t>(1, nparts)) != hipSuccess)
+        rc = fail(SFH_ERR_HIP, "compound batch alloc failed");
+    if (rc == SFH_OK && n &&
+        hipMemcpy(b->d_comp, cs.data(), sizeof(CompStatic) * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SFH_ERR_HIP, "compound upload failed");
+    if (rc == SFH_OK) rc = upload_units(b, units);
+    if (rc == SFH_OK) rc = apply_opts(b, opts);
+    if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
+    *out = b;
+    return SFH_OK;
+}
+
+// Rows of the inverse of a GL matrix's basis, by cofactors (btMatrix3x3::inverse,
+// btMatrix3x3.h:1093-1103).
+static bool basis_inverse_rows(const double* gl, double* M)
+{
+    double B[9];
+    gl_to_rows(gl, B, nullptr);
+#define CF(r1, c1, r2, c2) (B[3 * (r1) + (c1)] * B[3 * (r2) + (c2)] - B[3 * (r1) + (c2)] * B[3 * (r2) + (c1)])
+    const double co0 = CF(1, 1, 2, 2), co1 = CF(1, 2, 2, 0), co2 = CF(1, 0, 2, 1);
+    const double det = B[0] * co0 + B[1] * co1 + B[2] * co2;
+    if (!(det != 0.0) || !std::isfinite(det)) return false;
+    const double s = 1.0 / det;
+    const double R[9] = {co0 * s, CF(0, 2, 2, 1) * s, CF(0, 1, 1, 2) * s,
+                         co1 * s, CF(0, 0, 2, 2) * s, CF(0, 2, 1, 0) * s,
+                         co2 * s, CF(0, 1, 2, 0) * s, CF(0, 0, 1, 1) * s};
+#undef CF
+    memcpy(M, R, sizeof(R));
+    return true;
+}
+
+// Flags and correction data of an aerodynamic body or part: the proxy type in
+// the flags, fdApproxParams in Cd, the inverse proxy basis in cg2o.
+static int aero_static(BodyStatic& s, int32_t type, const double* params, const double* T_CG2H)
+{
+    if (type < SFH_APPROX_AUTO || type > SFH_APPROX_ELLIPSOID) return fail(SFH_ERR_INVALID, "bad approx_type");
+    if (!basis_inverse_rows(T_CG2H, s.cg2o)) return fail(SFH_ERR_INVALID, "T_CG2H basis is singular");
+    for (int k = 0; k < 3; ++k) s.Cd[k] = params[k];
+    s.flags |= BF_MODE_AERO | BF_AERO_CORRECT | ((uint32_t)type << BF_APPROX_SHIFT);
+    return SFH_OK;
+}
+
+extern "C" int sfh_aero_batch_create(const sfh_aero_desc* bodies, uint32_t n, const sfh_batch_opts* opts,
+                                     sfh_batch* out)
+{
+    if (!out || (!bodies && n)) return fail(SFH_ERR_INVALID, "null argument");
+    sfh_batch_t* b = new sfh_batch_t();
+    b->n = n;
+    b->nunits = n;
+    b->aero = true;
+    hipGetDevice(&b->device);
+    b->h_static.resize(n);
+    std::vector<UnitStatic> units(n);
+    int rc = SFH_OK;
+    for (uint32_t i = 0; i < n && rc == SFH_OK; ++i) {
+        const sfh_aero_desc& d = bodies[i];
+        BodyStatic s = make_static(d.mesh, 0u);
+        s.sidx = i;
+        if (d.mesh) {
+            d.mesh->refs.fetch_add(1);
+            b->meshes.push_back(d.mesh);
+        }
+        b->meshes_by_body.push_back(d.mesh);
+        gl_to_rows(d.T_CG2C, s.cg2c, s.cg2c + 9);
+        if (d.aero) rc = aero_static(s, d.approx_type, d.approx_params, d.T_CG2H);
+        b->h_static[i] = s;
+        b->nfaces += s.nf;
+    }
+    if (rc == SFH_OK && hipMalloc(&b->d_dyn, sizeof(BodyDyn) * std::max<size_t>(1, n)) != hipSuccess)
+        rc = fail(SFH_ERR_HIP, "batch alloc failed");
+    if (rc == SFH_OK) rc = upload_units(b, units);
+    if (rc == SFH_OK) rc = apply_opts(b, opts);
+    if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
+    *out = b;
+    return SFH_OK;
+}
+
+extern "C" int sfh_aero_compound_batch_create(const sfh_aero_compound_desc* comps, uint32_t n,
+                                              const sfh_batch_opts* opts, sfh_batch* out)
+{
+    if (!out || (!comps && n)) return fail(SFH_ERR_INVALID, "null argument");
+    uint64_t nparts = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        if (comps[c].nparts == 0 || !comps[c].parts) return fail(SFH_ERR_INVALID, "compound %u has no parts", c);
+        for (uint32_t p = 0; p < comps[c].nparts; ++p)
+            if (!comps[c].parts[p].mesh) return fail(SFH_ERR_INVALID, "compound %u part %u has no mesh", c, p);
+        nparts += comps[c].nparts;
+    }
+    if (nparts > (1u << 26)) return fail(SFH_ERR_SIZE, "too many parts");
+    sfh_batch_t* b = new sfh_batch_t();
+    b->n = (uint32_t)nparts;
+    b->nunits = n;
+    b->compound = true;
+    b->aero = true;
+    hipGetDevice(&b->device);
+    std::vector<CompStatic> cs(n);
+    std::vector<UnitStatic> units(n);
+    int rc = SFH_OK;
+    for (uint32_t c = 0; c < n; ++c) {
+        const sfh_aero_compound_desc& C = comps[c];
+        CompStatic k;
+        memset(&k, 0, sizeof(k));
+        k.part0 = (uint32_t)b->h_static.size();
+        k.nparts = C.nparts;
+        k.flags = C.aero ? BF_MODE_AERO : 0u;
+        cs[c] = k;
+        for (uint32_t p = 0; p < C.nparts; ++p) {
+            const sfh_aero_part_desc& P = C.parts[p];
+            // part rule of the device prologue: a part runs when inside and
+            // external with BF_PHYDRO, so BF_PHYDRO marks every aerodynamic part
+            BodyStatic s = make_static(P.mesh, BF_PART | (P.external ? BF_EXTERNAL : 0u));
+            s.sidx = c;
+            P.mesh->refs.fetch_add(1);
+            b->meshes.push_back(P.mesh);
+            b->meshes_by_body.push_back(P.mesh);
+            // T_C of the part = T_CG * T_CG2O * origin * O2C (Compound.cpp:458)
+            double o2p[16], cg2c[16];
+            gl_mul_host(C.T_CG2O, P.origin, o2p);
+            gl_mul_host(o2p, P.O2C, cg2c);
+            gl_to_rows(cg2c, s.cg2c, s.cg2c + 9);
+            if (C.aero && rc == SFH_OK) {
+                s.flags |= BF_PHYDRO;
+                rc = aero_static(s, P.approx_type, P.approx_params, P.T_CG2H);
+            }
+            b->h_static.push_back(s);
+            b->nfaces += s.nf;
+        }
+    }
+    if (rc == SFH_OK &&
+        (hipMalloc(&b->d_dyn, sizeof(BodyDyn) * std::max<uint64_t>(1, nparts)) != hipSuccess ||
+         hipMalloc(&b->d_comp, sizeof(CompStatic) * std::max<uint32_t>(1, n)) != hipSuccess ||
+         hipMalloc(&b->d_part_out, sizeof(double) * SFH_OUT_DOUBLES * std::max<uint64_t>(1, nparts)) != hipSuccess))
         rc = fail(SFH_ERR_HIP, "compound batch alloc failed");
     if (rc == SFH_OK && n &&
         hipMemcpy(b->d_comp, cs.data(), sizeof(CompStatic) * n, hipMemcpyHostToDevice) != hipSuccess)
@@ -2713,6 +2891,7 @@ extern "C" int sfh_surrogate_features(sfh_batch b, sfh_ocean o, const double* d_
                                       float* d_raw_target, float* d_target, void* stream)
 {
     if (!b || !d_states) return fail(SFH_ERR_INVALID, "null argument");
+    if (b->aero) return fail(SFH_ERR_INVALID, "the surrogate models hydrodynamic batches, not aerodynamic ones");
     int rc = check_ocean(o);
     if (rc != SFH_OK) return rc;
     if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "body range out of the batch");
@@ -2735,6 +2914,7 @@ extern "C" int sfh_surrogate_apply(sfh_batch b, sfh_ocean o, const double* d_sta
                                    void* stream)
 {
     if (!b || !d_states || !norm || !d_y || !d_ft) return fail(SFH_ERR_INVALID, "null argument");
+    if (b->aero) return fail(SFH_ERR_INVALID, "the surrogate models hydrodynamic batches, not aerodynamic ones");
     int rc = check_ocean(o);
     if (rc != SFH_OK) return rc;
     if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "body range out of the batch");
@@ -2945,6 +3125,7 @@ extern "C" int sfh_surrogate_plan_create(sfh_batch b, uint32_t ngroups, const ui
                                          sfh_sur_plan* out)
 {
     if (!b || !counts || !norms || !nets || !out) return fail(SFH_ERR_INVALID, "null argument");
+    if (b->aero) return fail(SFH_ERR_INVALID, "the surrogate models hydrodynamic batches, not aerodynamic ones");
     if (ngroups == 0 || ngroups > SFM_MAX_GROUP) return fail(SFH_ERR_SIZE, "1 to %d groups", SFM_MAX_GROUP);
     uint64_t total = 0;
     for (uint32_t g = 0; g < ngroups; ++g) {
@@ -3044,6 +3225,7 @@ extern "C" int sfh_surrogate_set_table(sfh_batch b, uint32_t first, uint32_t cou
                                        uint32_t ns, uint32_t ntheta, uint32_t nphi)
 {
     if (!b) return fail(SFH_ERR_INVALID, "null batch");
+    if (b->aero) return fail(SFH_ERR_INVALID, "the surrogate models hydrodynamic batches, not aerodynamic ones");
     if ((uint64_t)first + count > b->nunits) return fail(SFH_ERR_SIZE, "unit range out of the batch");
     if (d_table && (ns < 2 || ntheta < 2 || nphi < 2)) return fail(SFH_ERR_SIZE, "table dimensions must be >= 2");
     if (count == 0) return SFH_OK;

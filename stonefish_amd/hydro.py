@@ -402,6 +402,91 @@ class CompoundBatch(BodyBatch):
         self.bodies = comps
 
 
+@dataclass
+class AeroDesc:
+    """What SolidEntity::ComputeAerodynamicForces(atm) and
+    CorrectAerodynamicForces read from the entity (SolidEntity.cpp:1852-1948)."""
+    mesh: Mesh | None
+    T_CG2C: np.ndarray = field(default_factory=lambda: _ident())
+    T_CG2H: np.ndarray = field(default_factory=lambda: _ident())
+    approx_params: np.ndarray = field(default_factory=lambda: np.ones(3))
+    approx_type: int = N.SFH_APPROX_SPHERE
+    aero: bool = True
+
+
+@dataclass
+class AeroPartDesc:
+    """One part of Compound::ComputeAerodynamicForces (Compound.cpp:439-464)."""
+    mesh: Mesh
+    origin: np.ndarray = field(default_factory=lambda: _ident())
+    O2C: np.ndarray = field(default_factory=lambda: _ident())
+    T_CG2H: np.ndarray = field(default_factory=lambda: _ident())
+    approx_params: np.ndarray = field(default_factory=lambda: np.ones(3))
+    approx_type: int = N.SFH_APPROX_SPHERE
+    external: bool = True
+
+
+@dataclass
+class AeroCompoundDesc:
+    T_CG2O: np.ndarray
+    parts: list[AeroPartDesc]
+    aero: bool = True
+
+
+class AeroBatch(BodyBatch):
+    """Bodies in the air (sfh_aero_batch_create): compute on an Atmosphere;
+    each row holds Fda in [6:9] and Tda in [9:12] after CorrectAerodynamicForces."""
+
+    def __init__(self, bodies: list[AeroDesc], faces_per_lane: int = 0, waves: int = 0):
+        lib = N.load()
+        arr = (N.sfh_aero_desc * max(1, len(bodies)))()
+        for i, b in enumerate(bodies):
+            d = arr[i]
+            d.mesh = b.mesh.handle if b.mesh is not None else None
+            d.T_CG2C[:] = list(_vec(b.T_CG2C, 16))
+            d.T_CG2H[:] = list(_vec(b.T_CG2H, 16))
+            d.approx_params[:] = list(_vec(b.approx_params))
+            d.approx_type = int(b.approx_type)
+            d.aero = int(b.aero)
+        opts = N.sfh_batch_opts(faces_per_lane, waves)
+        h = C.c_void_p()
+        N.check(lib.sfh_aero_batch_create(arr, len(bodies), C.byref(opts), C.byref(h)))
+        self.handle = h
+        self.n = len(bodies)
+        self.bodies = bodies
+
+
+class AeroCompoundBatch(BodyBatch):
+    """Compounds in the air (sfh_aero_compound_batch_create); rows per compound."""
+
+    def __init__(self, comps: list[AeroCompoundDesc], faces_per_lane: int = 0, waves: int = 0):
+        lib = N.load()
+        arr = (N.sfh_aero_compound_desc * max(1, len(comps)))()
+        self._parts = []
+        for i, c in enumerate(comps):
+            arr[i].T_CG2O[:] = list(_vec(c.T_CG2O, 16))
+            arr[i].aero = int(c.aero)
+            parts = (N.sfh_aero_part_desc * max(1, len(c.parts)))()
+            for j, p in enumerate(c.parts):
+                q = parts[j]
+                q.mesh = p.mesh.handle
+                q.origin[:] = list(_vec(p.origin, 16))
+                q.O2C[:] = list(_vec(p.O2C, 16))
+                q.T_CG2H[:] = list(_vec(p.T_CG2H, 16))
+                q.approx_params[:] = list(_vec(p.approx_params))
+                q.approx_type = int(p.approx_type)
+                q.external = int(p.external)
+            self._parts.append(parts)
+            arr[i].parts = parts
+            arr[i].nparts = len(c.parts)
+        opts = N.sfh_batch_opts(faces_per_lane, waves)
+        h = C.c_void_p()
+        N.check(lib.sfh_aero_compound_batch_create(arr, len(comps), C.byref(opts), C.byref(h)))
+        self.handle = h
+        self.n = len(comps)
+        self.bodies = comps
+
+
 # ---- hull preprocessing (host) -------------------------------------------------
 
 def compute_physical_properties(verts, faces, thickness: float = 0.0, density: float = 1000.0) -> dict:
