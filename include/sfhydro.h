@@ -311,6 +311,38 @@ typedef struct {
 int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_t n,
                         float* d_grid, void* stream);
 
+/* ---- headless FFT wave producer ----------------------------------------- */
+
+/* Replaces OpenGLOcean::GenerateWavesSpectrum (OpenGLOcean.cpp:1025-1046) with
+ * the sea state of OpenGLRealOcean (OpenGLRealOcean.cpp:43-45: wind = 5 state
+ * + 2, omega = 5 exp(-state) + 0.2, A = 1) and OpenGLOcean's constants
+ * (km = 370, cm = 0.23, propagating waves): the h0(k) spectra of the four
+ * nested grids on the n x n texels, n = 2^passes, as the reference's
+ * texSpectrum12 / texSpectrum34 (RGBA per texel: grid 1 re, im, grid 2 re, im
+ * and grids 3, 4).  grid_sizes NULL = the reference's (893, 101, 21, 11) m.
+ * Random phases from the reference's LCG (SystemUtil.hpp:207-217) started at
+ * seed 1234, as for the first ocean of a process.  Host code. */
+int sfh_wave_spectrum(float state, uint32_t passes, const float grid_sizes[4], float* spectrum12,
+                      float* spectrum34);
+
+/* Replaces OpenGLOcean::Simulate's wave passes (OpenGLOcean.cpp:455-530:
+ * oceanInit.frag, oceanFFTX.frag, oceanFFTY.frag, the butterfly table of
+ * ComputeButterflyLookupTable, 879-925) and the readback of the height layer
+ * into fftData (OpenGLRealOcean.cpp:333-356), without OpenGL.  create uploads
+ * the spectra (rounded to binary16 like the reference's RGBA16F textures; n =
+ * 2^passes, 1 <= passes <= 11) and the butterfly table; step writes the wave
+ * heights at time t (seconds) into d_heights, device memory of n x n x 4
+ * floats (texel (x, y) at (y n + x) 4: the heights of grids 1-4), which is
+ * the reference's layer-0 layout: sfh_ocean_set_waves(ocean, d_heights, 0,
+ * n, 4, 2, grid_sizes) gives Ocean::GetDepth's two-grid query.  Three
+ * launches on `stream`, no allocation. */
+typedef struct sfh_wave_fft_t* sfh_wave_fft;
+int sfh_wave_fft_create(uint32_t passes, const float grid_sizes[4], const float* spectrum12,
+                        const float* spectrum34, sfh_wave_fft* out);
+int sfh_wave_fft_step(sfh_wave_fft w, float t, float* d_heights, void* stream);
+int sfh_wave_fft_info(sfh_wave_fft w, uint32_t* n);
+int sfh_wave_fft_destroy(sfh_wave_fft w);
+
 /* ---- surrogate glue (no counterpart in the reference) -------------------- */
 
 /* The learned surrogate (sfmlp.h) sees 13 features of a body's state, all in

@@ -151,6 +151,11 @@ _PROTOS = {
                                         C.POINTER(C.c_void_p)]),
     "sfh_aero_compound_batch_create": (C.c_int, [C.POINTER(sfh_aero_compound_desc), C.c_uint32,
                                                  C.POINTER(sfh_batch_opts), C.POINTER(C.c_void_p)]),
+    "sfh_wave_spectrum": (C.c_int, [C.c_float, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfh_wave_fft_create": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sfh_wave_fft_step": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]),
+    "sfh_wave_fft_info": (C.c_int, [C.c_void_p, c_uint32_p]),
+    "sfh_wave_fft_destroy": (C.c_int, [C.c_void_p]),
     "sfh_batch_destroy": (C.c_int, [C.c_void_p]),
     "sfh_batch_set_opts": (C.c_int, [C.c_void_p, C.POINTER(sfh_batch_opts)]),
     "sfh_batch_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint32_p, C.POINTER(C.c_uint64)]),

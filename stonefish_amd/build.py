@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 
 LIBS = {
-    "libsfhydro.so": ["sfh_hydro.hip", "sfh_hull.cpp"],
+    "libsfhydro.so": ["sfh_hydro.hip", "sfh_hull.cpp", "sfh_waves.hip"],
     "libsfmlp.so": ["sfh_mlp.hip"],
 }
 

@@ -64,3 +64,54 @@ def build_grid_device(comps: np.ndarray, n: int, d_grid: int, stream: int = 0) -
         arr[k].mx = int(c["mx"])
         arr[k].my = int(c["my"])
     N.check(N.load().sfh_wave_grid_build(arr, len(comps), n, C.c_void_p(d_grid), C.c_void_p(stream)))
+
+
+# ---- the reference's FFT ocean, headless ----------------------------------------
+
+REFERENCE_GRID_SIZES = (893.0, 101.0, 21.0, 11.0)
+
+
+def reference_spectrum(state: float, passes: int = 8, grid_sizes=REFERENCE_GRID_SIZES):
+    """sfh_wave_spectrum: (spectrum12, spectrum34), each (n, n, 4) float32,
+    n = 2^passes (OpenGLOcean::GenerateWavesSpectrum for the sea state
+    OpenGLRealOcean derives from `state`)."""
+    n = 1 << passes
+    s12 = np.zeros((n, n, 4), np.float32)
+    s34 = np.zeros((n, n, 4), np.float32)
+    g = np.ascontiguousarray(grid_sizes, dtype=np.float32)
+    N.check(N.load().sfh_wave_spectrum(float(state), int(passes), g.ctypes.data_as(C.c_void_p),
+                                       s12.ctypes.data_as(C.c_void_p), s34.ctypes.data_as(C.c_void_p)))
+    return s12, s34
+
+
+class WaveProducer:
+    """sfh_wave_fft_*: the reference's wave heights (fftData layer 0) at time t
+    computed on the device into an (n, n, 4) float32 buffer."""
+
+    def __init__(self, spectrum12, spectrum34, passes: int = 8, grid_sizes=REFERENCE_GRID_SIZES):
+        n = 1 << passes
+        a = np.ascontiguousarray(spectrum12, dtype=np.float32).reshape(n, n, 4)
+        b = np.ascontiguousarray(spectrum34, dtype=np.float32).reshape(n, n, 4)
+        self.grid_sizes = np.ascontiguousarray(grid_sizes, dtype=np.float32)
+        h = C.c_void_p()
+        N.check(N.load().sfh_wave_fft_create(int(passes), self.grid_sizes.ctypes.data_as(C.c_void_p),
+                                             a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                                             C.byref(h)))
+        self.handle = h
+        self.n = n
+        self.passes = passes
+
+    def step(self, t: float, d_heights: int, stream: int = 0) -> None:
+        """Writes the heights at time t to device memory d_heights (n*n*4 floats)."""
+        N.check(N.load().sfh_wave_fft_step(self.handle, float(t), C.c_void_p(d_heights), C.c_void_p(stream)))
+
+    def close(self) -> None:
+        if self.handle:
+            N.check(N.load().sfh_wave_fft_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
