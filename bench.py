@@ -134,7 +134,8 @@ def surrogate_section(args) -> dict:
     import torch
     from stonefish_amd import surrogate as S
     t0 = time.time()
-    cfg = S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=args.sur_lr, warmup=200)
+    cfg = S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=args.sur_lr, warmup=200,
+                        t_weight=args.sur_t_weight)
     r = S.run_training_workload(8, args.sur_per_hull, args.sur_steps, args.sur_holdout, cfg, seed=17, log=log)
     hulls, nets, norms, w, ocean = r["hulls"], r["nets"], r["norms"], r["workload"], r["ocean"]
     ds = r["dataset"]
@@ -301,6 +302,7 @@ def main() -> int:
     ap.add_argument("--sur-train-steps", type=int, default=40000)
     ap.add_argument("--sur-lr", type=float, default=5e-3, help="peak Adam learning rate (cosine schedule)")
     ap.add_argument("--sur-batch", type=int, default=16384)
+    ap.add_argument("--sur-t-weight", type=float, default=1.0, help="torque outputs' weight in the training loss")
     ap.add_argument("--rollout-bodies", type=int, default=16384)
     ap.add_argument("--rollout-steps", type=int, default=300)
     ap.add_argument("--rollout-data-steps", type=int, default=14)
