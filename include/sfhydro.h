@@ -311,6 +311,28 @@ typedef struct {
 int sfh_wave_grid_build(const sfh_wave_component* comps, uint32_t ncomp, uint32_t n,
                         float* d_grid, void* stream);
 
+/* ---- tick driver ----------------------------------------------------------- */
+
+/* Replaces the geometry-based force step of SimulationManager::
+ * SimulationTickCallback (SimulationManager.cpp:1583-1655) with
+ * Ocean::ApplyFluidForces (Ocean.cpp:219-268) or Atmosphere::ApplyFluidForces
+ * (Atmosphere.cpp:160-195) for every body of a batch: on ticks where
+ * fdCounter % fdPrescaler == 0 (the counter starts at 0 and advances every
+ * step) the bodies with d_overlap[i] != 0 (the fluid ghost's overlapping
+ * pairs; NULL = all) are recomputed into d_out, which holds the entity
+ * members between ticks (zero it once, as the entities start); every tick
+ * d_applied (n x 6 doubles, device) receives the force and torque the
+ * entities apply: Fb + Fdq + Fdf and Tb + Tdq + Tdf (ApplyHydrodynamicForces,
+ * SolidEntity.cpp:1958-1962), or Fda and Tda for an aerodynamic batch, and
+ * zeros for rows the fluid does not overlap.  Asynchronous on `stream`: the
+ * host can run other work until it synchronises; *recomputed (may be NULL)
+ * tells whether this tick recomputed.  The batch must outlive the driver. */
+typedef struct sfh_tick_t* sfh_tick;
+int sfh_tick_create(sfh_batch batch, uint32_t prescaler, sfh_tick* out);
+int sfh_tick_step(sfh_tick tick, sfh_ocean fluid, const double* d_states, const uint8_t* d_overlap,
+                  double* d_out, double* d_applied, int* recomputed, void* stream);
+int sfh_tick_destroy(sfh_tick tick);
+
 /* ---- headless FFT wave producer ----------------------------------------- */
 
 /* Replaces OpenGLOcean::GenerateWavesSpectrum (OpenGLOcean.cpp:1025-1046) with

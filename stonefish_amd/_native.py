@@ -151,6 +151,10 @@ _PROTOS = {
                                         C.POINTER(C.c_void_p)]),
     "sfh_aero_compound_batch_create": (C.c_int, [C.POINTER(sfh_aero_compound_desc), C.c_uint32,
                                                  C.POINTER(sfh_batch_opts), C.POINTER(C.c_void_p)]),
+    "sfh_tick_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sfh_tick_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.POINTER(C.c_int), C.c_void_p]),
+    "sfh_tick_destroy": (C.c_int, [C.c_void_p]),
     "sfh_wave_spectrum": (C.c_int, [C.c_float, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfh_wave_fft_create": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "sfh_wave_fft_step": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]),

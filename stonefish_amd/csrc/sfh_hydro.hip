@@ -2226,6 +2226,75 @@ extern "C" int sfh_batch_compute_masked(sfh_batch b, sfh_ocean o, const double* 
     return rc;
 }
 
+// ---- tick driver ---------------------------------------------------------------
+
+// ApplyHydrodynamicForces (SolidEntity.cpp:1958-1962) / ApplyAerodynamicForces
+// (1964-1968) of every row the fluid overlaps this tick: the force and torque
+// the entity hands to Bullet from the members the last recompute left.
+__global__ __launch_bounds__(128) void k_tick_apply(const double* __restrict__ out, const uint8_t* __restrict__ overlap,
+                                                    uint32_t n, int aero, double* __restrict__ applied)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* o = out + (size_t)i * SFH_OUT_DOUBLES;
+    double* a = applied + (size_t)i * 6;
+    if (overlap && !overlap[i]) {
+        for (int k = 0; k < 6; ++k) a[k] = 0.0;
+        return;
+    }
+    for (int k = 0; k < 3; ++k) {
+        a[k] = aero ? o[6 + k] : (o[k] + o[6 + k]) + o[12 + k];
+        a[3 + k] = aero ? o[9 + k] : (o[3 + k] + o[9 + k]) + o[15 + k];
+    }
+}
+
+struct sfh_tick_t {
+    sfh_batch batch = nullptr;
+    uint32_t prescaler = 1;
+    uint64_t counter = 0;
+};
+
+extern "C" int sfh_tick_create(sfh_batch batch, uint32_t prescaler, sfh_tick* out)
+{
+    if (!batch || !out) return fail(SFH_ERR_INVALID, "null argument");
+    if (prescaler == 0) return fail(SFH_ERR_INVALID, "prescaler must be at least 1");
+    sfh_tick_t* t = new sfh_tick_t();
+    t->batch = batch;
+    t->prescaler = prescaler;
+    *out = t;
+    return SFH_OK;
+}
+
+extern "C" int sfh_tick_step(sfh_tick t, sfh_ocean fluid, const double* d_states, const uint8_t* d_overlap,
+                             double* d_out, double* d_applied, int* recomputed, void* stream)
+{
+    if (!t || !d_states || !d_out || !d_applied) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(fluid);
+    if (rc != SFH_OK) return rc;
+    // recompute = fdCounter % fdPrescaler == 0; ++fdCounter (SimulationManager.cpp:1584-1585)
+    const bool recompute = t->counter % t->prescaler == 0;
+    ++t->counter;
+    hipStream_t s = (hipStream_t)stream;
+    if (recompute) {
+        rc = sfh_batch_compute_masked(t->batch, fluid, d_states, d_overlap, d_out, stream);
+        if (rc != SFH_OK) return rc;
+    }
+    const uint32_t n = t->batch->nunits;
+    if (n) hipLaunchKernelGGL(k_tick_apply, dim3((n + 127) / 128), dim3(128), 0, s, d_out, d_overlap, n,
+                              t->batch->aero ? 1 : 0, d_applied);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SFH_ERR_HIP, "tick apply: %s", hipGetErrorString(e));
+    if (recomputed) *recomputed = recompute ? 1 : 0;
+    return SFH_OK;
+}
+
+extern "C" int sfh_tick_destroy(sfh_tick t)
+{
+    if (!t) return fail(SFH_ERR_INVALID, "null argument");
+    delete t;
+    return SFH_OK;
+}
+
 extern "C" int sfh_batch_compute_host(sfh_batch b, sfh_ocean o, const double* states, double* out)
 {
     if (!b || !states || !out) return fail(SFH_ERR_INVALID, "null argument");

@@ -487,6 +487,37 @@ class AeroCompoundBatch(BodyBatch):
         self.bodies = comps
 
 
+class TickDriver:
+    """sfh_tick_*: the per-tick force step of the simulation loop for every
+    body of a batch, recomputing every `prescaler` ticks (fdPrescaler) and
+    applying the held forces every tick (SimulationManager.cpp:1583-1655)."""
+
+    def __init__(self, batch: BodyBatch, prescaler: int = 1):
+        h = C.c_void_p()
+        N.check(N.load().sfh_tick_create(batch.handle, int(prescaler), C.byref(h)))
+        self.handle = h
+        self.batch = batch
+
+    def step(self, fluid: Ocean, d_states: int, d_overlap: int | None, d_out: int, d_applied: int,
+             stream: int = 0) -> bool:
+        """Device pointers; returns whether this tick recomputed."""
+        r = C.c_int()
+        N.check(N.load().sfh_tick_step(self.handle, fluid.handle, C.c_void_p(d_states), C.c_void_p(d_overlap or 0),
+                                       C.c_void_p(d_out), C.c_void_p(d_applied), C.byref(r), C.c_void_p(stream)))
+        return bool(r.value)
+
+    def close(self) -> None:
+        if self.handle:
+            N.check(N.load().sfh_tick_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- hull preprocessing (host) -------------------------------------------------
 
 def compute_physical_properties(verts, faces, thickness: float = 0.0, density: float = 1000.0) -> dict:
