@@ -6,6 +6,8 @@
 //   SolidEntity::ComputeHydrodynamicForcesSurface    Library/include/entities/SolidEntity.h:135-137
 //   SolidEntity::ComputeHydrodynamicForcesSubmerged  Library/include/entities/SolidEntity.h:152-153
 //   SolidEntity::CorrectHydrodynamicForces           Library/include/entities/SolidEntity.h:113-114
+//   SolidEntity::ComputeAerodynamicForces (static)   Library/include/entities/SolidEntity.h:180-181
+//   SolidEntity::CorrectAerodynamicForces            Library/include/entities/SolidEntity.h:167
 // The value types are minimal stand-ins for the reference's (Scalar = double,
 // Vector3, Transform with getOpenGLMatrix, a Mesh of Vertex/Face); a Stonefish
 // build passes its own types through the small adaptors at the bottom instead.
@@ -121,8 +123,20 @@ public:
     void DisableCurrents() { check(sfh_ocean_enable_currents(h_, 0)); }
     sfh_ocean handle() const { return h_; }
 
-private:
+protected:
     sfh_ocean h_ = nullptr;
+};
+
+// Gas and wind fields (the reference's Atmosphere*): every wind field is
+// summed, as Atmosphere::GetFluidVelocity does (Atmosphere.cpp:146-152).
+class Atmosphere : public Ocean {
+public:
+    explicit Atmosphere(Scalar density = 1.225, const Vector3& gravity = Vector3(0, 0, 9.81))
+        : Ocean(density, gravity)
+    {
+        EnableCurrents();
+    }
+    void addUniformWind(const Vector3& v) { addUniformCurrent(v, true); }
 };
 
 struct SolidEntity {
@@ -151,6 +165,26 @@ struct SolidEntity {
         T_C.getOpenGLMatrix(tc);
         return sfh_forces_submerged(mesh ? mesh->handle() : nullptr, ocn->handle(), tcg, tc, v.v, omega.v, Fdq.v,
                                     Tdq.v, Fdf.v, Tdf.v);
+    }
+
+    // SolidEntity.cpp:1865-1923.
+    static int ComputeAerodynamicForces(const Mesh* mesh, Atmosphere* atm, const Transform& T_CG,
+                                        const Transform& T_C, const Vector3& v, const Vector3& omega,
+                                        Vector3& Fda, Vector3& Tda)
+    {
+        double tcg[16], tc[16];
+        T_CG.getOpenGLMatrix(tcg);
+        T_C.getOpenGLMatrix(tc);
+        return sfh_forces_aero(mesh ? mesh->handle() : nullptr, atm->handle(), tcg, tc, v.v, omega.v, Fda.v, Tda.v);
+    }
+
+    // SolidEntity.cpp:1925-1948; the entity's fdApproxType, fdApproxParams and T_CG2H.
+    static int CorrectAerodynamicForces(int fdApproxType, const Scalar fdApproxParams[3], const Transform& T_CG2H,
+                                        Vector3& Fda, Vector3& Tda)
+    {
+        double h[16];
+        T_CG2H.getOpenGLMatrix(h);
+        return sfh_correct_aero(fdApproxType, fdApproxParams, h, Fda.v, Tda.v);
     }
 
     // SolidEntity.cpp:1263-1287.

@@ -41,6 +41,8 @@ int main(int argc, char** argv)
             for (int k = 0; k < 3; ++k) faces[i].vertexID[k] = ids[3 * i + k];
         sfh::Mesh mesh(verts, faces);
         sfh::Ocean ocn(1000.0, sfh::Vector3(0, 0, 9.81));
+        sfh::Atmosphere atm(1.2);
+        atm.addUniformWind(sfh::Vector3(2.0, -1.0, 0.5));
         std::vector<double> out;
         for (uint32_t s = 0; s < ns; ++s) {
             const double* p = st.data() + 38 * (size_t)s;
@@ -66,6 +68,14 @@ int main(int argc, char** argv)
             sfh::Vector3 Cd(1.0, 0.5, 0.25), Cf(0.1, 0.05, 0.025);
             sfh::check(sfh::SolidEntity::CorrectHydrodynamicForces(&ocn, a, b, c, d, Cd, Cf, Tcg));
             for (const sfh::Vector3* x : {&a, &b, &c, &d})
+                for (int k = 0; k < 3; ++k) out.push_back((*x)[k]);
+            sfh::Vector3 Fda, Tda;
+            sfh::check(sfh::SolidEntity::ComputeAerodynamicForces(&mesh, &atm, Tcg, Tc, v, w, Fda, Tda));
+            for (const sfh::Vector3* x : {&Fda, &Tda})
+                for (int k = 0; k < 3; ++k) out.push_back((*x)[k]);
+            const sfh::Scalar params[3] = {0.9, 0.35, 0.3};
+            sfh::check(sfh::SolidEntity::CorrectAerodynamicForces(SFH_APPROX_ELLIPSOID, params, Tc, Fda, Tda));
+            for (const sfh::Vector3* x : {&Fda, &Tda})
                 for (int k = 0; k < 3; ++k) out.push_back((*x)[k]);
         }
         FILE* g = std::fopen(argv[2], "wb");
