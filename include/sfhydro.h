@@ -164,7 +164,8 @@ typedef struct {
 /* Per-body output: 24 doubles: Fb[3] Tb[3] Fdq[3] Tdq[3] Fdf[3] Tdf[3] Swet
  * Vsub, then the written mask (bit 0 Fb, 1 Tb, 2 Fdq, 3 Tdq, 4 Fdf, 5 Tdf,
  * 6 Swet, 7 Vsub) and the body/fluid position (0 outside, 1 inside,
- * 2 crossing), both stored as doubles, and 2 spare. */
+ * 2 crossing, -1 for a body that is not hydrodynamic: the mode gate returns
+ * before classifying it), both stored as doubles, and 2 spare. */
 #define SFH_OUT_DOUBLES 24
 
 typedef struct {

@@ -241,7 +241,10 @@ void orc_compound_forces_64(const orc_body* comp, const orc_part* parts, uint32_
                             const orc_ocean* o, const double T_CG[16],
                             const double v[3], const double w[3], orc_result* res)
 {
-    if (!comp->hydro) return;
+    if (!comp->hydro) {   /* mode gate (Compound.cpp:293): not classified */
+        res->position = -1;
+        return;
+    }
     int pos = orc_classify_64(comp, o, T_CG);
     res->position = pos;
     if (pos == 0) {

@@ -92,7 +92,7 @@ typedef struct {
     double Fb[3], Tb[3], Fdq[3], Tdq[3], Fdf[3], Tdf[3];
     double Swet, Vsub;
     uint32_t written;
-    int32_t position;   /* body-level entry points: 0 outside, 1 inside, 2 crossing */
+    int32_t position;   /* body-level entry points: 0 outside, 1 inside, 2 crossing, -1 not hydrodynamic */
     /* Error-analysis extras (not outputs of the reference):
      * S[6]  condition scale of Fb, Tb, Fdq, Tdq, Fdf, Tdf: the sum over faces of
      *       the norms of the per-face contributions (for the flat-sea volume
