@@ -394,6 +394,17 @@ def main() -> int:
     face_ms, total_ms, count = batch.kernel_ms()
     batch.timing(False)
 
+    # the host-buffer boundary (sfh_batch_compute_host: states in and outputs
+    # out over PCIe, synchronous), the rate a caller with host arrays sees
+    st_h = states_h[0]
+    out_h0 = np.zeros((args.bodies, 24))
+    batch.compute_host(go, st_h, out_h0)
+    t0 = time.perf_counter()
+    nh = 5
+    for _ in range(nh):
+        batch.compute_host(go, st_h, out_h0)
+    host_io = args.bodies * nh / (time.perf_counter() - t0)
+
     face_bytes = int((12 * nf + 12 * nv).sum()) + n * n * 16   # indices + vertices + wave quad texture
     traffic = measured_traffic()
     achieved = face_bytes / (face_ms * 1e-3) / 1e9
@@ -419,6 +430,7 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_faces", "kernel_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),
                      "algorithmic_bytes": face_bytes, "timed_launches": count},
+        "host_buffers_body_steps_per_s": round(host_io, 1),
     }
     if not args.no_parity:
         t0 = time.time()
