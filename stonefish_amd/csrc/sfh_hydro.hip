@@ -1197,8 +1197,22 @@ __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ 
                                                  const int8_t* __restrict__ bpos, uint32_t n, OceanK oc,
                                                  const double* __restrict__ partials, double* __restrict__ out)
 {
-    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x)
-        if (bpos[b] != -2) epilogue_body(st, dyn, b, oc, partials, out);
+    // The wavefront's bodies are blockIdx.x + k gridDim.x; lane k reads the
+    // position of the k-th of the next 64 in one load and only the bodies the
+    // step evaluates (position != -2) are visited, in the same order.  A
+    // masked step (few active bodies) thus costs two loads per 64 bodies
+    // instead of a chain of dependent ones.
+    const uint32_t G = gridDim.x;
+    for (uint32_t base = blockIdx.x; base < n; base += 64u * G) {
+        const uint32_t bl = base + (uint32_t)threadIdx.x * G;
+        const bool act = bl < n && bpos[bl] != -2;
+        uint64_t m = __ballot(act);
+        while (m) {
+            const uint32_t k = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1ull;
+            epilogue_body(st, dyn, base + k * G, oc, partials, out);
+        }
+    }
 }
 
 // Per row of the state array (a body, or a compound): what the surrogate's
