@@ -29,6 +29,7 @@ using sfm::acc_row;
 using sfm::bf16x8;
 using sfm::f2bf;
 using sfm::f2bf2;
+using sfm::ld8g;
 using sfm::f32x16;
 using sfm::gelu;
 using sfm::gelu_grad;
@@ -113,33 +114,37 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = f32x16{};
-    // operands of the next k-step are loaded while the current one multiplies
-    bf16x8 a[TM], b[TN];
+    // The operands of the next kPF k-steps are in flight while the current
+    // one multiplies (a ring of kPF register stages, refilled right after use):
+    // a k-step waits for a load issued kPF steps earlier, not for the last one.
+    constexpr int kPF = 4;
+    bf16x8 ra[kPF][TM], rb[kPF][TN];
 #pragma unroll
-    for (int i = 0; i < TM; ++i) a[i] = ld8(ap[i]);
+    for (int s = 0; s < kPF; ++s)
+        if (16 * s < kper) {
 #pragma unroll
-    for (int j = 0; j < TN; ++j) b[j] = ld8(bp[j]);
-    for (int k = 0; k < kper; k += 16) {
-        bf16x8 na[TM], nb[TN];
+            for (int i = 0; i < TM; ++i) ra[s][i] = ld8g(ap[i] + 16 * s);
 #pragma unroll
-        for (int i = 0; i < TM; ++i) na[i] = a[i];
-#pragma unroll
-        for (int j = 0; j < TN; ++j) nb[j] = b[j];
-        if (k + 16 < kper) {
-#pragma unroll
-            for (int i = 0; i < TM; ++i) na[i] = ld8(ap[i] + k + 16);
-#pragma unroll
-            for (int j = 0; j < TN; ++j) nb[j] = ld8(bp[j] + k + 16);
+            for (int j = 0; j < TN; ++j) rb[s][j] = ld8g(bp[j] + 16 * s);
         }
+    for (int k0 = 0; k0 < kper; k0 += 16 * kPF) {
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
+        for (int s = 0; s < kPF; ++s) {
+            const int k = k0 + 16 * s;
+            if (k < kper) {
 #pragma unroll
-            for (int j = 0; j < TN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int i = 0; i < TM; ++i) a[i] = na[i];
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[s][i], rb[s][j], acc[i][j], 0, 0, 0);
+                if (k + 16 * kPF < kper) {
 #pragma unroll
-        for (int j = 0; j < TN; ++j) b[j] = nb[j];
+                    for (int i = 0; i < TM; ++i) ra[s][i] = ld8g(ap[i] + k + 16 * kPF);
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) rb[s][j] = ld8g(bp[j] + k + 16 * kPF);
+                }
+            }
+        }
     }
     float sq = 0.f;
 #pragma unroll
