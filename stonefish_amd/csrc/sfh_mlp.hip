@@ -59,6 +59,7 @@ extern "C" const char* sfm_last_error(void) { return g_err; }
 static constexpr int kDims[SFM_LAYERS + 1] = {SFM_IN_PAD, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_OUT_PAD};
 static constexpr int kReal[SFM_LAYERS + 1] = {SFM_IN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_OUT};
 static constexpr int kSplitK = 256;   // batch rows per weight-gradient split
+static constexpr int kMaxSplits = 16; // batch splits of a weight-gradient product
 
 // ---------------------------------------------------------------------------
 // numerics
@@ -285,29 +286,48 @@ __global__ __launch_bounds__(256) void k_rowsum(const uint16_t* __restrict__ xt,
 // Adam on n parameters of a [rows x cols] matrix (cols = 1 for biases): the
 // gradient is the sum of nsplit partials (fixed order); keeps the FP32 master
 // copy and refreshes the BF16 copy and its transpose.
-__global__ void k_adam(float* __restrict__ w, float* __restrict__ mo, float* __restrict__ ve,
-                       const float* __restrict__ part, int nsplit, int rows, int cols, float lr, float b1, float b2,
-                       float eps, float bc1, float bc2, float* __restrict__ gsave, uint16_t* __restrict__ wb,
-                       uint16_t* __restrict__ wbt, uint16_t* __restrict__ wp)
+// One parameter tensor of an Adam step: its FP32 weights and moments, the
+// gradient partials of the batch splits (summed in a fixed order), where the
+// summed gradient is kept, and (weights only) the BF16 copies to refresh.
+struct AdamSeg {
+    float *w, *mo, *ve;
+    const float* part;
+    float* gsave;
+    uint16_t *wb, *wbt, *wp;
+    int nsplit, rows, cols, block0;   // block0: first workgroup of this tensor in the launch
+};
+
+struct AdamSet {
+    AdamSeg seg[2 * SFM_LAYERS];
+    int nseg;
+};
+
+// Adam with bias correction for every weight and bias tensor of the network
+// in one launch (each workgroup finds its tensor from the block offsets).
+__global__ __launch_bounds__(256) void k_adam(AdamSet S, float lr, float b1, float b2, float eps, float bc1,
+                                              float bc2)
 {
-    const int n = rows * cols;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int si = 0;
+    while (si + 1 < S.nseg && (int)blockIdx.x >= S.seg[si + 1].block0) ++si;
+    const AdamSeg& g = S.seg[si];
+    const int n = g.rows * g.cols;
+    const int i = ((int)blockIdx.x - g.block0) * (int)blockDim.x + (int)threadIdx.x;
     if (i >= n) return;
     float gr = 0.f;
-    for (int s = 0; s < nsplit; ++s) gr += part[(size_t)s * n + i];
-    gsave[i] = gr;
-    const float m = b1 * mo[i] + (1.f - b1) * gr;
-    const float v = b2 * ve[i] + (1.f - b2) * gr * gr;
-    mo[i] = m;
-    ve[i] = v;
-    const float wn = w[i] - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
-    w[i] = wn;
-    if (wb) {
+    for (int s = 0; s < g.nsplit; ++s) gr += g.part[(size_t)s * n + i];
+    g.gsave[i] = gr;
+    const float m = b1 * g.mo[i] + (1.f - b1) * gr;
+    const float v = b2 * g.ve[i] + (1.f - b2) * gr * gr;
+    g.mo[i] = m;
+    g.ve[i] = v;
+    const float wn = g.w[i] - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+    g.w[i] = wn;
+    if (g.wb) {
         const uint16_t h = f2bf(wn);
-        wb[i] = h;
-        const int r = i / cols, c = i - r * cols;
-        wbt[(size_t)c * rows + r] = h;
-        wp[sfm::frag_index(r, c, cols)] = h;
+        g.wb[i] = h;
+        const int r = i / g.cols, c = i - r * g.cols;
+        g.wbt[(size_t)c * g.rows + r] = h;
+        g.wp[sfm::frag_index(r, c, g.cols)] = h;
     }
 }
 
@@ -353,8 +373,8 @@ struct sfm_net_t {
     uint16_t* Ht[SFM_LAYERS] = {};   // transposed inputs
     float* Z[SFM_LAYERS] = {};       // pre-activations of layers 0..3 outputs (Z[l] = output of layer l)
     uint16_t *dZ[2] = {}, *dZt[2] = {};
-    float* part = nullptr;
-    float* dbias = nullptr;
+    float* part[SFM_LAYERS] = {};    // per layer: weight-gradient partials of the batch splits (<= 16)
+    float* dbias[SFM_LAYERS] = {};   // per layer: bias gradient
     uint64_t step = 0;
     bool timing = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -381,8 +401,7 @@ static void net_free(sfm_net_t* n)
     }
     for (int i = 0; i < SFM_LAYERS; ++i) { hipFree(n->H[i]); hipFree(n->Ht[i]); hipFree(n->Z[i]); }
     for (int i = 0; i < 2; ++i) { hipFree(n->dZ[i]); hipFree(n->dZt[i]); }
-    hipFree(n->part);
-    hipFree(n->dbias);
+    for (int i = 0; i < SFM_LAYERS; ++i) { hipFree(n->part[i]); hipFree(n->dbias[i]); }
     for (int i = 0; i < 2; ++i)
         if (n->ev[i]) hipEventDestroy(n->ev[i]);
 }
@@ -437,8 +456,9 @@ extern "C" int sfm_create(uint32_t max_batch, uint64_t seed, sfm_net* out)
     for (int i = 0; i < 2 && ok; ++i)
         ok = alloc((void**)&n->dZ[i], 2 * (size_t)max_batch * SFM_HIDDEN) &&
              alloc((void**)&n->dZt[i], 2 * (size_t)max_batch * SFM_HIDDEN);
-    if (ok) ok = alloc((void**)&n->part, 4 * (size_t)(max_batch / kSplitK) * SFM_HIDDEN * SFM_HIDDEN) &&
-                 alloc((void**)&n->dbias, 4 * SFM_HIDDEN);
+    for (int l = 0; l < SFM_LAYERS && ok; ++l)
+        ok = alloc((void**)&n->part[l], 4 * (size_t)kMaxSplits * kDims[l + 1] * kDims[l]) &&
+             alloc((void**)&n->dbias[l], 4 * (size_t)kDims[l + 1]);
     if (!ok) { net_free(n); delete n; return fail(SFM_ERR_HIP, "allocation failed"); }
     // Kaiming-uniform weights on the real fan-in, zero biases, padding zero
     uint64_t s = seed;
@@ -653,6 +673,9 @@ extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target,
     n->step++;
     const float bc1 = 1.0f - std::pow(beta1, (float)n->step), bc2 = 1.0f - std::pow(beta2, (float)n->step);
     int cur = 0;
+    AdamSet adam;
+    adam.nseg = 0;
+    int nblk = 0;
     for (int l = SFM_LAYERS - 1; l >= 0; --l) {
         const int dout = kDims[l + 1], din = kDims[l];
         Layer& L = n->L[l];
@@ -661,13 +684,13 @@ extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target,
             // batch splits of the weight-gradient product: at most 16, each a
             // multiple of 256 rows, summed in a fixed order by k_adam
             int splits = 1;
-            for (int d = 16; d >= 1; --d)
+            for (int d = kMaxSplits; d >= 1; --d)
                 if ((M / kSplitK) % d == 0) { splits = d; break; }
             GemmArgs g = gemm(n->dZt[cur], M, n->Ht[l], M, dout, din, M);
             g.ksplit = splits;
-            g.part = n->part;
+            g.part = n->part[l];
             launch_gemm<EPI_WGRAD>(g, s);
-            hipLaunchKernelGGL(k_rowsum, dim3(dout), dim3(256), 0, s, n->dZt[cur], M, n->dbias);
+            hipLaunchKernelGGL(k_rowsum, dim3(dout), dim3(256), 0, s, n->dZt[cur], M, n->dbias[l]);
             // dH = dZ W  (with the pre-update weights), then dZ of the layer below
             if (l > 0) {
                 GemmArgs h = gemm(n->dZ[cur], dout, L.Wbt, dout, M, din, dout);
@@ -676,15 +699,19 @@ extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target,
                 h.Ht = n->dZt[cur ^ 1];
                 launch_gemm<EPI_BWD>(h, s);
             }
-            const int nw = dout * din;
-            hipLaunchKernelGGL(k_adam, dim3((nw + 255) / 256), dim3(256), 0, s, L.W, L.mW, L.vW, n->part, splits, dout,
-                               din, lr, beta1, beta2, eps, bc1, bc2, L.gW, L.Wb, L.Wbt, L.Wp);
-            hipLaunchKernelGGL(k_adam, dim3((dout + 255) / 256), dim3(256), 0, s, L.b, L.mb, L.vb, n->dbias, 1, dout, 1,
-                               lr, beta1, beta2, eps, bc1, bc2, L.gb, (uint16_t*)nullptr, (uint16_t*)nullptr,
-                               (uint16_t*)nullptr);
+            // every Adam update waits for the end of the step: the products
+            // above read the pre-update weights, and each layer keeps its own
+            // gradient partials, so one launch serves all tensors
+            AdamSeg& sw = adam.seg[adam.nseg++];
+            sw = AdamSeg{L.W, L.mW, L.vW, n->part[l], L.gW, L.Wb, L.Wbt, L.Wp, splits, dout, din, nblk};
+            nblk += (dout * din + 255) / 256;
+            AdamSeg& sb = adam.seg[adam.nseg++];
+            sb = AdamSeg{L.b, L.mb, L.vb, n->dbias[l], L.gb, nullptr, nullptr, nullptr, 1, dout, 1, nblk};
+            nblk += (dout + 255) / 256;
         }
         cur ^= 1;
     }
+    hipLaunchKernelGGL(k_adam, dim3(nblk), dim3(256), 0, s, adam, lr, beta1, beta2, eps, bc1, bc2);
     time_end(n, s, 3.0 * forward_flops(batch));
     HIPCHK(hipGetLastError());
     return SFM_OK;
