@@ -305,7 +305,7 @@ def main() -> int:
     ap.add_argument("--sur-per-hull", type=int, default=8192)
     ap.add_argument("--sur-steps", type=int, default=31)
     ap.add_argument("--sur-holdout", type=int, default=3)
-    ap.add_argument("--sur-train-steps", type=int, default=40000)
+    ap.add_argument("--sur-train-steps", type=int, default=60000)
     ap.add_argument("--sur-lr", type=float, default=5e-3, help="peak Adam learning rate (cosine schedule)")
     ap.add_argument("--sur-batch", type=int, default=16384)
     ap.add_argument("--sur-t-weight", type=float, default=1.0, help="torque outputs' weight in the training loss")
