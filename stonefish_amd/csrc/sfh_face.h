@@ -212,6 +212,17 @@ SFH_HD void tetra_term(f3 bo, f3 q, f3 n, f3 s, Acc<T>& acc)
     acc.V6 += (T)v6;
 }
 
+// tetra_term with q = 0 (the polygon's first point is its base vertex):
+// the same values without the three additions of zero.
+template <typename T>
+SFH_HD void tetra_term0(f3 bo, f3 n, f3 s, Acc<T>& acc)
+{
+    float v6 = dot(bo, n);
+    f3 g = (bo * 3.0f + s) * 0.25f;
+    add3(acc.CB, g * v6);
+    acc.V6 += (T)v6;
+}
+
 // Surface face: waterline clip (7 cases, SolidEntity.cpp:1362-1647), tetra
 // volume, face properties.  d0..d2 are the vertex depths (>= 0 is wet).  The
 // tetra volume is accumulated before the degeneracy test, as in the reference.
@@ -262,7 +273,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
             f3 q3 = (p3 - p1) * (d0 / (fabsf(d2) + d0));
             fv1 = q2; fv2 = q3;
             f3 n = cross(fv1, fv2);
-            tetra_term(base - B.o, f3{0.f, 0.f, 0.f}, n, q2 + q3, acc);
+            tetra_term0(base - B.o, n, q2 + q3, acc);
             csum = q2 + q3;
         } else {                        // E: only p2 dry, quad p1 p2' p4 p3 (1521-1568)
             base = p1;
@@ -270,7 +281,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
             f3 q2 = (p2 - p1) * (d0 / (fabsf(d1) + d0));
             f3 q4 = q3 + (p2 - p3) * (d2 / (fabsf(d1) + d2));
             f3 bo = base - B.o;
-            tetra_term(bo, f3{0.f, 0.f, 0.f}, cross(q2, q3), q2 + q3, acc);     // (p1, p2', p3)
+            tetra_term0(bo, cross(q2, q3), q2 + q3, acc);     // (p1, p2', p3)
             tetra_term(bo, q2, cross(q4 - q2, q3 - q2), q2 + q4 + q3, acc);     // (p2', p4, p3)
             fv1 = q2; fv2 = q3; fv3 = q2 - q3; fv4 = q4 - q3;
             csum = q2 + q3 + q4;
@@ -282,8 +293,8 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
         f3 q3 = q2 + (p3 - p2) * (d1 / (fabsf(d2) + d1));
         f3 q4 = (p3 - p1) * (d0 / (fabsf(d2) + d0));
         f3 bo = base - B.o;
-        tetra_term(bo, f3{0.f, 0.f, 0.f}, cross(q2, q3), q2 + q3, acc);         // (p1, p2, p3')
-        tetra_term(bo, f3{0.f, 0.f, 0.f}, cross(q3, q4), q3 + q4, acc);         // (p1, p3', p4)
+        tetra_term0(bo, cross(q2, q3), q2 + q3, acc);         // (p1, p2, p3')
+        tetra_term0(bo, cross(q3, q4), q3 + q4, acc);         // (p1, p3', p4)
         fv1 = q2; fv2 = q4; fv3 = q2 - q3; fv4 = q4 - q3;
         csum = q2 + q3 + q4;
         quad = true;
@@ -291,7 +302,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
         base = p1;
         fv1 = p2 - p1; fv2 = p3 - p1;
         f3 n = cross(fv1, fv2);
-        tetra_term(base - B.o, f3{0.f, 0.f, 0.f}, n, fv1 + fv2, acc);
+        tetra_term0(base - B.o, n, fv1 + fv2, acc);
         csum = fv1 + fv2;
     }
     f3 n = cross(fv1, fv2);
@@ -323,7 +334,7 @@ SFH_HD Wet face_allwet(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
     const f3 base = p1;
     const f3 fv1 = p2 - p1, fv2 = p3 - p1;
     f3 n = cross(fv1, fv2);
-    tetra_term(base - B.o, f3{0.f, 0.f, 0.f}, n, fv1 + fv2, acc);
+    tetra_term0(base - B.o, n, fv1 + fv2, acc);
     const f3 csum = fv1 + fv2;
     n = cross(fv1, fv2);
     float n2 = dot(n, n);
