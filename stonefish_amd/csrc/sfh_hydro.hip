@@ -916,7 +916,7 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
 // together afterwards, 64 at a time, so the clip code runs once per 64 cut
 // faces rather than on every pass over the chunk.
 template <int K, bool NU>
-__global__ __launch_bounds__(256) void k_faces_cross(const uint32_t* __restrict__ list,
+__global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
                                                      const Chunk* __restrict__ chunks,
                                                      const BodyStatic* __restrict__ st,
