@@ -819,7 +819,7 @@ __device__ __forceinline__ int xreduce12_slot(int lane)
 // across the wavefront and across chunks in FP64.  `list` holds the ids of the
 // chunks of this step's inside bodies.
 template <int K, bool NU, bool AERO = false>
-__global__ __launch_bounds__(256) void k_faces_sub(const uint32_t* __restrict__ list,
+__global__ __launch_bounds__(256, NU ? 1 : 8) void k_faces_sub(const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ count,
                                                    const Chunk* __restrict__ chunks,
                                                    const BodyStatic* __restrict__ st,
