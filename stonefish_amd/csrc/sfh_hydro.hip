@@ -362,7 +362,8 @@ __device__ int classify64_g(const double* Rcg, const double* tcg, const double* 
 static constexpr int kPrologueLanes = 8;
 
 __global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
-                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active, BodyDyn* __restrict__ dyn,
+                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active,
+                           const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
                            int8_t* __restrict__ bpos)
 {
     const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__
     const int sub = (int)(gt % kPrologueLanes);
     if (b >= n) return;   // whole groups: blockDim is a multiple of kPrologueLanes
     const BodyStatic& S = st[b];
-    if (active && !active[S.sidx]) {
+    if (active && !active[sidx[b]]) {   // the compact index: a skipped body's record is never read
         if (sub == 0) {
             dyn[b].pos = -2;
             bpos[b] = -2;
@@ -1425,6 +1426,7 @@ struct sfh_batch_t {
     uint64_t nverts = 0;
     std::vector<BodyStatic> h_static;
     BodyStatic* d_static = nullptr;
+    uint32_t* d_sidx = nullptr;     // per body: BodyStatic::sidx, compact (the masked prologue's first read)
     Chunk* d_chunks = nullptr;
     Chunk* d_vchunks = nullptr;
     float* d_depth = nullptr;
@@ -1674,6 +1676,7 @@ static void batch_free(sfh_batch_t* b)
     for (int i = 0; i < 4; ++i)
         if (b->ev[i]) hipEventDestroy(b->ev[i]);
     hipFree(b->d_static);
+    hipFree(b->d_sidx);
     hipFree(b->d_chunks);
     hipFree(b->d_vchunks);
     hipFree(b->d_depth);
@@ -1737,6 +1740,7 @@ static int batch_build_chunks(sfh_batch_t* b)
         }
     }
     hipFree(b->d_static);
+    hipFree(b->d_sidx);
     hipFree(b->d_chunks);
     hipFree(b->d_vchunks);
     hipFree(b->d_depth);
@@ -1746,7 +1750,7 @@ static int batch_build_chunks(sfh_batch_t* b)
     hipFree(b->d_boff);
     hipFree(b->d_scan);
     hipFree(b->d_lists);
-    b->d_static = nullptr; b->d_chunks = nullptr; b->d_vchunks = nullptr; b->d_depth = nullptr;
+    b->d_static = nullptr; b->d_sidx = nullptr; b->d_chunks = nullptr; b->d_vchunks = nullptr; b->d_depth = nullptr;
     b->d_partials = nullptr; b->d_bpos = nullptr; b->d_bch = nullptr; b->d_boff = nullptr; b->d_lists = nullptr;
     b->d_scan = nullptr;
     b->nchunks = (uint32_t)chunks.size();
@@ -1754,6 +1758,12 @@ static int batch_build_chunks(sfh_batch_t* b)
     b->nverts = dep;
     HIPCHK(hipMalloc(&b->d_static, sizeof(BodyStatic) * std::max<size_t>(1, b->n)));
     HIPCHK(hipMemcpy(b->d_static, b->h_static.data(), sizeof(BodyStatic) * b->n, hipMemcpyHostToDevice));
+    {
+        std::vector<uint32_t> sx(b->n);
+        for (uint32_t i = 0; i < b->n; ++i) sx[i] = b->h_static[i].sidx;
+        HIPCHK(hipMalloc(&b->d_sidx, sizeof(uint32_t) * std::max<size_t>(1, b->n)));
+        if (b->n) HIPCHK(hipMemcpy(b->d_sidx, sx.data(), sizeof(uint32_t) * b->n, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMalloc(&b->d_chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size())));
     if (!chunks.empty())
         HIPCHK(hipMemcpy(b->d_chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice));
@@ -2179,7 +2189,7 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
     const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + 127) / 128);
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
     hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, active,
-                       b->d_dyn, b->d_bpos);
+                       b->d_sidx, b->d_dyn, b->d_bpos);
     int rc = launch_faces(b, oc, s, timed);
     if (rc != SFH_OK) return rc;
     if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
