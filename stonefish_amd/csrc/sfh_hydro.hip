@@ -371,11 +371,12 @@ __global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__
     const int sub = (int)(gt % kPrologueLanes);
     if (b >= n) return;   // whole groups: blockDim is a multiple of kPrologueLanes
     const BodyStatic& S = st[b];
-    if (active && !active[sidx[b]]) {   // the compact index: a skipped body's record is never read
-        if (sub == 0) {
-            dyn[b].pos = -2;
-            bpos[b] = -2;
-        }
+    // A body the step skips is marked in bpos only: its static record and its
+    // BodyDyn row are not touched (the later passes test bpos, and the
+    // compound pass returns first for an inactive compound, whose parts share
+    // its activity).
+    if (active && !active[sidx[b]]) {
+        if (sub == 0) bpos[b] = -2;
         return;
     }
     BodyDyn D;
