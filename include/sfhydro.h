@@ -272,7 +272,8 @@ int sfh_aero_compound_batch_create(const sfh_aero_compound_desc* comps, uint32_t
  * untouched keep the values d_out held. */
 int sfh_batch_compute(sfh_batch batch, sfh_ocean ocean, const double* d_states, double* d_out, void* stream);
 
-/* Same with host buffers; synchronous. */
+/* Same with host buffers; synchronous.  Its device copies of the buffers are
+ * allocated on the first call and kept with the batch. */
 int sfh_batch_compute_host(sfh_batch batch, sfh_ocean ocean, const double* states, double* out);
 
 /* sfh_batch_compute restricted to the bodies with d_active[i] != 0 (device,

@@ -1428,6 +1428,8 @@ struct sfh_batch_t {
     std::vector<BodyStatic> h_static;
     BodyStatic* d_static = nullptr;
     uint32_t* d_sidx = nullptr;     // per body: BodyStatic::sidx, compact (the masked prologue's first read)
+    double* d_hstates = nullptr;    // device copies of sfh_batch_compute_host's host buffers (first call)
+    double* d_hout = nullptr;
     Chunk* d_chunks = nullptr;
     Chunk* d_vchunks = nullptr;
     float* d_depth = nullptr;
@@ -1678,6 +1680,9 @@ static void batch_free(sfh_batch_t* b)
         if (b->ev[i]) hipEventDestroy(b->ev[i]);
     hipFree(b->d_static);
     hipFree(b->d_sidx);
+    hipFree(b->d_hstates);
+    hipFree(b->d_hout);
+    b->d_hstates = b->d_hout = nullptr;
     hipFree(b->d_chunks);
     hipFree(b->d_vchunks);
     hipFree(b->d_depth);
@@ -2323,20 +2328,23 @@ extern "C" int sfh_tick_destroy(sfh_tick t)
 extern "C" int sfh_batch_compute_host(sfh_batch b, sfh_ocean o, const double* states, double* out)
 {
     if (!b || !states || !out) return fail(SFH_ERR_INVALID, "null argument");
-    double *ds = nullptr, *dout = nullptr;
     size_t ns = (size_t)b->nunits * SFH_STATE_DOUBLES, no = (size_t)b->nunits * SFH_OUT_DOUBLES;
-    HIPCHK(hipMalloc(&ds, sizeof(double) * std::max<size_t>(1, ns)));
-    hipError_t e = hipMalloc(&dout, sizeof(double) * std::max<size_t>(1, no));
-    if (e != hipSuccess) { hipFree(ds); return fail(SFH_ERR_HIP, "alloc: %s", hipGetErrorString(e)); }
-    int rc = SFH_OK;
-    e = hipMemcpy(ds, states, sizeof(double) * ns, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dout, out, sizeof(double) * no, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        rc = sfh_batch_compute(b, o, ds, dout, nullptr);
-        if (rc == SFH_OK) e = hipMemcpy(out, dout, sizeof(double) * no, hipMemcpyDeviceToHost);
+    if (!b->d_hstates) {   // the device copies live with the batch (sizes fixed per batch)
+        HIPCHK(hipMalloc(&b->d_hstates, sizeof(double) * std::max<size_t>(1, ns)));
+        hipError_t ea = hipMalloc(&b->d_hout, sizeof(double) * std::max<size_t>(1, no));
+        if (ea != hipSuccess) {
+            hipFree(b->d_hstates);
+            b->d_hstates = nullptr;
+            return fail(SFH_ERR_HIP, "alloc: %s", hipGetErrorString(ea));
+        }
     }
-    hipFree(ds);
-    hipFree(dout);
+    int rc = SFH_OK;
+    hipError_t e = hipMemcpy(b->d_hstates, states, sizeof(double) * ns, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_hout, out, sizeof(double) * no, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = sfh_batch_compute(b, o, b->d_hstates, b->d_hout, nullptr);
+        if (rc == SFH_OK) e = hipMemcpy(out, b->d_hout, sizeof(double) * no, hipMemcpyDeviceToHost);
+    }
     if (e != hipSuccess) return fail(SFH_ERR_HIP, "batch host compute: %s", hipGetErrorString(e));
     return rc;
 }
