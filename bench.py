@@ -81,7 +81,7 @@ def parity_sample(w, states, grid, out_gpu: np.ndarray, n: int = 96) -> dict:
     sys.path.insert(0, str(ROOT / "oracle"))
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle as O
-    from util import bound_ratio
+    from util import bound_ratio, component_errors
     oo = O.Ocean(wave_data=grid, wave_n=w.sea.wave_n, channels=1, lengths=[w.sea.wave_len])
     r, x = [], []
     for i in range(n):
@@ -98,7 +98,8 @@ def parity_sample(w, states, grid, out_gpu: np.ndarray, n: int = 96) -> dict:
     pos_ok = bool(np.array_equal(out_gpu[:n, 21], r[:, 21]) and np.array_equal(out_gpu[:n, 20], r[:, 20]))
     return {"bodies": n, "positions_and_masks_equal": pos_ok,
             "bound_ratio_max": {k: round(float(v), 4) for k, v in worst.items()},
-            "norm_rel_err_max": {k: float(f"{v:.3e}") for k, v in plain.items()}}
+            "norm_rel_err_max": {k: float(f"{v:.3e}") for k, v in plain.items()},
+            "component_err_max": component_errors(out_gpu[:n], r)}
 
 
 def measured_traffic():

@@ -121,8 +121,10 @@ int sfh_forces_submerged(sfh_mesh mesh, sfh_ocean ocean,
                          double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3]);
 
 /* Replaces: SolidEntity::ComputeAerodynamicForces(mesh, atm, ...) (SolidEntity.h:180,
- * SolidEntity.cpp:1865-1918): every face, quadratic drag of the normal relative
- * velocity (no skin friction), times 0.5 rho_gas.  The atmosphere is an
+ * SolidEntity.cpp:1865-1918): every face whose normal relative velocity vn
+ * points into it adds the reference's `vn * vn.length() * A`
+ * (SolidEntity.cpp:1913), where glm's vec3::length() is the component count 3,
+ * so 3 A vn (no skin friction); the sums times 0.5 rho_gas.  The atmosphere is an
  * sfh_ocean made with the gas density, flat, with every wind field added
  * enabled and sfh_ocean_enable_currents(h, 1): Atmosphere::GetFluidVelocity
  * (Atmosphere.cpp:146-152) sums all of them.  mesh == NULL zeroes the outputs

@@ -328,7 +328,12 @@ void orc_compound_forces_64(const orc_body* comp, const orc_part* parts, uint32_
 
 /* ------------------------------------------------------------------ */
 /* SolidEntity::ComputeAerodynamicForces static (SolidEntity.cpp:1866-1925) */
+/* The face term is the reference's `vn * vn.length() * A`                */
+/* (SolidEntity.cpp:1913) on a glm::vec3: glm's member length() is the    */
+/* GLSL one and returns the component count, 3 (the Euclidean norm is the  */
+/* free function glm::length), so q = 3 A vn.                              */
 /* ------------------------------------------------------------------ */
+#define ORC_GLM_VEC3_LENGTH 3.0
 
 void orc_aero_64(const orc_mesh* mesh, double gas_density, const orc_current* wind, uint32_t nwind,
                  const double T_CG[16], const double T_C[16],
@@ -366,7 +371,7 @@ void orc_aero_64(const orc_mesh* mesh, double gas_density, const orc_current* wi
         v3_64 vc = sub_64(mk_64(u[0], u[1], u[2]), add_64(v, cross_64(omega, r)));
         v3_64 vn = mul_64(fn1, dot_64(vc, fn1));
         if (dot_64(fn1, vn) < (double)-1e-12f) {
-            v3_64 q = mul_64(mul_64(vn, sqrt(len2_64(vn))), Ar);
+            v3_64 q = mul_64(mul_64(vn, ORC_GLM_VEC3_LENGTH), Ar);
             v3_64 t = cross_64(r, q);
             F = add_64(F, q);
             T = add_64(T, t);

@@ -159,12 +159,16 @@ SFH_HD void face_submerged(const FaceBody& B, f3 p1, f3 p2, f3 p3, AccT& acc)
     face_submerged(B, p1, p2, p3, NoFlow(), acc);
 }
 
-// Aerodynamic face (SolidEntity.cpp:1884-1912): quadratic drag of the normal
-// velocity only, vn = (vc.n1) n1, applied when n1.vn < -1e-12:
-// q = vn |vn| A.  With vm = -vc and s = vm.n1: vn = -s n1, the condition is
-// s > 1e-12 and q = n1 (-s |s| A).  Accumulated in the quadratic-drag slot;
-// there is no skin friction.  The wind (the atmosphere's velocity fields) is
-// sampled at the centroid as for the currents.
+// Aerodynamic face (SolidEntity.cpp:1884-1912): drag of the normal velocity
+// only, vn = (vc.n1) n1, applied when n1.vn < -1e-12, with the reference's
+// integrand `vn * vn.length() * A` (SolidEntity.cpp:1913).  vn is a glm::vec3
+// and glm's member length() is the GLSL one, the number of components (3),
+// not the Euclidean norm (glm::length(vn)), so the term is q = 3 A vn: linear
+// in the normal velocity.  With vm = -vc and s = vm.n1: vn = -s n1, the
+// condition is s > 1e-12 and q = n1 (-3 s A).  Accumulated in the
+// quadratic-drag slot; there is no skin friction.  The wind (the atmosphere's
+// velocity fields) is sampled at the centroid as for the currents.
+#define SFH_GLM_VEC3_LENGTH 3.0f
 template <class Flow, class AccT>
 SFH_HD void face_aero(const FaceBody& B, f3 p1, f3 p2, f3 p3, const Flow& flow, AccT& acc)
 {
@@ -183,7 +187,7 @@ SFH_HD void face_aero(const FaceBody& B, f3 p1, f3 p2, f3 p3, const Flow& flow, 
                        ((double)p1.z + (double)p2.z + (double)p3.z) / 3.0);
     float s = dot(vm, n1);
     if (s > -SFH_APPROACH_VN) {
-        f3 q = n1 * (-(s * fabsf(s) * A));
+        f3 q = n1 * (-(s * SFH_GLM_VEC3_LENGTH * A));
         add3(acc.Fdq, q);
         add3(acc.Mdq, cross(c, q));
     }

@@ -16,7 +16,7 @@ from stonefish_amd import meshes as M
 from stonefish_amd import transforms as T
 from stonefish_amd import waves as W
 from stonefish_amd import workloads as WL
-from util import NAMES, bound_ratio
+from util import NAMES, bound_ratio, component_errors, format_component_table
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-6
@@ -31,6 +31,7 @@ def assert_bound(gpu, ref, extras, what="", names=NAMES, Cd=None, Cf=None):
     worst, plain = bound_ratio(gpu, ref, extras, names=names, Cd=Cd, Cf=Cf)
     log(f"{what}: bound ratio " + ", ".join(f"{k} {v:.2f}" for k, v in worst.items())
         + " | plain norm-rel " + ", ".join(f"{k} {v:.1e}" for k, v in plain.items()))
+    log(format_component_table(component_errors(gpu, ref, names=names)))
     for k, v in worst.items():
         assert v <= 1.0, f"{what}: {k} error exceeds the bound by {v:.2f}x"
 

@@ -539,9 +539,14 @@ def test_body_in_uniform_pipe_core_equals_moving_body():
 
 def test_aero_translating_box_closed_form():
     """A box of half extents (hx, hy, hz) in any orientation translating at u
-    through still air: only the upwind faces (n.u_hat > 0) act, each with
-    q = vn |vn| A, vn = (vc.n) n, vc = -u: F = -0.5 rho |u|^2 sum A_i (n_i.u_hat)^2 n_i
-    and no torque about the centre (SolidEntity.cpp:1865-1918)."""
+    through still air: only the upwind faces (n.u_hat > 0) act.  The
+    reference's term is `vn * vn.length() * A` (SolidEntity.cpp:1913) with
+    vn = (vc.n) n, vc = -u, on a glm::vec3, whose member length() is the
+    component count 3 (GLSL semantics; the norm is the free glm::length), so
+    F = 0.5 rho sum 3 A_i (vc.n_i) n_i = -1.5 rho |u| sum A_i (n_i.u_hat) n_i:
+    linear in the speed, not quadratic, and no torque about the centre
+    (SolidEntity.cpp:1865-1918).  The quadratic law -0.5 rho |u|^2 sum
+    A_i (n_i.u_hat)^2 n_i (the norm read of length()) is checked to differ."""
     hx, hy, hz = (float(np.float32(x)) for x in (0.7, 0.4, 0.2))
     m = O.Mesh.from_tri(M.box(hx, hy, hz, 4))
     rng = np.random.default_rng(12)
@@ -551,17 +556,39 @@ def test_aero_translating_box_closed_form():
         R = np.array(Tcg).reshape(4, 4).T[:3, :3]
         u = rng.uniform(-3, 3, 3)
         F, T_ = O.aero(m, rho, [], Tcg, Tcg, u, [0, 0, 0])
-        uh = u / np.linalg.norm(u)
+        U = np.linalg.norm(u)
+        uh = u / U
         areas = np.array([4 * hy * hz, 4 * hx * hz, 4 * hx * hy])
         expect = np.zeros(3)
+        quad = np.zeros(3)
         for k in range(3):
             for s in (1.0, -1.0):
                 n = s * R[:, k]
                 c = n @ uh
                 if c > 0:
-                    expect += -0.5 * rho * (u @ u) * areas[k] * c * c * n
+                    expect += -1.5 * rho * U * areas[k] * c * n
+                    quad += -0.5 * rho * U * U * areas[k] * c * c * n
         np.testing.assert_allclose(F, expect, rtol=1e-9, atol=1e-12)
         np.testing.assert_allclose(T_, 0, atol=1e-9)
+        assert np.linalg.norm(F - quad) > 1e-3 * np.linalg.norm(F)
+        # linear law: twice the speed, twice the force
+        F2, _ = O.aero(m, rho, [], Tcg, Tcg, 2.0 * u, [0, 0, 0])
+        np.testing.assert_allclose(F2, 2.0 * F, rtol=1e-12, atol=1e-12)
+
+
+def test_aero_face_term_is_three_area_normal_velocity():
+    """One triangle in the x-y plane (unit normal +z, area 0.5) moving down
+    at speed s (vc = +s z, vc.n = s > 0: not upwind, no force) and up at
+    speed s (vc.n = -s < 0): q = 3 A vn = (0, 0, -1.5 s), Fda = 0.5 rho q.
+    Pins the factor 3 of glm's vec3::length() and its sign."""
+    m = O.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    T = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, -10.0, 1.0]
+    rho = 1.2
+    for s in (0.5, 2.0):
+        F, _ = O.aero(m, rho, [], T, T, [0, 0, -s], [0, 0, 0])
+        np.testing.assert_allclose(F, 0.0, atol=0)
+        F, _ = O.aero(m, rho, [], T, T, [0, 0, s], [0, 0, 0])
+        np.testing.assert_allclose(F, [0.0, 0.0, -0.5 * rho * 3.0 * 0.5 * s], rtol=1e-12, atol=1e-15)
 
 
 def test_aero_uniform_wind_equals_moving_body():

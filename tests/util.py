@@ -75,3 +75,67 @@ def bound_ratio(gpu, ref, extras, names=NAMES, Cd=None, Cf=None, rho=1000.0):
         worst[name] = r_worst
         plain[name] = p_worst
     return worst, plain
+
+
+TOTALS = {"F": (slice(0, 3), slice(6, 9), slice(12, 15)), "T": (slice(3, 6), slice(9, 12), slice(15, 18))}
+
+
+def component_errors(gpu, ref, names=NAMES, floor_frac=1e-3):
+    """Per-component error table of the error statement, worst over a batch.
+
+    For every F/T vector x (the six outputs, and the totals F = Fb + Fdq + Fdf,
+    T = Tb + Tdq + Tdf over the slots the reference wrote) and each component
+    k: |dx_k| / |x64_k| where |x64_k| >= floor_frac |x64|_2; a component below
+    that floor gets |dx_k| / |x64|_2 instead (marked "abs").  Returns
+    {name: {"rel": [x, y, z], "abs": [x, y, z], "norm_rel": worst |dx|/|x64|}}
+    with None where no body had a component of that kind."""
+    gpu, ref = np.atleast_2d(gpu), np.atleast_2d(ref)
+    table = {}
+
+    def vecs(i, name):
+        if name in VEC:
+            j = NAMES.index(name)
+            if not (int(ref[i, 20]) >> j) & 1:
+                return None
+            return gpu[i, VEC[name]], ref[i, VEC[name]]
+        g = np.zeros(3)
+        r = np.zeros(3)
+        for j, sl in zip((0, 2, 4) if name == "F" else (1, 3, 5), TOTALS[name]):
+            if (int(ref[i, 20]) >> j) & 1:
+                g = g + gpu[i, sl]
+                r = r + ref[i, sl]
+        return g, r
+
+    for name in list(names) + ["F", "T"]:
+        rel = [None, None, None]
+        ab = [None, None, None]
+        nr = None
+        for i in range(gpu.shape[0]):
+            v = vecs(i, name)
+            if v is None:
+                continue
+            g, r = v
+            n = float(np.linalg.norm(r))
+            if n == 0.0:
+                continue
+            d = np.abs(g - r)
+            nr = max(nr or 0.0, float(np.linalg.norm(g - r)) / n)
+            for k in range(3):
+                if abs(r[k]) >= floor_frac * n:
+                    rel[k] = max(rel[k] or 0.0, float(d[k] / abs(r[k])))
+                else:
+                    ab[k] = max(ab[k] or 0.0, float(d[k] / n))
+        table[name] = {"rel": [None if x is None else float(f"{x:.2e}") for x in rel],
+                       "abs": [None if x is None else float(f"{x:.2e}") for x in ab],
+                       "norm_rel": None if nr is None else float(f"{nr:.2e}")}
+    return table
+
+
+def format_component_table(table) -> str:
+    """The component table as text lines (for test logs)."""
+    lines = ["vector  norm_rel   |dx|/|x_k| (x, y, z)            |dx_k|/|x| below the 1e-3 floor"]
+    for name, t in table.items():
+        f = lambda v: "    -    " if v is None else f"{v:9.2e}"
+        lines.append(f"{name:6s} {f(t['norm_rel'])}  {' '.join(f(v) for v in t['rel'])}   "
+                     f"{' '.join(f(v) for v in t['abs'])}")
+    return "\n".join(lines)
