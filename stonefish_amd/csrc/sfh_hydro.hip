@@ -159,6 +159,25 @@ struct OceanK {
 // device helpers
 // ---------------------------------------------------------------------------
 
+// Global-memory views of pointers the kernels read out of structures in
+// memory.  Through a generic pointer the compiler issues flat loads, which
+// count against the scalar-load counter as well as the vector-load counter,
+// so waiting for a scalar load also waits for every flat load in flight —
+// including the next face's vertices the face passes fetch ahead of use.
+// Global loads are counted by the vector counter alone.
+#define SFH_GLOBAL __attribute__((address_space(1)))
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ const SFH_GLOBAL T* gptr(const T* p)
+{
+    return (const SFH_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ SFH_GLOBAL T* gptr_w(T* p)
+{
+    return (SFH_GLOBAL T*)p;
+}
+
 __device__ __forceinline__ int wrap_n(int i, int n)
 {
     // i = k + floor(u) with k in [0, n) and |floor(u)| below n for any body
@@ -235,6 +254,66 @@ __device__ __forceinline__ double depth_centroid(const BodyDyn& D, const OceanK&
     }
     return d;
 }
+
+// depth_centroid with the body's wave constants hoisted out of the face loop
+// for a chunk: NL layers known at compile time (0, 1, 2), their constants
+// read once into registers; NL < 0 reads the layer count at run time and
+// defers to depth_centroid.  Same arithmetic as depth_centroid.
+template <int NL>
+struct WaveQ {
+    static constexpr int M = NL > 0 ? NL : 1;
+    double tz, rz[3];
+    float fu[M], fv[M], ru[M][3], rv[M][3];
+    int ku[M], kv[M];
+    const SFH_GLOBAL f32x4* quad[M];
+    int n, nmask;
+    const BodyDyn* D;
+    const OceanK* oc;
+
+    __device__ __forceinline__ void load(const BodyDyn& Dy, const OceanK& o)
+    {
+        D = &Dy;
+        oc = &o;
+        if (NL < 0) return;
+        tz = Dy.tz;
+        for (int k = 0; k < 3; ++k) rz[k] = Dy.rz[k];
+        n = o.n;
+        nmask = o.nmask;
+#pragma unroll
+        for (int l = 0; l < (NL > 0 ? NL : 0); ++l) {
+            const LayerK& L = Dy.layer[l];
+            fu[l] = L.fu32;
+            fv[l] = L.fv32;
+            for (int k = 0; k < 3; ++k) { ru[l][k] = L.ru32[k]; rv[l][k] = L.rv32[k]; }
+            ku[l] = L.ku;
+            kv[l] = L.kv;
+            quad[l] = (const SFH_GLOBAL f32x4*)o.quad[l];
+        }
+    }
+
+    __device__ __forceinline__ double depth(f3 p) const
+    {
+        if (NL < 0) return depth_centroid(*D, *oc, p);
+        double d = tz + rz[0] * (double)p.x + rz[1] * (double)p.y + rz[2] * (double)p.z;
+#pragma unroll
+        for (int l = 0; l < (NL > 0 ? NL : 0); ++l) {
+            const float u = fu[l] + ru[l][0] * p.x + ru[l][1] * p.y + ru[l][2] * p.z;
+            const float v = fv[l] + rv[l][0] * p.x + rv[l][1] * p.y + rv[l][2] * p.z;
+            const float fi = floorf(u), fj = floorf(v);
+            int i, j;
+            if (nmask >= 0) {
+                i = (ku[l] + (int)fi) & nmask;
+                j = (kv[l] + (int)fj) & nmask;
+            } else {
+                i = wrap_n(ku[l] + (int)fi, n);
+                j = wrap_n(kv[l] + (int)fj, n);
+            }
+            const f32x4 t = quad[l][(size_t)j * n + i];
+            d += texel_mix((double)(u - fi), (double)(v - fj), make_float4(t.x, t.y, t.z, t.w));
+        }
+        return d;
+    }
+};
 
 // floor(u) mod n in [0, n) for a texel coordinate floor fu.
 __device__ __forceinline__ int wrap_texel(double fu, const OceanK& oc)
@@ -704,9 +783,11 @@ __global__ __launch_bounds__(256) void k_depths(const uint32_t* __restrict__ lis
         const BodyStatic& S = st[b];
         const uint32_t v0 = __builtin_amdgcn_readfirstlane(ch.first);
         const uint32_t n = __builtin_amdgcn_readfirstlane(ch.count);
+        const SFH_GLOBAL float* verts = gptr(S.verts);
+        SFH_GLOBAL float* dout = gptr_w(depth) + S.dep_off;
         for (uint32_t v = lane; v < n; v += 64) {
-            const float* p = S.verts + 3 * (size_t)(v0 + v);
-            depth[S.dep_off + v0 + v] = (float)depth64(D, oc, p[0], p[1], p[2]);
+            const SFH_GLOBAL float* p = verts + 3 * (size_t)(v0 + v);
+            dout[v0 + v] = (float)depth64(D, oc, p[0], p[1], p[2]);
         }
     }
 }
@@ -770,9 +851,9 @@ __device__ __forceinline__ int xreduce24_slot(int lane)
     return (q < 23 && !(lane & 1)) ? q : -1;
 }
 
-__device__ __forceinline__ f3 vert(const float* __restrict__ verts, uint32_t i)
+__device__ __forceinline__ f3 vert(const SFH_GLOBAL float* __restrict__ verts, uint32_t i)
 {
-    const float* v = verts + 3 * (size_t)i;
+    const SFH_GLOBAL float* v = verts + 3 * (size_t)i;
     return mk3(v[0], v[1], v[2]);
 }
 
@@ -838,8 +919,8 @@ __global__ __launch_bounds__(256, NU ? 1 : 8) void k_faces_sub(const uint32_t* _
         const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
         const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
         const BodyStatic& S = st[b];
-        const float* __restrict__ verts = S.verts;
-        const uint32_t* __restrict__ faces = S.faces + 3 * (size_t)f0;
+        const SFH_GLOBAL float* __restrict__ verts = gptr(S.verts);
+        const SFH_GLOBAL uint32_t* __restrict__ faces = gptr(S.faces) + 3 * (size_t)f0;
         const FaceBody fb = dyn[b].fb;
         const CurFlow flow{&dyn[b].cur};
         AccDrag<float> acc;
@@ -880,20 +961,20 @@ __global__ __launch_bounds__(256, NU ? 1 : 8) void k_faces_sub(const uint32_t* _
         }
         xreduce12(vals, lane);
         const int q = xreduce12_slot(lane);
-        if (q >= 0) partials[(size_t)c * kChunkDoubles + q] = vals[0];
+        if (q >= 0) gptr_w(partials)[(size_t)c * kChunkDoubles + q] = vals[0];
     }
 }
 
 // Contributions of a wet (possibly clipped) face: pressure buoyancy at the
 // wet centroid when the surface has waves (1649-1658), drag, wetted area.
-template <bool NU>
-__device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, const BodyDyn& D, const OceanK& oc,
+template <bool NU, int NL>
+__device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, const BodyDyn& D, const WaveQ<NL>& wq,
                                             bool wave_buoy, bool damping, Acc<float>& acc)
 {
     if (!wt.ok) return;
     f3 cpt = wt.base + wt.off;
-    if (wave_buoy) {
-        const float dc = (float)depth_centroid(D, oc, cpt);
+    if (NL != 0 && wave_buoy) {
+        const float dc = (float)wq.depth(cpt);
         f3 fbi = wt.n1 * (-wt.A * dc);
         add3(acc.Fb, fbi);
         add3(acc.Mb, cross(cpt, fbi));
@@ -917,7 +998,7 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
 // cuts (clip cases A-F) are queued in LDS by each wavefront and processed
 // together afterwards, 64 at a time, so the clip code runs once per 64 cut
 // faces rather than on every pass over the chunk.
-template <int K, bool NU>
+template <int K, bool NU, int NL>
 __global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
                                                      const Chunk* __restrict__ chunks,
@@ -940,13 +1021,15 @@ __global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t*
         const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
         const BodyDyn& D = dyn[b];
         const BodyStatic& S = st[b];
-        const float* __restrict__ verts = S.verts;
-        const uint32_t* __restrict__ faces = S.faces + 3 * (size_t)f0;
+        const SFH_GLOBAL float* __restrict__ verts = gptr(S.verts);
+        const SFH_GLOBAL uint32_t* __restrict__ faces = gptr(S.faces) + 3 * (size_t)f0;
         const FaceBody fb = D.fb;
         const uint32_t flags = D.flags;
         const bool damping = flags & BF_DAMPING;
         const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
-        const float* __restrict__ dep = depth + S.dep_off;
+        const SFH_GLOBAL float* __restrict__ dep = gptr(depth) + S.dep_off;
+        WaveQ<NL> wq;
+        wq.load(D, oc);
         Acc<float> acc;
         acc_zero(acc);
         const uint32_t flast = nfc - 1;
@@ -965,7 +1048,7 @@ __global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t*
             const bool allwet = !(d0 < 0.f) && !(d1 < 0.f) && !(d2 < 0.f);
             const bool cut = valid && !allwet && !(d0 < 0.f && d1 < 0.f && d2 < 0.f);
             if (valid && allwet)
-                wet_contrib<NU>(face_allwet(fb, p1, p2, p3, acc), fb, D, oc, wave_buoy, damping, acc);
+                wet_contrib<NU, NL>(face_allwet(fb, p1, p2, p3, acc), fb, D, wq, wave_buoy, damping, acc);
             const uint64_t m = __ballot(cut);
             if (cut) cutq[wl][ncut + (uint32_t)__popcll(m & below)] = (uint16_t)(64u * j + lane);
             ncut += (uint32_t)__popcll(m);
@@ -982,7 +1065,7 @@ __global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t*
                 const uint32_t i0 = faces[3 * fo], i1 = faces[3 * fo + 1], i2 = faces[3 * fo + 2];
                 const Wet wt = face_clip(fb, vert(verts, i0), vert(verts, i1), vert(verts, i2), dep[i0], dep[i1],
                                          dep[i2], acc);
-                wet_contrib<NU>(wt, fb, D, oc, wave_buoy, damping, acc);
+                wet_contrib<NU, NL>(wt, fb, D, wq, wave_buoy, damping, acc);
             }
         }
         __builtin_amdgcn_wave_barrier();   // the queue is rewritten by the next chunk
@@ -1000,7 +1083,7 @@ __global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t*
         vals[23] = 0.0;
         xreduce24(vals, lane);
         const int q = xreduce24_slot(lane);
-        if (q >= 0) partials[(size_t)c * kChunkDoubles + q] = vals[0];
+        if (q >= 0) gptr_w(partials)[(size_t)c * kChunkDoubles + q] = vals[0];
     }
 }
 
@@ -2158,11 +2241,21 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
     if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
     uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
     uint32_t blocks = (waves + 3) / 4;
+#define LAUNCH_CROSS(KK, NU, NL)                                                                          \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_cross<KK, NU, NL>), dim3(blocks), dim3(256), 0, s, list_cross,  \
+                       counts + 1, b->d_chunks, b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);
 #define LAUNCH_KN(KK, NU)                                                                                 \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, NU>), dim3(blocks), dim3(256), 0, s, list_sub, counts,  \
                        b->d_chunks, b->d_static, b->d_dyn, b->d_partials);                                    \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_cross<KK, NU>), dim3(blocks), dim3(256), 0, s, list_cross,      \
-                       counts + 1, b->d_chunks, b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);
+    if (NU || oc.nlayers > 2) {                                                                           \
+        LAUNCH_CROSS(KK, NU, -1)                                                                          \
+    } else if (oc.nlayers == 0) {                                                                         \
+        LAUNCH_CROSS(KK, false, 0)                                                                        \
+    } else if (oc.nlayers == 1) {                                                                         \
+        LAUNCH_CROSS(KK, false, 1)                                                                        \
+    } else {                                                                                              \
+        LAUNCH_CROSS(KK, false, 2)                                                                        \
+    }
 #define LAUNCH_K(KK)                                                                                 \
     case KK:                                                                                         \
         if (b->aero) {                                                                               \
@@ -2184,6 +2277,7 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
     }
 #undef LAUNCH_K
 #undef LAUNCH_KN
+#undef LAUNCH_CROSS
     return SFH_OK;
 }
 
@@ -2399,85 +2493,227 @@ extern "C" int sfh_batch_kernel_ms(sfh_batch b, double* face_ms, double* total_m
     return SFH_OK;
 }
 
+// ---- per-body evaluations ------------------------------------------------------
+
+// One per-body evaluation (a single-body batch) needs, beyond the mesh: the
+// body's static record and per-step constants, its chunk lists, the work
+// lists, the chunk partials and the vertex depths.  They are carved out of
+// one device buffer sized for meshes of up to F faces and V vertices: a
+// workspace of the per-body statics (device memory from hipMalloc, grown on
+// demand) or the no-alloc entry point's file-scope array.  The static
+// record and chunk lists are written by a kernel from its arguments, so a
+// call needs no host-to-device copy of them.
+static constexpr uint32_t kSingleK = 8;   // faces per lane of per-body calls
+
+struct SingleHead {
+    BodyStatic st;
+    BodyDyn dyn;
+    BodyChunks bch;
+    uint4 boff;
+    uint4 scan;
+    uint32_t sidx;
+    int8_t bpos;
+};
+
+__host__ __device__ constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+__host__ __device__ constexpr uint32_t single_nchunks(uint32_t F) { return (F + 64u * kSingleK - 1) / (64u * kSingleK); }
+__host__ __device__ constexpr uint32_t single_nvchunks(uint32_t V) { return (V + 511u) / 512u; }
+
+// Byte offsets of the parts in a buffer for (F, V): head, lists, chunks,
+// vchunks, partials, depth, end.
+struct SingleLayout {
+    size_t lists, chunks, vchunks, partials, depth, end;
+};
+
+__host__ __device__ constexpr SingleLayout single_layout(uint32_t F, uint32_t V)
+{
+    const size_t nc = single_nchunks(F), nvc = single_nvchunks(V);
+    const size_t lists = align256(sizeof(SingleHead));
+    const size_t chunks = align256(lists + sizeof(uint32_t) * (2 * nc + nvc + 8));
+    const size_t vchunks = align256(chunks + sizeof(Chunk) * (nc ? nc : 1));
+    const size_t partials = align256(vchunks + sizeof(Chunk) * (nvc ? nvc : 1));
+    const size_t depth = align256(partials + sizeof(double) * kChunkDoubles * (nc ? nc : 1));
+    const size_t end = align256(depth + sizeof(float) * (V ? V : 1));
+    return SingleLayout{lists, chunks, vchunks, partials, depth, end};
+}
+
+// Writes the body's static record and its face and vertex chunk lists.
+__global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H, Chunk* __restrict__ chunks,
+                                                      Chunk* __restrict__ vchunks, BodyStatic st)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t csize = 64u * kSingleK;
+    if (t == 0) {
+        H->st = st;
+        H->bch.chunk0 = 0; H->bch.nchunks = st.nchunks;
+        H->bch.vchunk0 = 0; H->bch.nvchunks = st.nvchunks;
+        H->sidx = 0;
+    }
+    if (t < st.nchunks) {
+        Chunk c;
+        c.body = 0; c.first = t * csize; c.count = min(csize, st.nf - t * csize); c.pad = 0;
+        chunks[t] = c;
+    }
+    if (t < st.nvchunks) {
+        Chunk c;
+        c.body = 0; c.first = t * 512u; c.count = min(512u, st.nv - t * 512u); c.pad = 0;
+        vchunks[t] = c;
+    }
+}
+
+// One evaluation of `m` with `flags` from the 38-double state d_state into
+// the output row d_out, on stream s, in the buffer `buf` laid out for the
+// mesh's (F, V).  No allocation.
+static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags, const sfh_ocean_t* o,
+                         const double* d_state, double* d_out, hipStream_t s, uint32_t waves)
+{
+    const SingleLayout L = single_layout(m->nf, m->nv);
+    SingleHead* H = (SingleHead*)buf;
+    Chunk* chunks = (Chunk*)(buf + L.chunks);
+    Chunk* vchunks = (Chunk*)(buf + L.vchunks);
+    BodyStatic st = make_static(m, flags);
+    st.chunk0 = 0;
+    st.nchunks = single_nchunks(m->nf);
+    st.vchunk0 = 0;
+    st.nvchunks = single_nvchunks(m->nv);
+    st.sidx = 0;
+    st.dep_off = 0;
+    const uint32_t nt = std::max(std::max(st.nchunks, st.nvchunks), 1u);
+    hipLaunchKernelGGL(k_single_setup, dim3((nt + 255) / 256), dim3(256), 0, s, H, chunks, vchunks, st);
+    // a batch view of the buffer: no allocation (empty vectors, borrowed pointers)
+    sfh_batch_t b;
+    b.n = 1;
+    b.nunits = 1;
+    b.aero = (flags & BF_MODE_AERO) != 0;
+    b.nchunks = st.nchunks;
+    b.nvchunks = st.nvchunks;
+    b.nfaces = m->nf;
+    b.nverts = m->nv;
+    b.K = kSingleK;
+    b.waves = waves;
+    b.d_static = &H->st;
+    b.d_sidx = &H->sidx;
+    b.d_chunks = chunks;
+    b.d_vchunks = vchunks;
+    b.d_depth = (float*)(buf + L.depth);
+    b.d_bpos = &H->bpos;
+    b.d_bch = &H->bch;
+    b.d_boff = &H->boff;
+    b.d_scan = &H->scan;
+    b.d_lists = (uint32_t*)(buf + L.lists);
+    b.d_dyn = &H->dyn;
+    b.d_partials = (double*)(buf + L.partials);
+    return batch_launch(&b, o, d_state, 1, d_out, s, false);
+}
+
 // ---- per-body reference-shaped calls ------------------------------------------
 
-// One reusable single-body workspace; the reference's statics are called from
-// pool threads, so calls are serialised here.
-static std::mutex g_single_mu;
-struct Single {
-    sfh_batch_t* b = nullptr;
-    uint64_t mesh_id = 0;
-    uint32_t flags = 0;
-    double* d_state = nullptr;
-    double* d_out = nullptr;
-    double* d_aux = nullptr;
+// The reference calls its statics from pool threads, one body per task, with
+// any mesh.  Each call takes a workspace from a per-device pool (one per
+// concurrently calling thread): its own stream, pinned host staging for the
+// state and outputs, and a device buffer grown to the largest mesh it has
+// evaluated.  Calls on different workspaces run concurrently; nothing is
+// rebuilt when consecutive calls use different meshes.
+struct Workspace {
     int device = -1;
+    hipStream_t stream = nullptr;
+    double* h_io = nullptr;       // pinned: state (38) | outputs (24) | aux (64)
+    double* d_io = nullptr;       // device, same layout
+    unsigned char* d_buf = nullptr;
+    size_t cap = 0;               // bytes of d_buf
+    uint32_t waves = 0;
 };
-static Single g_single;
+static constexpr size_t kIoDoubles = 128;
 
-static int single_alloc()
+static std::mutex g_ws_mu;
+static std::vector<Workspace*> g_ws_free[64];
+
+static void ws_destroy(Workspace* w)
 {
-    Single& S = g_single;
+    if (!w) return;
+    if (w->stream) hipStreamDestroy(w->stream);
+    if (w->h_io) hipHostFree(w->h_io);
+    hipFree(w->d_io);
+    hipFree(w->d_buf);
+    delete w;
+}
+
+static int ws_acquire(Workspace** out)
+{
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
-    if (S.device != dev) {   // workspace belongs to another device: start afresh
-        S.b = nullptr;
-        S.d_state = S.d_out = S.d_aux = nullptr;
-        S.device = dev;
+    if (dev < 0 || dev >= 64) return fail(SFH_ERR_UNSUPPORTED, "device %d beyond the 64 the per-body pool tracks", dev);
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        if (!g_ws_free[dev].empty()) {
+            *out = g_ws_free[dev].back();
+            g_ws_free[dev].pop_back();
+            return SFH_OK;
+        }
     }
-    if (!S.d_state) {
-        HIPCHK(hipMalloc(&S.d_state, sizeof(double) * 64));
-        HIPCHK(hipMalloc(&S.d_out, sizeof(double) * SFH_OUT_DOUBLES));
-        HIPCHK(hipMalloc(&S.d_aux, sizeof(double) * 32));
+    Workspace* w = new Workspace();
+    w->device = dev;
+    w->waves = default_waves();
+    hipError_t e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&w->h_io, sizeof(double) * kIoDoubles, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&w->d_io, sizeof(double) * kIoDoubles);
+    if (e != hipSuccess) {
+        ws_destroy(w);
+        return fail(SFH_ERR_HIP, "per-body workspace: %s", hipGetErrorString(e));
     }
+    *out = w;
     return SFH_OK;
 }
 
-static int single_prepare(sfh_mesh_t* m, uint32_t flags)
+static void ws_release(Workspace* w)
 {
-    Single& S = g_single;
-    int rc = single_alloc();
-    if (rc != SFH_OK) return rc;
-    int dev = S.device;
-    if (!S.b || S.mesh_id != m->id || S.flags != flags) {
-        if (S.b) { batch_free(S.b); delete S.b; S.b = nullptr; }
-        sfh_batch_t* b = new sfh_batch_t();
-        b->n = 1;
-        b->nunits = 1;
-        b->device = dev;
-        m->refs.fetch_add(1);
-        b->meshes.push_back(m);
-        b->meshes_by_body.push_back(m);
-        b->h_static.push_back(make_static(m, flags));
-        b->nfaces = b->h_static[0].nf;
-        b->aero = flags & BF_MODE_AERO;
-        b->K = 0;
-        int rc = hipMalloc(&b->d_dyn, sizeof(BodyDyn)) == hipSuccess ? SFH_OK : fail(SFH_ERR_HIP, "alloc failed");
-        sfh_batch_opts opts = {8, 0};
-        if (rc == SFH_OK) rc = apply_opts(b, &opts);
-        if (rc != SFH_OK) { batch_free(b); delete b; return rc; }
-        S.b = b;
-        S.mesh_id = m->id;
-        S.flags = flags;
-    }
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_ws_free[w->device].push_back(w);
+}
+
+// Grows the workspace's buffer to the layout of an (F, V) mesh.
+static int ws_reserve(Workspace* w, uint32_t F, uint32_t V)
+{
+    const size_t need = single_layout(F, V).end;
+    if (need <= w->cap) return SFH_OK;
+    const size_t cap = std::max(need, 2 * w->cap);
+    HIPCHK(hipStreamSynchronize(w->stream));
+    hipFree(w->d_buf);
+    w->d_buf = nullptr;
+    w->cap = 0;
+    HIPCHK(hipMalloc(&w->d_buf, cap));
+    w->cap = cap;
     return SFH_OK;
 }
 
-static int single_run(const sfh_ocean_t* o, const double T_CG[16], const double T_C[16], const double v[3],
-                      const double w[3], double* out)
+// Runs one per-body evaluation with host inputs and returns the 24-double
+// output row in out.
+static int single_call(sfh_mesh_t* m, uint32_t flags, const sfh_ocean_t* o, const double T_CG[16],
+                       const double T_C[16], const double v[3], const double w[3], double* out)
 {
-    Single& S = g_single;
-    double st[38];
-    memcpy(st, T_CG, 16 * sizeof(double));
-    memcpy(st + 16, T_C, 16 * sizeof(double));
-    memcpy(st + 32, v, 3 * sizeof(double));
-    memcpy(st + 35, w, 3 * sizeof(double));
-    HIPCHK(hipMemcpy(S.d_state, st, sizeof(st), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(S.d_out, 0, sizeof(double) * SFH_OUT_DOUBLES));
-    int rc = batch_launch(S.b, o, S.d_state, 1, S.d_out, nullptr, false);
+    Workspace* ws = nullptr;
+    int rc = ws_acquire(&ws);
     if (rc != SFH_OK) return rc;
-    HIPCHK(hipMemcpy(out, S.d_out, sizeof(double) * SFH_OUT_DOUBLES, hipMemcpyDeviceToHost));
-    return SFH_OK;
+    rc = ws_reserve(ws, m->nf, m->nv);
+    if (rc == SFH_OK) {
+        double* h = ws->h_io;
+        memcpy(h, T_CG, 16 * sizeof(double));
+        memcpy(h + 16, T_C, 16 * sizeof(double));
+        memcpy(h + 32, v, 3 * sizeof(double));
+        memcpy(h + 35, w, 3 * sizeof(double));
+        hipError_t e = hipMemcpyAsync(ws->d_io, h, 38 * sizeof(double), hipMemcpyHostToDevice, ws->stream);
+        if (e != hipSuccess) rc = fail(SFH_ERR_HIP, "state upload: %s", hipGetErrorString(e));
+        if (rc == SFH_OK) rc = single_launch(ws->d_buf, m, flags, o, ws->d_io, ws->d_io + 38, ws->stream, ws->waves);
+        if (rc == SFH_OK) {
+            e = hipMemcpyAsync(h + 38, ws->d_io + 38, SFH_OUT_DOUBLES * sizeof(double), hipMemcpyDeviceToHost,
+                               ws->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ws->stream);
+            if (e != hipSuccess) rc = fail(SFH_ERR_HIP, "per-body call: %s", hipGetErrorString(e));
+            else memcpy(out, h + 38, SFH_OUT_DOUBLES * sizeof(double));
+        }
+    }
+    ws_release(ws);
+    return rc;
 }
 
 extern "C" int sfh_forces_surface(const sfh_settings* settings, sfh_mesh mesh, sfh_ocean ocean,
@@ -2501,11 +2737,8 @@ extern "C" int sfh_forces_surface(const sfh_settings* settings, sfh_mesh mesh, s
     uint32_t flags = BF_HYDRO | BF_MODE_SURF | BF_EXPLICIT_TC;
     if (settings->damping_forces) flags |= BF_DAMPING;
     if (settings->realistic_buoyancy) flags |= BF_BUOYANT;
-    std::lock_guard<std::mutex> lk(g_single_mu);
-    rc = single_prepare(mesh, flags);
-    if (rc != SFH_OK) return rc;
     double out[SFH_OUT_DOUBLES];
-    rc = single_run(ocean, T_CG, T_C, v, omega, out);
+    rc = single_call(mesh, flags, ocean, T_CG, T_C, v, omega, out);
     if (rc != SFH_OK) return rc;
     uint32_t wm = (uint32_t)out[20];
     if (wm & 1u) memcpy(Fb, out, 24);
@@ -2530,11 +2763,8 @@ extern "C" int sfh_forces_submerged(sfh_mesh mesh, sfh_ocean ocean, const double
         for (int k = 0; k < 3; ++k) Fdq[k] = Tdq[k] = Fdf[k] = Tdf[k] = 0.0;
         return SFH_OK;
     }
-    std::lock_guard<std::mutex> lk(g_single_mu);
-    rc = single_prepare(mesh, BF_HYDRO | BF_MODE_SUBM | BF_EXPLICIT_TC | BF_DAMPING);
-    if (rc != SFH_OK) return rc;
     double out[SFH_OUT_DOUBLES];
-    rc = single_run(ocean, T_CG, T_C, v, omega, out);
+    rc = single_call(mesh, BF_HYDRO | BF_MODE_SUBM | BF_EXPLICIT_TC | BF_DAMPING, ocean, T_CG, T_C, v, omega, out);
     if (rc != SFH_OK) return rc;
     memcpy(Fdq, out + 6, 24);
     memcpy(Tdq, out + 9, 24);
@@ -2553,11 +2783,8 @@ extern "C" int sfh_forces_aero(sfh_mesh mesh, sfh_ocean atmosphere, const double
         for (int k = 0; k < 3; ++k) Fda[k] = Tda[k] = 0.0;
         return SFH_OK;
     }
-    std::lock_guard<std::mutex> lk(g_single_mu);
-    rc = single_prepare(mesh, BF_MODE_AERO | BF_EXPLICIT_TC | BF_DAMPING);
-    if (rc != SFH_OK) return rc;
     double out[SFH_OUT_DOUBLES];
-    rc = single_run(atmosphere, T_CG, T_C, v, omega, out);
+    rc = single_call(mesh, BF_MODE_AERO | BF_EXPLICIT_TC | BF_DAMPING, atmosphere, T_CG, T_C, v, omega, out);
     if (rc != SFH_OK) return rc;
     memcpy(Fda, out + 6, 24);
     memcpy(Tda, out + 9, 24);
@@ -2568,22 +2795,28 @@ extern "C" int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double
                            const double Cd[3], const double Cf[3], const double T_O[16])
 {
     if (!ocean || !Fdq || !Tdq || !Fdf || !Tdf || !Cd || !Cf || !T_O) return fail(SFH_ERR_INVALID, "null argument");
-    std::lock_guard<std::mutex> lk(g_single_mu);
-    int rc = single_alloc();
+    Workspace* ws = nullptr;
+    int rc = ws_acquire(&ws);
     if (rc != SFH_OK) return rc;
-    double in[15], io[12];
-    gl_to_rows(T_O, in, nullptr);
-    memcpy(in + 9, Cd, 24);
-    memcpy(in + 12, Cf, 24);
-    memcpy(io, Fdq, 24); memcpy(io + 3, Tdq, 24); memcpy(io + 6, Fdf, 24); memcpy(io + 9, Tdf, 24);
-    Single& S = g_single;
-    HIPCHK(hipMemcpy(S.d_aux, in, sizeof(in), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(S.d_aux + 16, io, sizeof(io), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_correct, dim3(1), dim3(64), 0, 0, ocean->rho, S.d_aux, S.d_aux + 16);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpy(io, S.d_aux + 16, sizeof(io), hipMemcpyDeviceToHost));
-    memcpy(Fdq, io, 24); memcpy(Tdq, io + 3, 24); memcpy(Fdf, io + 6, 24); memcpy(Tdf, io + 9, 24);
-    return SFH_OK;
+    double* h = ws->h_io + 64;   // aux: in[15] at 0, io[12] at 16
+    gl_to_rows(T_O, h, nullptr);
+    memcpy(h + 9, Cd, 24);
+    memcpy(h + 12, Cf, 24);
+    memcpy(h + 16, Fdq, 24); memcpy(h + 19, Tdq, 24); memcpy(h + 22, Fdf, 24); memcpy(h + 25, Tdf, 24);
+    double* d = ws->d_io + 64;
+    hipError_t e = hipMemcpyAsync(d, h, 28 * sizeof(double), hipMemcpyHostToDevice, ws->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_correct, dim3(1), dim3(64), 0, ws->stream, ocean->rho, d, d + 16);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h + 16, d + 16, 12 * sizeof(double), hipMemcpyDeviceToHost, ws->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ws->stream);
+    if (e != hipSuccess) rc = fail(SFH_ERR_HIP, "correct: %s", hipGetErrorString(e));
+    else {
+        memcpy(Fdq, h + 16, 24); memcpy(Tdq, h + 19, 24); memcpy(Fdf, h + 22, 24); memcpy(Tdf, h + 25, 24);
+    }
+    ws_release(ws);
+    return rc;
 }
 
 // ---- wave grid ----------------------------------------------------------------

@@ -354,3 +354,47 @@ def test_unstaged_chunks(gpu):
         g[s, 18:20] = sv
         r[s], x[s] = O.surface(64, 1, 1, om, oo, Tcg, Tcg, v, w_, full=True)
     assert_bound(g, r, x, "unstaged chunks")
+
+
+def test_statics_from_many_threads_with_mesh_switching(gpu):
+    """The reference calls its statics from pool threads, one body per task,
+    with any mesh.  Calls from 4 host threads, each alternating between three
+    meshes of different sizes, equal the same calls made serially bit for bit
+    (each call takes its own workspace; a mesh switch rebuilds nothing)."""
+    import threading
+    log("case: per-body statics from 4 threads, 3 meshes")
+    ws = [WL.ellipsoid_surface(), WL.capsule_submerged()]
+    meshes = [H.Mesh(ws[0].bodies[0].mesh), H.Mesh(ws[1].bodies[0].mesh), H.Mesh(M.box(0.5, 0.3, 0.2, 3))]
+    go = H.Ocean()
+    rng = np.random.default_rng(4)
+    settings = H.HydrodynamicsSettings(True, True)
+    jobs = []
+    for t in range(4):
+        for k in range(9):
+            q = T.random_unit_quaternions(rng, 1)[0]
+            Tm = T.quat_to_gl(q, np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-0.2, 0.2)]))
+            jobs.append((t, k % 3, np.ascontiguousarray(Tm, dtype=np.float64).reshape(-1), rng.uniform(-2, 2, 3),
+                         rng.uniform(-1, 1, 3)))
+
+    def run(job):
+        _, mi, Tm, v, w = job
+        outs = [np.zeros(3) for _ in range(6)]
+        sv = np.zeros(2)
+        H.ComputeHydrodynamicForcesSurface(settings, meshes[mi], go, Tm, Tm, v, w, *outs, sv)
+        return np.concatenate(outs + [sv])
+
+    serial = [run(j) for j in jobs]
+    par = [None] * len(jobs)
+
+    def worker(t):
+        for i, j in enumerate(jobs):
+            if j[0] == t:
+                par[i] = run(j)
+    th = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for a, b in zip(serial, par):
+        np.testing.assert_array_equal(a, b)
+    assert max(np.abs(a).max() for a in serial) > 0
