@@ -144,6 +144,31 @@ int sfh_correct_aero(int type, const double params[3], const double T_CG2H[16], 
 int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double Fdf[3], double Tdf[3],
                 const double Cd[3], const double Cf[3], const double T_O[16]);
 
+/* ---- per-body evaluation that allocates nothing --------------------------- */
+
+#define SFH_NOALLOC_SURFACE 0    /* as sfh_forces_surface */
+#define SFH_NOALLOC_SUBMERGED 1  /* as sfh_forces_submerged */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Computes what sfh_forces_surface (mode
+ * SFH_NOALLOC_SURFACE, with `settings`) or sfh_forces_submerged (mode
+ * SFH_NOALLOC_SUBMERGED, settings ignored) computes, with the same kernels,
+ * asynchronously on `stream` (a hipStream_t, NULL for the default stream),
+ * from device memory:
+ *   d_state: 38 doubles, T_CG[16] T_C[16] v[3] omega[3] (OpenGL order);
+ *   d_out:   SFH_OUT_DOUBLES doubles laid out as a batch output row; slots
+ *            the reference leaves untouched keep their contents, the
+ *            written mask and position are stored in slots 20 and 21.
+ * Its work buffers live in one file-scope device array of about 2.3 MB per
+ * GPU (held until the process ends), sized for meshes of at most 524,288
+ * faces and 524,288 vertices; a larger mesh returns SFH_ERR_SIZE.  It calls
+ * no hipMalloc, hipFree or allocating routine.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (several streams or threads)
+ * race, so the caller orders them.  mesh must not be NULL. */
+int sfh_forces_noalloc(const sfh_settings* settings, int mode, sfh_mesh mesh, sfh_ocean ocean,
+                       const double* d_state, double* d_out, void* stream);
+
 /* ---- batched body-level evaluation ------------------------------------- */
 
 /* Static description of one body: what SolidEntity::ComputeHydrodynamicForces

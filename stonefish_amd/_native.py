@@ -19,6 +19,8 @@ SFH_ERR_UNSUPPORTED = -5
 
 SFH_STATE_DOUBLES = 22
 SFH_OUT_DOUBLES = 24
+SFH_NOALLOC_SURFACE = 0
+SFH_NOALLOC_SUBMERGED = 1
 
 c_double_p = C.POINTER(C.c_double)
 c_float_p = C.POINTER(C.c_float)
@@ -143,6 +145,8 @@ _PROTOS = {
                                        c_double_p, c_double_p, c_double_p, c_double_p]),
     "sfh_correct": (C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
                               c_double_p]),
+    "sfh_forces_noalloc": (C.c_int, [C.POINTER(sfh_settings), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
     "sfh_batch_create": (C.c_int, [C.POINTER(sfh_body_desc), C.c_uint32, C.POINTER(sfh_batch_opts),
                                    C.POINTER(C.c_void_p)]),
     "sfh_compound_batch_create": (C.c_int, [C.POINTER(sfh_compound_desc), C.c_uint32, C.POINTER(sfh_batch_opts),

@@ -57,6 +57,7 @@ extern "C" const char* sfh_last_error(void) { return g_err; }
 
 // The error path of the host-only translation units of the library (sfh_hull.cpp).
 int sfh_internal_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+int sfh_internal_na_init(void);
 
 // ---------------------------------------------------------------------------
 // device-side data
@@ -1572,7 +1573,7 @@ extern "C" int sfh_init(int device)
     if (e != hipSuccess || count == 0) return fail(SFH_ERR_NODEVICE, "no HIP device");
     if (device < 0 || device >= count) return fail(SFH_ERR_INVALID, "device %d out of range", device);
     HIPCHK(hipSetDevice(device));
-    return SFH_OK;
+    return sfh_internal_na_init();   // this device's no-alloc scratch address
 }
 
 // ---- meshes -----------------------------------------------------------------
@@ -2817,6 +2818,73 @@ extern "C" int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double
     }
     ws_release(ws);
     return rc;
+}
+
+// ---- per-body call that allocates nothing ----------------------------------------
+
+// The buffer of the no-alloc entry point: the per-body layout for meshes of
+// up to kNaMaxFaces faces and kNaMaxVerts vertices, in one file-scope device
+// array.  Each GPU the library runs on holds its own copy until the process
+// ends; every call on a GPU shares it, so calls that overlap on one GPU (two
+// streams, or two host threads) race.
+static constexpr uint32_t kNaMaxFaces = 1u << 19;   // 524,288
+static constexpr uint32_t kNaMaxVerts = 1u << 19;
+static constexpr size_t kNaBytes = single_layout(kNaMaxFaces, kNaMaxVerts).end;
+static_assert(kNaBytes <= (5u << 19), "no-alloc scratch above 2.5 MiB");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+static std::mutex g_na_mu;
+static void* g_na_ptr[64];       // per device: the address of `scratch` (hipGetSymbolAddress)
+static uint32_t g_na_waves[64];  // per device: resident wavefronts of the face passes
+
+// Looks up this device's copy of `scratch` once (with the device current):
+// from sfh_init, or from the first no-alloc call on a device sfh_init did not see.
+static int na_scratch(unsigned char** out, uint32_t* waves)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(SFH_ERR_UNSUPPORTED, "device %d beyond the 64 the no-alloc path tracks", dev);
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(SFH_ERR_HIP, "scratch symbol address: %s", hipGetErrorString(e));
+        g_na_waves[dev] = default_waves();
+        g_na_ptr[dev] = p;
+    }
+    if (out) *out = (unsigned char*)g_na_ptr[dev];
+    if (waves) *waves = g_na_waves[dev];
+    return SFH_OK;
+}
+
+int sfh_internal_na_init(void) { return na_scratch(nullptr, nullptr); }
+
+extern "C" int sfh_forces_noalloc(const sfh_settings* settings, int mode, sfh_mesh mesh, sfh_ocean ocean,
+                                  const double* d_state, double* d_out, void* stream)
+{
+    if (!settings || !mesh || !d_state || !d_out) return fail(SFH_ERR_INVALID, "null argument");
+    if (mode != SFH_NOALLOC_SURFACE && mode != SFH_NOALLOC_SUBMERGED) return fail(SFH_ERR_INVALID, "mode %d", mode);
+    int rc = check_ocean(ocean);
+    if (rc != SFH_OK) return rc;
+    if (mesh->nf > kNaMaxFaces || mesh->nv > kNaMaxVerts)
+        return fail(SFH_ERR_SIZE, "mesh of %u faces / %u vertices exceeds the no-alloc limit of %u / %u", mesh->nf,
+                    mesh->nv, kNaMaxFaces, kNaMaxVerts);
+    unsigned char* buf = nullptr;
+    uint32_t waves = 0;
+    rc = na_scratch(&buf, &waves);
+    if (rc != SFH_OK) return rc;
+    uint32_t flags = BF_HYDRO | BF_EXPLICIT_TC;
+    if (mode == SFH_NOALLOC_SURFACE) {
+        flags |= BF_MODE_SURF;
+        if (settings->damping_forces) flags |= BF_DAMPING;
+        if (settings->realistic_buoyancy) flags |= BF_BUOYANT;
+    } else {
+        flags |= BF_MODE_SUBM | BF_DAMPING;
+    }
+    // the layout for this mesh fits in the array's: every part's offset and
+    // size grows with F and V
+    return single_launch(buf, mesh, flags, ocean, d_state, d_out, (hipStream_t)stream, waves);
 }
 
 // ---- wave grid ----------------------------------------------------------------

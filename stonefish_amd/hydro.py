@@ -193,6 +193,18 @@ def ComputeHydrodynamicForcesSubmerged(mesh: Mesh | None, ocn: Ocean, T_CG, T_C,
                                      _dp(v), _dp(omega), _dp(Fdq), _dp(Tdq), _dp(Fdf), _dp(Tdf)))
 
 
+def forces_noalloc(settings: HydrodynamicsSettings, mode: int, mesh: Mesh, ocn: Ocean, d_state: int, d_out: int,
+                   stream: int = 0) -> None:
+    """sfh_forces_noalloc: the per-body Surface (mode N.SFH_NOALLOC_SURFACE) or
+    Submerged (N.SFH_NOALLOC_SUBMERGED) evaluation from device memory, with no
+    device allocation: d_state points to 38 doubles (T_CG, T_C, v, omega),
+    d_out to a 24-double output row; asynchronous on `stream`.  No
+    counterpart in the reference (for callers that cannot allocate)."""
+    s = N.sfh_settings(int(settings.dampingForces), int(settings.reallisticBuoyancy))
+    N.check(N.load().sfh_forces_noalloc(C.byref(s), int(mode), mesh.handle, ocn.handle, C.c_void_p(d_state),
+                                        C.c_void_p(d_out), C.c_void_p(stream)))
+
+
 def ComputeAerodynamicForces(mesh: Mesh | None, atm: Ocean, T_CG, T_C, v, omega, Fda, Tda) -> None:
     """SolidEntity::ComputeAerodynamicForces(mesh, atm, ...) (SolidEntity.cpp:1865-1918).
     atm: an Atmosphere (or an Ocean made with the gas density whose enabled
