@@ -45,11 +45,69 @@ def build_fleet(n_bodies: int, seed: int):
     return w
 
 
-def cpu_baseline(w, states: np.ndarray, ocean_grid: np.ndarray, budget_s: float = 12.0) -> dict:
-    """FP32 restatement of the reference path on the host cores (oracle)."""
+BOX_CPU_SHARE = 16   # host threads one GPU's job may use on the measuring machine
+
+
+def host_cpu_info() -> dict:
+    """CPU model, logical CPUs this process may run on, the physical cores
+    among them (unique (package, core) pairs of /proc/cpuinfo, the count the
+    reference's pool is sized to: SimulationApp.cpp:47, GetPhysicalCores),
+    and the cgroup CPU quota."""
+    info = {"model": "", "logical_cpus": 0, "physical_cores": 0, "host_physical_cores": 0, "cgroup_cpus": None}
+    try:
+        aff = sorted(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = list(range(os.cpu_count() or 1))
+    info["logical_cpus"] = len(aff)
+    cores, host_cores, cpu = set(), set(), {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                cpu = {"id": int(v)}
+            elif k == "model name" and not info["model"]:
+                info["model"] = v
+            elif k in ("physical id", "core id"):
+                cpu[k] = v
+                if "physical id" in cpu and "core id" in cpu:
+                    key = (cpu["physical id"], cpu["core id"])
+                    host_cores.add(key)
+                    if cpu["id"] in aff:
+                        cores.add(key)
+    except OSError:
+        pass
+    info["physical_cores"] = len(cores) or len(aff)
+    info["host_physical_cores"] = len(host_cores) or info["physical_cores"]
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            info["cgroup_cpus"] = round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                m = line.split(":", 1)[1].strip()
+                if len(m) > len(info["model"]):
+                    info["model"] = m
+    except Exception:
+        pass
+    return info
+
+
+def cpu_baseline(w, states: np.ndarray, ocean_grid: np.ndarray, budget_s: float = 12.0, budget1_s: float = 6.0) -> dict:
+    """FP32 restatement of the reference path (oracle ref32) on the host cores:
+    one task per body on a pool of N threads, N the physical cores this job
+    may use (the reference's pool size, SimulationApp.cpp:47), capped at the
+    measuring machine's CPU share of one GPU's job; and on one core."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    info = host_cpu_info()
+    threads = max(1, min(info["physical_cores"], BOX_CPU_SHARE,
+                         int(info["cgroup_cpus"]) if info["cgroup_cpus"] else 1 << 30))
     oo = O.Ocean(wave_data=ocean_grid, wave_n=w.sea.wave_n, channels=1, lengths=[w.sea.wave_len])
     meshes = [O.Mesh.from_tri(b.mesh) for b in w.bodies]
     bodies = [O.make_body(m, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface) for m, b in zip(meshes, w.bodies)]
@@ -61,19 +119,28 @@ def cpu_baseline(w, states: np.ndarray, ocean_grid: np.ndarray, budget_s: float 
     steps = max(1, int(budget_s / (per_body * n)))
     _, secs = O.batch_forces(32, bodies, oo, states, steps=steps, nthreads=threads)
     rate = n * steps / secs
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    # one core: the first n1 bodies (in order, so the face-count mix is the
+    # generator's) for about budget1_s
+    n1 = int(min(n, max(64, budget1_s / (per_body * threads))))
+    _, secs1 = O.batch_forces(32, bodies[:n1], oo, states[:n1], steps=1, nthreads=1)
+    rate1 = n1 / secs1
+    nf1 = sum(b.mesh.nf for b in w.bodies[:n1])
+    nf = sum(b.mesh.nf for b in w.bodies)
     return {"value": rate, "unit": "body-steps/s", "cores": threads, "kind": "port",
             "sample": f"all {n} bodies x {steps} step(s) of the same workload, FP32 restatement of "
-                      f"SolidEntity::ComputeHydrodynamicForces (oracle ref32), one task per body on "
-                      f"{threads} threads ({cpu_model})",
-            "seconds": secs}
+                      f"SolidEntity::ComputeHydrodynamicForces (oracle ref32), one task per body on a pool of "
+                      f"{threads} threads",
+            "seconds": round(secs, 3),
+            "one_core": {"value": round(rate1, 1), "unit": "body-steps/s", "cores": 1,
+                         "sample": f"first {n1} bodies x 1 step ({nf1} faces; the whole fleet has {nf})",
+                         "seconds": round(secs1, 3),
+                         "faces_per_s": round(nf1 / secs1, 1)},
+            "faces_per_s": round(nf * steps / secs, 1),
+            "host": info,
+            "pool_rule": f"physical cores available to this job ({info['physical_cores']}) capped at the "
+                         f"{BOX_CPU_SHARE}-thread CPU share one GPU's job has on the measuring machine "
+                         f"(host: {info['host_physical_cores']} physical cores, {info['logical_cpus']} logical "
+                         f"CPUs visible)"}
 
 
 def parity_sample(w, states, grid, out_gpu: np.ndarray, n: int = 96) -> dict:
@@ -126,6 +193,135 @@ def _time_replays(fn, k: int) -> float:
         fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / k
+
+
+def single_body_section(args) -> dict:
+    """The two single-body workloads: a 2,048-triangle capsule fully submerged
+    at 5 m (1,000 steps) and a 20,480-triangle ellipsoid crossing a flat
+    surface (10,000 steps), random velocities and orientations per step.
+
+    Each is timed three ways: through the per-body statics of the drop-in
+    boundary (one synchronous call per step: ComputeHydrodynamicForcesSubmerged
+    for the capsule, ComputeHydrodynamicForcesSurface for the ellipsoid, host
+    buffers in and out, as SolidEntity calls them); as one batch of all the
+    steps' states of the one body (SolidEntity::ComputeHydrodynamicForces,
+    classification, buoyancy and CorrectHydrodynamicForces included, states
+    resident in HBM, one captured graph per replay); and on the host with the
+    FP32 restatement (oracle ref32) on the N-thread pool and on one core.
+    The error of the batch against the FP64 oracle is stated per vector and
+    per component on a sample of steps."""
+    import torch
+    sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle as O
+    from stonefish_amd import hydro as H
+    from stonefish_amd import workloads as WL
+    from util import bound_ratio, component_errors
+    dev = torch.device("cuda:0")
+    info = host_cpu_info()
+    threads = max(1, min(info["physical_cores"], BOX_CPU_SHARE,
+                         int(info["cgroup_cpus"]) if info["cgroup_cpus"] else 1 << 30))
+    res = {}
+    for key, w, nsteps in (("capsule_submerged_2048", WL.capsule_submerged(), 1000),
+                           ("ellipsoid_surface_20480", WL.ellipsoid_surface(), 10000)):
+        b = w.bodies[0]
+        gm = H.Mesh(b.mesh)
+        go = H.Ocean()
+        st = WL.random_states(w, nsteps)[:, 0, :]            # (steps, 22)
+        submerged = key.startswith("capsule")
+        # (1) per-body statics, one synchronous call per step
+        outs = [np.zeros(3) for _ in range(6)]
+        sv = np.zeros(2)
+        settings = H.HydrodynamicsSettings(True, True)
+
+        def call(i):
+            if submerged:
+                H.ComputeHydrodynamicForcesSubmerged(gm, go, st[i, :16], st[i, :16], st[i, 16:19], st[i, 19:22],
+                                                     *outs[2:])
+            else:
+                H.ComputeHydrodynamicForcesSurface(settings, gm, go, st[i, :16], st[i, :16], st[i, 16:19],
+                                                   st[i, 19:22], *outs, sv)
+        for i in range(20):
+            call(i)
+        t0 = time.perf_counter()
+        for i in range(nsteps):
+            call(i)
+        statics_rate = nsteps / (time.perf_counter() - t0)
+        # (2) all steps as one batch of the body's states (body level)
+        desc = H.BodyDesc(mesh=gm, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface)
+        batch = H.BodyBatch([desc] * nsteps)
+        d_st = torch.tensor(st, dtype=torch.float64, device=dev).contiguous()
+        d_out = torch.zeros((nsteps, 24), dtype=torch.float64, device=dev)
+        gstream = torch.cuda.Stream()
+        with torch.cuda.stream(gstream):
+            batch.capture(go, d_st.data_ptr(), d_out.data_ptr(), gstream.cuda_stream)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream().cuda_stream
+        k = 20
+        sec = _time_replays(lambda: batch.replay(stream), k)
+        batch_rate = nsteps / sec
+        batch.timing(True)
+        for _ in range(10):
+            batch.compute(go, d_st.data_ptr(), d_out.data_ptr(), stream)
+            batch.kernel_ms()
+        torch.cuda.synchronize()
+        face_ms, total_ms, _ = batch.kernel_ms()
+        batch.timing(False)
+        out = d_out.cpu().numpy()
+        nf, nv = b.mesh.nf, b.mesh.nv
+        alg = (12 * nf + 12 * nv) * nsteps
+        # (3) host: FP32 restatement on N threads and on one core, bounded sample
+        om = O.Mesh.from_tri(b.mesh)
+        ob = O.make_body(om, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface)
+        oo = O.Ocean()
+        ns_cal = min(nsteps, 64)
+        _, c = O.batch_forces(32, [ob] * ns_cal, oo, st[:ns_cal], steps=1, nthreads=1)
+        per = max(c / ns_cal, 1e-7)
+        n1 = int(min(nsteps, max(32, 4.0 / per)))
+        _, c1 = O.batch_forces(32, [ob] * n1, oo, st[:n1], steps=1, nthreads=1)
+        nN = int(min(nsteps, max(64, 4.0 * threads / per)))
+        reps = max(1, int(4.0 * threads / (per * nN)))
+        _, cN = O.batch_forces(32, [ob] * nN, oo, st[:nN], steps=reps, nthreads=threads)
+        # error of the batch against ref64 on a sample of steps
+        ns = 64
+        r, x = [], []
+        for i in range(ns):
+            a, e = O.body_forces(64, ob, oo, st[i, :16], st[i, 16:19], st[i, 19:22], full=True)
+            r.append(a)
+            x.append(e)
+        r, x = np.stack(r), np.stack(x)
+        Cd = np.repeat(np.asarray(b.Cd)[None], ns, 0)
+        Cf = np.repeat(np.asarray(b.Cf)[None], ns, 0)
+        worst, plain = bound_ratio(out[:ns], r, x, Cd=Cd, Cf=Cf)
+        pos = np.bincount(out[:, 21].astype(int) + 1, minlength=4)[1:].tolist()
+        res[key] = {
+            "faces": nf, "vertices": nv, "steps": nsteps,
+            "positions_outside_inside_crossing": pos,
+            "gpu_per_body_statics": {"value": round(statics_rate, 1), "unit": "body-steps/s",
+                                     "call": "ComputeHydrodynamicForcesSubmerged" if submerged
+                                     else "ComputeHydrodynamicForcesSurface",
+                                     "note": "one synchronous call per step from one host thread, host buffers"},
+            "gpu_batch": {"value": round(batch_rate, 1), "unit": "body-steps/s", "ms_per_replay": round(sec * 1e3, 4),
+                          "face_pass_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),
+                          "roofline": {"bound": "hbm", "achieved": round(alg / (face_ms * 1e-3) / 1e9, 1),
+                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": round(alg / (face_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                       "algorithmic_bytes": alg,
+                                       "note": "12 F + 12 V per body-step; the steps share one mesh, so the "
+                                               "face passes read it from cache, not HBM"}},
+            "cpu_n_cores": {"value": round(nN * reps / cN, 1), "unit": "body-steps/s", "cores": threads,
+                            "sample": f"{nN} steps x {reps}"},
+            "cpu_one_core": {"value": round(n1 / c1, 1), "unit": "body-steps/s", "cores": 1,
+                             "sample": f"{n1} steps"},
+            "parity": {"steps": ns, "positions_and_masks_equal": bool(np.array_equal(out[:ns, 20:22], r[:, 20:22])),
+                       "bound_ratio_max": {k2: round(float(v), 4) for k2, v in worst.items()},
+                       "norm_rel_err_max": {k2: float(f"{v:.3e}") for k2, v in plain.items()},
+                       "component_err_max": component_errors(out[:ns], r)},
+        }
+        log(f"{key}: statics {statics_rate:.0f}/s, batch {batch_rate:.0f}/s, cpu N {nN * reps / cN:.0f}/s, "
+            f"cpu 1 {n1 / c1:.0f}/s")
+        del batch
+    return res
 
 
 def surrogate_section(args) -> dict:
@@ -298,6 +494,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1, help="replay a captured HIP graph per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-body workloads")
+    ap.add_argument("--single-only", action="store_true", help="run only the single-body workloads")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate and rollout sections")
     ap.add_argument("--rollout-only", action="store_true", help="run only the mixed rollout section (profiling)")
     ap.add_argument("--surrogate-only", action="store_true", help="run only the surrogate section (profiling)")
@@ -335,6 +533,9 @@ def main() -> int:
         del a, b
     if args.rollout_only:
         print(json.dumps({"rollout": rollout_section(args, WL.fleet(1, seed=args.seed).sea)}), flush=True)
+        return 0
+    if args.single_only:
+        print(json.dumps({"single_body": single_body_section(args)}), flush=True)
         return 0
     if args.surrogate_only:
         print(json.dumps({"surrogate": surrogate_section(args)}), flush=True)
@@ -450,6 +651,11 @@ def main() -> int:
         t0 = time.time()
         res["cpu_baseline"] = cpu_baseline(w, states_h[0], grid.cpu().numpy())
         log(f"cpu baseline in {time.time() - t0:.1f}s")
+    if not args.no_single:
+        t0 = time.time()
+        graphs = None
+        res["single_body"] = single_body_section(args)
+        log(f"single-body workloads in {time.time() - t0:.1f}s")
     if not args.no_surrogate:
         graphs = None
         res["surrogate"] = surrogate_section(args)
