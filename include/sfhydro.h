@@ -207,7 +207,30 @@ typedef struct {
  * work list (chunks of faces, each owned by one body). */
 int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const sfh_batch_opts* opts, sfh_batch* out);
 int sfh_batch_destroy(sfh_batch batch);
+/* A new faces_per_lane rebuilds the batch's work lists: graphs captured from
+ * the batch before (sfh_batch_capture, or by the caller around its compute)
+ * must be captured again. */
 int sfh_batch_set_opts(sfh_batch batch, const sfh_batch_opts* opts);
+int sfh_batch_get_opts(sfh_batch batch, sfh_batch_opts* opts);
+
+typedef struct {
+    uint32_t faces_per_lane;   /* chunk of 64 x faces_per_lane faces */
+    uint32_t waves_per_cu;     /* resident wavefronts of the face passes per compute unit */
+    double step_ms;            /* average time of the whole step (every launch) */
+    double face_ms;            /* of the two face passes, last timed step */
+} sfh_tune_point;
+
+/* No counterpart in the reference (its only knob is the thread pool size,
+ * SimulationApp.cpp:47).  Autotuner of the batched step: times the step on
+ * the caller's own inputs (d_states; d_active as in sfh_batch_compute_masked,
+ * or NULL) for faces_per_lane in {4, 8, 16, 32} x waves_per_cu in {8, 16,
+ * 24, 32, 48, 64, 96}, reps timed launches each (0 = 5), and leaves the batch
+ * set to the fastest.  d_out is written.  The measured points (up to
+ * max_points) go to `points`, their number to *npoints.  Synchronous; graphs
+ * captured from the batch before must be captured again. */
+int sfh_batch_autotune(sfh_batch batch, sfh_ocean ocean, const double* d_states, const uint8_t* d_active,
+                       double* d_out, void* stream, uint32_t reps, sfh_tune_point* points,
+                       uint32_t max_points, uint32_t* npoints);
 /* nbodies: rows of the state and output arrays (bodies, or compounds);
  * nchunks, nfaces: the face passes' work (over the parts of compounds). */
 int sfh_batch_info(sfh_batch batch, uint32_t* nbodies, uint32_t* nchunks, uint64_t* nfaces);

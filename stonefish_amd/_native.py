@@ -85,6 +85,11 @@ class sfh_batch_opts(C.Structure):
     _fields_ = [("faces_per_lane", C.c_uint32), ("waves", C.c_uint32)]
 
 
+class sfh_tune_point(C.Structure):
+    _fields_ = [("faces_per_lane", C.c_uint32), ("waves_per_cu", C.c_uint32), ("step_ms", C.c_double),
+                ("face_ms", C.c_double)]
+
+
 class sfh_wave_component(C.Structure):
     _fields_ = [("amplitude", C.c_float), ("phase", C.c_float), ("mx", C.c_int32), ("my", C.c_int32)]
 
@@ -166,6 +171,9 @@ _PROTOS = {
     "sfh_wave_fft_destroy": (C.c_int, [C.c_void_p]),
     "sfh_batch_destroy": (C.c_int, [C.c_void_p]),
     "sfh_batch_set_opts": (C.c_int, [C.c_void_p, C.POINTER(sfh_batch_opts)]),
+    "sfh_batch_get_opts": (C.c_int, [C.c_void_p, C.POINTER(sfh_batch_opts)]),
+    "sfh_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_uint32, C.POINTER(sfh_tune_point), C.c_uint32, C.POINTER(C.c_uint32)]),
     "sfh_batch_info": (C.c_int, [C.c_void_p, c_uint32_p, c_uint32_p, C.POINTER(C.c_uint64)]),
     "sfh_batch_compute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfh_batch_compute_host": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p]),

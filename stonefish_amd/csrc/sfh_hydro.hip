@@ -2,15 +2,18 @@
 // hydrodynamics (SolidEntity.cpp:1239-1850, Ocean.cpp:149-201,
 // OpenGLRealOcean.cpp:280-331, Compound-free body path).
 //
-// One evaluation pass over a batch of bodies is five launches on one stream:
+// One evaluation pass over a batch of bodies is a few launches on one stream:
 //   k_prologue  one lane per body, FP64: C transform, body/fluid classification
 //               (8 AABB corners against the surface), per-body FP32 constants of
 //               the face loop in the mesh frame, FP64 constants of the epilogue;
-//   k_depths    FP64 depth of every vertex of the bodies crossing the surface;
+//   k_scan_part / k_scan_off / k_scatter
+//               the step's work lists (chunks of inside and crossing bodies);
 //   k_faces_sub / k_faces_cross
 //               one wavefront per chunk of up to 64*K faces of one body: FP32
 //               face arithmetic with FP32 per-lane partials, FP64 wavefront
-//               reduction; one 24-double partial per chunk;
+//               reduction; one 24-double partial per chunk (the crossing pass
+//               first evaluates the FP64 depths of the chunk's vertex window
+//               into LDS);
 //   k_epilogue  one wavefront per body, FP64: sums the body's chunk partials
 //               (deterministic order), rotates to world, moment about the CoG,
 //               buoyancy, CorrectHydrodynamicForces, output write mask.
@@ -100,10 +103,7 @@ struct BodyStatic {
     uint32_t chunk0;
     uint32_t nchunks;
     uint32_t nv;
-    uint32_t vchunk0;   // first vertex chunk of the body (depth pass)
-    uint32_t nvchunks;
     uint32_t sidx;      // row of the state array (and active mask) the body reads: its compound for a part
-    uint64_t dep_off;   // offset of this body's vertex depths in the depth buffer
     double cg2c[12];   // basis row-major, origin
     double cg2o[9];    // basis row-major
     double aabb_c[3];  // CG-frame AABB centre and half extents
@@ -120,7 +120,6 @@ struct LayerK {
     double fu, fv;      // fractional texel coordinate of the mesh origin
     double ru[3], rv[3];// rows 0/1 of R scaled by n / length: texels per mesh unit
     int ku, kv;         // integer texel coordinate of the mesh origin, mod n
-    float fu32, fv32, ru32[3], rv32[3];   // the same in FP32 (depth32)
 };
 
 struct BodyDyn {
@@ -228,43 +227,21 @@ __device__ __forceinline__ float depth_vertex(const BodyDyn& D, const OceanK& oc
     return (float)depth64(D, oc, p.x, p.y, p.z);
 }
 
-// depth64 with the texel coordinates in FP32, for the wet centroids of the
-// pressure buoyancy (SolidEntity.cpp:1650-1658).  The coordinates are
-// relative to the body's texel (ku + fu), so FP32 places the centroid to
-// ~1e-7 texel, a height error of ~1e-8 m; the world z and the bilinear blend
-// stay FP64 (in FP32 their rounding of metre-sized heights, ~1e-7 m per
-// face, does not average out over the few wet faces of a small mesh).  The
-// vertex depths that decide and place the waterline clip are depth64.
-__device__ __forceinline__ double depth_centroid(const BodyDyn& D, const OceanK& oc, f3 p)
-{
-    double d = D.tz + D.rz[0] * (double)p.x + D.rz[1] * (double)p.y + D.rz[2] * (double)p.z;
-    for (int l = 0; l < oc.nlayers; ++l) {
-        const LayerK& L = D.layer[l];
-        const float u = L.fu32 + L.ru32[0] * p.x + L.ru32[1] * p.y + L.ru32[2] * p.z;
-        const float v = L.fv32 + L.rv32[0] * p.x + L.rv32[1] * p.y + L.rv32[2] * p.z;
-        const float fi = floorf(u), fj = floorf(v);
-        int i, j;
-        if (oc.nmask >= 0) {
-            i = (L.ku + (int)fi) & oc.nmask;
-            j = (L.kv + (int)fj) & oc.nmask;
-        } else {
-            i = wrap_n(L.ku + (int)fi, oc.n);
-            j = wrap_n(L.kv + (int)fj, oc.n);
-        }
-        d += texel_mix((double)(u - fi), (double)(v - fj), oc.quad[l][(size_t)j * oc.n + i]);
-    }
-    return d;
-}
-
-// depth_centroid with the body's wave constants hoisted out of the face loop
-// for a chunk: NL layers known at compile time (0, 1, 2), their constants
-// read once into registers; NL < 0 reads the layer count at run time and
-// defers to depth_centroid.  Same arithmetic as depth_centroid.
+// Depth of a wet polygon's centroid for the pressure buoyancy
+// (SolidEntity.cpp:1650-1658), with the body's wave constants hoisted out of
+// the face loop for a chunk: NL layers known at compile time (0, 1, 2),
+// their constants read once into registers; NL < 0 reads the layer count at
+// run time.  The centroid is formed in FP64 from its FP32 wet base vertex
+// and offset (exact: both are FP32 numbers at most a face apart), and its
+// depth is the FP64 query of depth64.  Near the waterline a depth is a
+// small difference of body-scale values, so a centroid rounded to FP32 at
+// body scale (~1e-7 m) would carry that error into the millimetre depths of
+// thin wet slivers.
 template <int NL>
 struct WaveQ {
     static constexpr int M = NL > 0 ? NL : 1;
     double tz, rz[3];
-    float fu[M], fv[M], ru[M][3], rv[M][3];
+    double fu[M], fv[M], ru[M][3], rv[M][3];
     int ku[M], kv[M];
     const SFH_GLOBAL f32x4* quad[M];
     int n, nmask;
@@ -283,24 +260,27 @@ struct WaveQ {
 #pragma unroll
         for (int l = 0; l < (NL > 0 ? NL : 0); ++l) {
             const LayerK& L = Dy.layer[l];
-            fu[l] = L.fu32;
-            fv[l] = L.fv32;
-            for (int k = 0; k < 3; ++k) { ru[l][k] = L.ru32[k]; rv[l][k] = L.rv32[k]; }
+            fu[l] = L.fu;
+            fv[l] = L.fv;
+            for (int k = 0; k < 3; ++k) { ru[l][k] = L.ru[k]; rv[l][k] = L.rv[k]; }
             ku[l] = L.ku;
             kv[l] = L.kv;
             quad[l] = (const SFH_GLOBAL f32x4*)o.quad[l];
         }
     }
 
-    __device__ __forceinline__ double depth(f3 p) const
+    // depth64 at the mesh-frame point base + off (FP32 parts, summed in FP64)
+    __device__ __forceinline__ double depth(f3 base, f3 off) const
     {
-        if (NL < 0) return depth_centroid(*D, *oc, p);
-        double d = tz + rz[0] * (double)p.x + rz[1] * (double)p.y + rz[2] * (double)p.z;
+        const double x = (double)base.x + (double)off.x, y = (double)base.y + (double)off.y,
+                     z = (double)base.z + (double)off.z;
+        if (NL < 0) return depth64(*D, *oc, x, y, z);
+        double d = tz + rz[0] * x + rz[1] * y + rz[2] * z;
 #pragma unroll
         for (int l = 0; l < (NL > 0 ? NL : 0); ++l) {
-            const float u = fu[l] + ru[l][0] * p.x + ru[l][1] * p.y + ru[l][2] * p.z;
-            const float v = fv[l] + rv[l][0] * p.x + rv[l][1] * p.y + rv[l][2] * p.z;
-            const float fi = floorf(u), fj = floorf(v);
+            const double u = fu[l] + ru[l][0] * x + ru[l][1] * y + ru[l][2] * z;
+            const double v = fv[l] + rv[l][0] * x + rv[l][1] * y + rv[l][2] * z;
+            const double fi = floor(u), fj = floor(v);
             int i, j;
             if (nmask >= 0) {
                 i = (ku[l] + (int)fi) & nmask;
@@ -310,7 +290,7 @@ struct WaveQ {
                 j = wrap_n(kv[l] + (int)fj, n);
             }
             const f32x4 t = quad[l][(size_t)j * n + i];
-            d += texel_mix((double)(u - fi), (double)(v - fj), make_float4(t.x, t.y, t.z, t.w));
+            d += texel_mix(u - fi, v - fj, make_float4(t.x, t.y, t.z, t.w));
         }
         return d;
     }
@@ -539,9 +519,6 @@ __global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__
             L.kv = wrap_texel(fv, oc);
             for (int k = 0; k < 3; ++k) { L.ru[k] = R[k] * s; L.rv[k] = R[3 + k] * s; }
         }
-        L.fu32 = (float)L.fu;
-        L.fv32 = (float)L.fv;
-        for (int k = 0; k < 3; ++k) { L.ru32[k] = (float)L.ru[k]; L.rv32[k] = (float)L.rv[k]; }
         D.layer[l] = L;
     }
     D.cur.n = oc.ncur;
@@ -601,19 +578,27 @@ struct CurFlow {
 };
 
 // ---------------------------------------------------------------------------
-// k_depths: FP64 depth of every vertex of the bodies crossing the surface
+// work lists
 // ---------------------------------------------------------------------------
 
-// A work item of a pass: a run of consecutive faces (k_faces) or vertices
-// (k_depths) of one body.
+// A work item of the face passes: a run of consecutive faces of one body and
+// the window of vertex ids they reference (vertices are numbered in order of
+// first use, so a run of faces of a coherent mesh references a compact
+// window).  vn = 0: the window is larger than the crossing pass keeps in LDS.
 struct Chunk {
-    uint32_t body, first, count, pad;
+    uint32_t body, first, count;
+    uint32_t vlo, vn;
+    uint32_t pad[3];
 };
+
+// Vertex-window capacity (vertices) of a crossing-pass wavefront for chunks
+// of 64 K faces; the window's FP64 depths are kept in LDS as floats.
+__host__ __device__ constexpr uint32_t win_cap(uint32_t K) { return K <= 16 ? 64u * K : 1536u; }
 
 // Per-step work lists.  k_prologue writes every body's position to bpos; a
 // two-pass prefix sum in body order turns the positions into each body's
-// offsets in the three lists (chunks of inside bodies, chunks of crossing
-// bodies, vertex chunks of crossing bodies): k_scan_part sums blocks of
+// offsets in the two lists (chunks of inside bodies, chunks of crossing
+// bodies): k_scan_part sums blocks of
 // kScanBlock bodies, k_scan_off adds the sums of the blocks before its own
 // and scans within the block; k_scatter (one wavefront per body,
 // grid-strided) writes the chunk ids.  The passes then take list entries
@@ -624,71 +609,70 @@ static constexpr int kScanPer = 8;                              // consecutive b
 static constexpr uint32_t kScanBlock = kScanThreads * kScanPer; // bodies per workgroup
 
 struct BodyChunks {
-    uint32_t chunk0, nchunks, vchunk0, nvchunks;
+    uint32_t chunk0, nchunks;
 };
 
 __device__ __forceinline__ void body_counts(const int8_t* __restrict__ bpos, const BodyChunks* __restrict__ bch,
-                                            uint32_t b, uint32_t& c0, uint32_t& c1, uint32_t& c2)
+                                            uint32_t b, uint32_t& c0, uint32_t& c1)
 {
     const int p = bpos[b];
     const BodyChunks bc = bch[b];
     c0 = (p == 1) ? bc.nchunks : 0u;
     c1 = (p == 2) ? bc.nchunks : 0u;
-    c2 = (p == 2) ? bc.nvchunks : 0u;
 }
 
-// Exclusive scan of three counters over the threads of a workgroup (in
-// thread order); the workgroup totals land in tot of thread 0.
+// Exclusive scan of two counters over the threads of a workgroup (in thread
+// order); the workgroup totals land in tot.
 template <int NT>
-__device__ __forceinline__ void block_scan3(uint32_t (&v)[3], uint32_t (&tot)[3])
+__device__ __forceinline__ void block_scan2(uint32_t (&v)[2], uint32_t (&tot)[2])
 {
-    __shared__ uint32_t wsum[3][NT / 64];
+    __shared__ uint32_t wsum[2][NT / 64];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint32_t inc[3] = {v[0], v[1], v[2]};
+    uint32_t inc[2] = {v[0], v[1]};
     for (int off = 1; off < 64; off <<= 1)
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 2; ++k) {
             const uint32_t y = __shfl_up(inc[k], off, 64);
             if ((int)lane >= off) inc[k] += y;
         }
     if (lane == 63)
-        for (int k = 0; k < 3; ++k) wsum[k][w] = inc[k];
+        for (int k = 0; k < 2; ++k) wsum[k][w] = inc[k];
     __syncthreads();
     if (t == 0) {
-        uint32_t run[3] = {0, 0, 0};
+        uint32_t run[2] = {0, 0};
         for (int i = 0; i < NT / 64; ++i)
-            for (int k = 0; k < 3; ++k) { const uint32_t x = wsum[k][i]; wsum[k][i] = run[k]; run[k] += x; }
-        for (int k = 0; k < 3; ++k) tot[k] = run[k];
+            for (int k = 0; k < 2; ++k) { const uint32_t x = wsum[k][i]; wsum[k][i] = run[k]; run[k] += x; }
+        for (int k = 0; k < 2; ++k) tot[k] = run[k];
     }
     __syncthreads();
-    for (int k = 0; k < 3; ++k) v[k] = wsum[k][w] + inc[k] - v[k];
+    for (int k = 0; k < 2; ++k) v[k] = wsum[k][w] + inc[k] - v[k];
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_part(const int8_t* __restrict__ bpos,
                                                             const BodyChunks* __restrict__ bch, uint32_t n,
-                                                            uint4* __restrict__ part)
+                                                            uint2* __restrict__ part)
 {
-    __shared__ uint32_t red[3][kScanThreads / 64];
+    __shared__ uint32_t red[2][kScanThreads / 64];
     const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
-    uint32_t c[3] = {0, 0, 0};
+    uint32_t c[2] = {0, 0};
     for (int j = 0; j < kScanPer; ++j) {
         const uint32_t b = b0 + j;
         if (b < n) {
-            uint32_t x0, x1, x2;
-            body_counts(bpos, bch, b, x0, x1, x2);
-            c[0] += x0; c[1] += x1; c[2] += x2;
+            uint32_t x0, x1;
+            body_counts(bpos, bch, b, x0, x1);
+            c[0] += x0; c[1] += x1;
         }
     }
     for (int off = 32; off >= 1; off >>= 1)
-        for (int k = 0; k < 3; ++k) c[k] += __shfl_xor(c[k], off, 64);
+        for (int k = 0; k < 2; ++k) c[k] += __shfl_xor(c[k], off, 64);
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0)
-        for (int k = 0; k < 3; ++k) red[k][w] = c[k];
+        for (int k = 0; k < 2; ++k) red[k][w] = c[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t s[3] = {0, 0, 0};
+        uint32_t s[2] = {0, 0};
         for (int i = 0; i < kScanThreads / 64; ++i)
-            for (int k = 0; k < 3; ++k) s[k] += red[k][i];
-        part[blockIdx.x] = make_uint4(s[0], s[1], s[2], 0u);
+            for (int k = 0; k < 2; ++k) s[k] += red[k][i];
+        part[blockIdx.x] = make_uint2(s[0], s[1]);
     }
 }
 
@@ -697,57 +681,55 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_part(const int8_t* __rest
 // needed for a scan of the block sums); the last block writes the totals.
 __global__ __launch_bounds__(kScanThreads) void k_scan_off(const int8_t* __restrict__ bpos,
                                                            const BodyChunks* __restrict__ bch, uint32_t n,
-                                                           const uint4* __restrict__ part, uint4* __restrict__ boff,
+                                                           const uint2* __restrict__ part, uint2* __restrict__ boff,
                                                            uint32_t* __restrict__ counts)
 {
-    __shared__ uint32_t red[3][kScanThreads / 64];
+    __shared__ uint32_t red[2][kScanThreads / 64];
     const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
-    uint32_t pre[3] = {0, 0, 0};
+    uint32_t pre[2] = {0, 0};
     for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) {
-        const uint4 p = part[j];
-        pre[0] += p.x; pre[1] += p.y; pre[2] += p.z;
+        const uint2 p = part[j];
+        pre[0] += p.x; pre[1] += p.y;
     }
     for (int off = 32; off >= 1; off >>= 1)
-        for (int k = 0; k < 3; ++k) pre[k] += __shfl_xor(pre[k], off, 64);
+        for (int k = 0; k < 2; ++k) pre[k] += __shfl_xor(pre[k], off, 64);
     if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 3; ++k) red[k][threadIdx.x >> 6] = pre[k];
+        for (int k = 0; k < 2; ++k) red[k][threadIdx.x >> 6] = pre[k];
     __syncthreads();
-    uint4 base = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t base[2] = {0, 0};
     for (int i = 0; i < kScanThreads / 64; ++i) {
-        base.x += red[0][i]; base.y += red[1][i]; base.z += red[2][i];
+        base[0] += red[0][i]; base[1] += red[1][i];
     }
-    uint32_t c[3] = {0, 0, 0};
+    uint32_t c[2] = {0, 0};
     for (int j = 0; j < kScanPer; ++j) {
         const uint32_t b = b0 + j;
         if (b < n) {
-            uint32_t x0, x1, x2;
-            body_counts(bpos, bch, b, x0, x1, x2);
-            c[0] += x0; c[1] += x1; c[2] += x2;
+            uint32_t x0, x1;
+            body_counts(bpos, bch, b, x0, x1);
+            c[0] += x0; c[1] += x1;
         }
     }
-    uint32_t tot[3];
-    block_scan3<kScanThreads>(c, tot);
+    uint32_t tot[2];
+    block_scan2<kScanThreads>(c, tot);
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-        counts[0] = base.x + tot[0];
-        counts[1] = base.y + tot[1];
-        counts[2] = base.z + tot[2];
+        counts[0] = base[0] + tot[0];
+        counts[1] = base[1] + tot[1];
     }
-    uint32_t o[3] = {base.x + c[0], base.y + c[1], base.z + c[2]};
+    uint32_t o[2] = {base[0] + c[0], base[1] + c[1]};
     for (int j = 0; j < kScanPer; ++j) {
         const uint32_t b = b0 + j;
         if (b < n) {
-            uint32_t x0, x1, x2;
-            body_counts(bpos, bch, b, x0, x1, x2);
-            boff[b] = make_uint4(o[0], o[1], o[2], 0u);
-            o[0] += x0; o[1] += x1; o[2] += x2;
+            uint32_t x0, x1;
+            body_counts(bpos, bch, b, x0, x1);
+            boff[b] = make_uint2(o[0], o[1]);
+            o[0] += x0; o[1] += x1;
         }
     }
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const int8_t* __restrict__ bpos, const BodyChunks* __restrict__ bch,
-                                                 const uint4* __restrict__ boff, uint32_t n,
-                                                 uint32_t* __restrict__ list_sub, uint32_t* __restrict__ list_cross,
-                                                 uint32_t* __restrict__ list_vert)
+                                                 const uint2* __restrict__ boff, uint32_t n,
+                                                 uint32_t* __restrict__ list_sub, uint32_t* __restrict__ list_cross)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -756,40 +738,9 @@ __global__ __launch_bounds__(256) void k_scatter(const int8_t* __restrict__ bpos
         const int p = bpos[b];
         if (p != 1 && p != 2) continue;
         const BodyChunks bc = bch[b];
-        const uint4 o = boff[b];
+        const uint2 o = boff[b];
         uint32_t* L = (p == 1) ? list_sub + o.x : list_cross + o.y;
         for (uint32_t i = lane; i < bc.nchunks; i += 64) L[i] = bc.chunk0 + i;
-        if (p == 2)
-            for (uint32_t i = lane; i < bc.nvchunks; i += 64) list_vert[o.z + i] = bc.vchunk0 + i;
-    }
-}
-
-// Near the waterline a depth is a small difference of body-scale values, and
-// the clip points of thin wet slivers are built from it, so it is evaluated in
-// FP64 once per vertex (a mesh vertex is shared by about six faces) and stored
-// as float (relative precision is kept for small depths).
-__global__ __launch_bounds__(256) void k_depths(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-                                                const Chunk* __restrict__ vchunks, const BodyStatic* __restrict__ st,
-                                                const BodyDyn* __restrict__ dyn, OceanK oc,
-                                                float* __restrict__ depth)
-{
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t nl = count[0];
-    for (uint32_t li = wave; li < nl; li += nwaves) {
-        const Chunk ch = vchunks[__builtin_amdgcn_readfirstlane(list[li])];
-        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
-        const BodyDyn& D = dyn[b];
-        const BodyStatic& S = st[b];
-        const uint32_t v0 = __builtin_amdgcn_readfirstlane(ch.first);
-        const uint32_t n = __builtin_amdgcn_readfirstlane(ch.count);
-        const SFH_GLOBAL float* verts = gptr(S.verts);
-        SFH_GLOBAL float* dout = gptr_w(depth) + S.dep_off;
-        for (uint32_t v = lane; v < n; v += 64) {
-            const SFH_GLOBAL float* p = verts + 3 * (size_t)(v0 + v);
-            dout[v0 + v] = (float)depth64(D, oc, p[0], p[1], p[2]);
-        }
     }
 }
 
@@ -975,7 +926,7 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
     if (!wt.ok) return;
     f3 cpt = wt.base + wt.off;
     if (NL != 0 && wave_buoy) {
-        const float dc = (float)wq.depth(cpt);
+        const float dc = (float)wq.depth(wt.base, wt.off);
         f3 fbi = wt.n1 * (-wt.A * dc);
         add3(acc.Fb, fbi);
         add3(acc.Mb, cross(cpt, fbi));
@@ -992,25 +943,121 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
     acc.Swet += wt.A;
 }
 
-// Surface loop (bodies crossing the surface): waterline clip with the vertex
-// depths of k_depths, tetra volumes, pressure buoyancy with waves, drag; the
-// chunks of this step's crossing bodies.
-// The chunk's all-wet faces are processed as they come; faces the waterline
-// cuts (clip cases A-F) are queued in LDS by each wavefront and processed
-// together afterwards, 64 at a time, so the clip code runs once per 64 cut
-// faces rather than on every pass over the chunk.
+// Surface loop (bodies crossing the surface), one chunk: the FP64 depths of
+// the chunk's vertex window (WIN: into the wavefront's LDS slice first;
+// otherwise per face vertex as the faces come), then the faces: all-wet
+// faces inline; faces the waterline cuts (clip cases A-F) queued in LDS and
+// clipped together afterwards, 64 at a time, so the clip code runs once per
+// 64 cut faces rather than on every pass over the chunk.  Tetra volumes,
+// pressure buoyancy with waves, drag.
+template <int K, bool NU, int NL, bool WIN>
+__device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nfc, uint32_t vlo, uint32_t vn,
+                                            const BodyStatic& S, const BodyDyn& D, const OceanK& oc,
+                                            const WaveQ<NL>& wq, float* __restrict__ ldep,
+                                            uint16_t* __restrict__ cutq, double* __restrict__ partials, int lane)
+{
+    const uint64_t below = (1ull << lane) - 1ull;
+    const SFH_GLOBAL float* __restrict__ verts = gptr(S.verts);
+    const SFH_GLOBAL uint32_t* __restrict__ faces = gptr(S.faces) + 3 * (size_t)f0;
+    const FaceBody fb = D.fb;
+    const uint32_t flags = D.flags;
+    const bool damping = flags & BF_DAMPING;
+    const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
+    if (WIN) {
+        // four vertices per lane in flight: their loads, then their texel
+        // gathers, issue together (indices past the window are clamped to
+        // its last vertex; those results are not stored)
+        const f3 zero = mk3(0.f, 0.f, 0.f);
+        for (uint32_t v0 = 0; v0 < vn; v0 += 256u) {
+            f3 p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) p[k] = vert(verts, vlo + min(v0 + 64u * k + lane, vn - 1));
+            float d[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d[k] = (float)wq.depth(p[k], zero);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (v0 + 64u * k + lane < vn) ldep[v0 + 64u * k + lane] = d[k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    auto dep = [&](uint32_t i, f3 p) -> float {
+        if (WIN) return ldep[i - vlo];
+        return (float)wq.depth(p, mk3(0.f, 0.f, 0.f));   // = depth64 at p
+    };
+    Acc<float> acc;
+    acc_zero(acc);
+    const uint32_t flast = nfc - 1;
+    uint32_t fc0 = min((uint32_t)lane, flast), fc1 = min(64u + lane, flast);
+    uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
+    uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
+    f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
+    uint32_t ncut = 0;
+    for (int j = 0; j < K; ++j) {
+        const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
+        f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
+        uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
+        const float d0 = dep(a0, p1), d1 = dep(a1, p2), d2 = dep(a2, p3);
+        const bool valid = 64u * j + lane < nfc;
+        const bool allwet = !(d0 < 0.f) && !(d1 < 0.f) && !(d2 < 0.f);
+        const bool cut = valid && !allwet && !(d0 < 0.f && d1 < 0.f && d2 < 0.f);
+        if (valid && allwet)
+            wet_contrib<NU, NL>(face_allwet(fb, p1, p2, p3, acc), fb, D, wq, wave_buoy, damping, acc);
+        const uint64_t m = __ballot(cut);
+        if (cut) cutq[ncut + (uint32_t)__popcll(m & below)] = (uint16_t)(64u * j + lane);
+        ncut += (uint32_t)__popcll(m);
+        p1 = q1; p2 = q2; p3 = q3;
+        a0 = b0; a1 = b1; a2 = b2;
+        b0 = c0; b1 = c1; b2 = c2;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t qb = 0; qb < ncut; qb += 64) {
+        if (qb + lane < ncut) {
+            const uint32_t fo = cutq[qb + lane];
+            const uint32_t i0 = faces[3 * fo], i1 = faces[3 * fo + 1], i2 = faces[3 * fo + 2];
+            const f3 v0 = vert(verts, i0), v1 = vert(verts, i1), v2 = vert(verts, i2);
+            const Wet wt = face_clip(fb, v0, v1, v2, dep(i0, v0), dep(i1, v1), dep(i2, v2), acc);
+            wet_contrib<NU, NL>(wt, fb, D, wq, wave_buoy, damping, acc);
+        }
+    }
+    // the queue and the depth window are rewritten by the next chunk
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // chunk partial layout: Fdq Mdq Fdf Mdf Fb Mb CB (3 each), V6, Swet
+    double vals[24];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        vals[k] = acc.Fdq[k]; vals[3 + k] = acc.Mdq[k];
+        vals[6 + k] = acc.Fdf[k]; vals[9 + k] = acc.Mdf[k];
+        vals[12 + k] = acc.Fb[k]; vals[15 + k] = acc.Mb[k];
+        vals[18 + k] = acc.CB[k];
+    }
+    vals[21] = acc.V6;
+    vals[22] = acc.Swet;
+    vals[23] = 0.0;
+    xreduce24(vals, lane);
+    const int q = xreduce24_slot(lane);
+    if (q >= 0) gptr_w(partials)[(size_t)c * kChunkDoubles + q] = vals[0];
+}
+
+// Surface loop over the chunks of this step's crossing bodies (`list`).
 template <int K, bool NU, int NL>
 __global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
                                                      const Chunk* __restrict__ chunks,
                                                      const BodyStatic* __restrict__ st,
                                                      const BodyDyn* __restrict__ dyn, OceanK oc,
-                                                     const float* __restrict__ depth, double* __restrict__ partials)
+                                                     double* __restrict__ partials)
 {
     __shared__ uint16_t cutq[4][64 * K];
+    __shared__ float ldep[4][win_cap(K)];
     const int lane = threadIdx.x & 63;
     const int wl = threadIdx.x >> 6;
-    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t n = count[0];
@@ -1020,71 +1067,16 @@ __global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t*
         const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
         const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
         const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
+        const uint32_t vlo = __builtin_amdgcn_readfirstlane(ch.vlo);
+        const uint32_t vn = __builtin_amdgcn_readfirstlane(ch.vn);
         const BodyDyn& D = dyn[b];
         const BodyStatic& S = st[b];
-        const SFH_GLOBAL float* __restrict__ verts = gptr(S.verts);
-        const SFH_GLOBAL uint32_t* __restrict__ faces = gptr(S.faces) + 3 * (size_t)f0;
-        const FaceBody fb = D.fb;
-        const uint32_t flags = D.flags;
-        const bool damping = flags & BF_DAMPING;
-        const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
-        const SFH_GLOBAL float* __restrict__ dep = gptr(depth) + S.dep_off;
         WaveQ<NL> wq;
         wq.load(D, oc);
-        Acc<float> acc;
-        acc_zero(acc);
-        const uint32_t flast = nfc - 1;
-        uint32_t fc0 = min((uint32_t)lane, flast), fc1 = min(64u + lane, flast);
-        uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
-        uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
-        f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
-        float d0 = dep[a0], d1 = dep[a1], d2 = dep[a2];
-        uint32_t ncut = 0;
-        for (int j = 0; j < K; ++j) {
-            const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
-            f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
-            float e0 = dep[b0], e1 = dep[b1], e2 = dep[b2];
-            uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
-            const bool valid = 64u * j + lane < nfc;
-            const bool allwet = !(d0 < 0.f) && !(d1 < 0.f) && !(d2 < 0.f);
-            const bool cut = valid && !allwet && !(d0 < 0.f && d1 < 0.f && d2 < 0.f);
-            if (valid && allwet)
-                wet_contrib<NU, NL>(face_allwet(fb, p1, p2, p3, acc), fb, D, wq, wave_buoy, damping, acc);
-            const uint64_t m = __ballot(cut);
-            if (cut) cutq[wl][ncut + (uint32_t)__popcll(m & below)] = (uint16_t)(64u * j + lane);
-            ncut += (uint32_t)__popcll(m);
-            p1 = q1; p2 = q2; p3 = q3;
-            d0 = e0; d1 = e1; d2 = e2;
-            b0 = c0; b1 = c1; b2 = c2;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t qb = 0; qb < ncut; qb += 64) {
-            if (qb + lane < ncut) {
-                const uint32_t fo = cutq[wl][qb + lane];
-                const uint32_t i0 = faces[3 * fo], i1 = faces[3 * fo + 1], i2 = faces[3 * fo + 2];
-                const Wet wt = face_clip(fb, vert(verts, i0), vert(verts, i1), vert(verts, i2), dep[i0], dep[i1],
-                                         dep[i2], acc);
-                wet_contrib<NU, NL>(wt, fb, D, wq, wave_buoy, damping, acc);
-            }
-        }
-        __builtin_amdgcn_wave_barrier();   // the queue is rewritten by the next chunk
-        // chunk partial layout: Fdq Mdq Fdf Mdf Fb Mb CB (3 each), V6, Swet
-        double vals[24];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            vals[k] = acc.Fdq[k]; vals[3 + k] = acc.Mdq[k];
-            vals[6 + k] = acc.Fdf[k]; vals[9 + k] = acc.Mdf[k];
-            vals[12 + k] = acc.Fb[k]; vals[15 + k] = acc.Mb[k];
-            vals[18 + k] = acc.CB[k];
-        }
-        vals[21] = acc.V6;
-        vals[22] = acc.Swet;
-        vals[23] = 0.0;
-        xreduce24(vals, lane);
-        const int q = xreduce24_slot(lane);
-        if (q >= 0) gptr_w(partials)[(size_t)c * kChunkDoubles + q] = vals[0];
+        if (vn > 0 && vn <= win_cap(K))
+            cross_chunk<K, NU, NL, true>(c, f0, nfc, vlo, vn, S, D, oc, wq, ldep[wl], cutq[wl], partials, lane);
+        else
+            cross_chunk<K, NU, NL, false>(c, f0, nfc, vlo, vn, S, D, oc, wq, ldep[wl], cutq[wl], partials, lane);
     }
 }
 
@@ -1460,6 +1452,8 @@ __global__ void k_wave_quad(const float* __restrict__ data, uint32_t n, uint32_t
 struct sfh_mesh_t {
     float* d_verts = nullptr;
     uint32_t* d_faces = nullptr;
+    uint2* d_blkwin = nullptr;          // per block of 64 faces: lowest and highest vertex id referenced
+    std::vector<uint2> blkwin;          // the same on the host
     uint32_t nv = 0, nf = 0;
     float aabb_min[3], aabb_max[3];
     uint64_t id = 0;
@@ -1473,6 +1467,7 @@ static void mesh_release(sfh_mesh_t* m)
     if (m && m->refs.fetch_sub(1) == 1) {
         hipFree(m->d_verts);
         hipFree(m->d_faces);
+        hipFree(m->d_blkwin);
         delete m;
     }
 }
@@ -1499,6 +1494,7 @@ struct sfh_batch_t {
     uint32_t n = 0;         // bodies of the face passes (the parts, for a compound batch)
     uint32_t nunits = 0;    // rows of the state and output arrays: bodies, or compounds
     bool compound = false;
+    int fixed_pos = 0;      // per-body calls: 1 or 2, every chunk in that pass's list (no per-step lists)
     bool aero = false;      // every body runs the aerodynamic face loop (sfh_forces_aero)
     CompStatic* d_comp = nullptr;
     double* d_part_out = nullptr;
@@ -1507,21 +1503,17 @@ struct sfh_batch_t {
     uint64_t nfaces = 0;
     uint32_t K = 8;
     uint32_t waves = 0;
-    uint32_t nvchunks = 0;
-    uint64_t nverts = 0;
     std::vector<BodyStatic> h_static;
     BodyStatic* d_static = nullptr;
     uint32_t* d_sidx = nullptr;     // per body: BodyStatic::sidx, compact (the masked prologue's first read)
     double* d_hstates = nullptr;    // device copies of sfh_batch_compute_host's host buffers (first call)
     double* d_hout = nullptr;
     Chunk* d_chunks = nullptr;
-    Chunk* d_vchunks = nullptr;
-    float* d_depth = nullptr;
     int8_t* d_bpos = nullptr;       // per body: this step's position (-2 inactive, -1 not hydrodynamic, 0, 1, 2)
-    BodyChunks* d_bch = nullptr;    // per body: its face and vertex chunk ranges
-    uint4* d_boff = nullptr;        // per body: this step's offsets in the three lists
-    uint4* d_scan = nullptr;        // per block of kScanBlock bodies: sums, then exclusive offsets
-    uint32_t* d_lists = nullptr;    // list_sub [nchunks], list_cross [nchunks], list_vert [nvchunks], counts [4]
+    BodyChunks* d_bch = nullptr;    // per body: its chunk range
+    uint2* d_boff = nullptr;        // per body: this step's offsets in the two lists
+    uint2* d_scan = nullptr;        // per block of kScanBlock bodies: sums, then exclusive offsets
+    uint32_t* d_lists = nullptr;    // list_sub [nchunks], list_cross [nchunks], counts [4]
     BodyDyn* d_dyn = nullptr;
     double* d_partials = nullptr;
     hipGraph_t graph = nullptr;
@@ -1622,6 +1614,15 @@ extern "C" int sfh_mesh_create(const void* vertex_pos, size_t vertex_stride, uin
     m->id = g_mesh_ids.fetch_add(1);
     m->nv = nv;
     m->nf = nf;
+    m->blkwin.resize((nf + 63) / 64);
+    for (size_t k = 0; k < m->blkwin.size(); ++k) {
+        uint32_t lo = UINT32_MAX, hi = 0;
+        for (size_t i = 3 * 64 * k; i < std::min(3 * 64 * (k + 1), 3 * (size_t)nf); ++i) {
+            lo = std::min(lo, faces[i]);
+            hi = std::max(hi, faces[i]);
+        }
+        m->blkwin[k] = make_uint2(lo, hi);
+    }
     for (int k = 0; k < 3; ++k) { m->aabb_min[k] = mn[k]; m->aabb_max[k] = mx[k]; }
     if (nv) {
         hipError_t e = hipMalloc(&m->d_verts, verts.size() * sizeof(float));
@@ -1631,7 +1632,13 @@ extern "C" int sfh_mesh_create(const void* vertex_pos, size_t vertex_stride, uin
     if (nf) {
         hipError_t e = hipMalloc(&m->d_faces, faces.size() * sizeof(uint32_t));
         if (e == hipSuccess) e = hipMemcpy(m->d_faces, faces.data(), faces.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-        if (e != hipSuccess) { hipFree(m->d_verts); hipFree(m->d_faces); delete m; return fail(SFH_ERR_HIP, "mesh upload: %s", hipGetErrorString(e)); }
+        if (e == hipSuccess) e = hipMalloc(&m->d_blkwin, m->blkwin.size() * sizeof(uint2));
+        if (e == hipSuccess)
+            e = hipMemcpy(m->d_blkwin, m->blkwin.data(), m->blkwin.size() * sizeof(uint2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(m->d_verts); hipFree(m->d_faces); hipFree(m->d_blkwin); delete m;
+            return fail(SFH_ERR_HIP, "mesh upload: %s", hipGetErrorString(e));
+        }
     }
     *out = m;
     return SFH_OK;
@@ -1768,8 +1775,6 @@ static void batch_free(sfh_batch_t* b)
     hipFree(b->d_hout);
     b->d_hstates = b->d_hout = nullptr;
     hipFree(b->d_chunks);
-    hipFree(b->d_vchunks);
-    hipFree(b->d_depth);
     hipFree(b->d_bpos);
     hipFree(b->d_bch);
     hipFree(b->d_boff);
@@ -1795,57 +1800,61 @@ static uint32_t default_waves()
     return (uint32_t)cus * 64u;
 }
 
+// Vertex window of the faces [first, first + count) of a mesh from its
+// per-64-face block windows: (lowest id, number of ids), or (0, 0) when the
+// window is larger than `cap` (the crossing pass then evaluates depths per
+// face vertex).
+static void chunk_window(const std::vector<uint2>& blk, uint32_t first, uint32_t count, uint32_t cap, uint32_t& vlo,
+                         uint32_t& vn)
+{
+    uint32_t lo = UINT32_MAX, hi = 0;
+    for (uint32_t k = first / 64; k < (first + count + 63) / 64 && k < blk.size(); ++k) {
+        lo = std::min(lo, blk[k].x);
+        hi = std::max(hi, blk[k].y);
+    }
+    vlo = 0;
+    vn = 0;
+    if (lo <= hi && hi - lo + 1 <= cap) {
+        vlo = lo;
+        vn = hi - lo + 1;
+    }
+}
+
 // (Re)builds the work lists: every body's faces cut into chunks of 64*K
-// consecutive faces (the face-prefix work list of the face pass) and its
-// vertices into runs of 512 (the depth pass), plus each body's offset in the
-// vertex-depth buffer.
+// consecutive faces (the face-prefix work list of the face passes), each
+// with its vertex window.
 static int batch_build_chunks(sfh_batch_t* b)
 {
-    std::vector<Chunk> chunks, vchunks;
-    const uint32_t csize = 64u * b->K, vsize = 512u;
-    uint64_t dep = 0;
+    std::vector<Chunk> chunks;
+    const uint32_t csize = 64u * b->K;
     for (uint32_t i = 0; i < b->n; ++i) {
         BodyStatic& s = b->h_static[i];
+        const sfh_mesh_t* m = i < b->meshes_by_body.size() ? b->meshes_by_body[i] : nullptr;
         s.chunk0 = (uint32_t)chunks.size();
         s.nchunks = (s.nf + csize - 1) / csize;
         for (uint32_t c = 0; c < s.nchunks; ++c) {
             Chunk ch;
+            memset(&ch, 0, sizeof(ch));
             ch.body = i;
             ch.first = c * csize;
             ch.count = std::min(csize, s.nf - c * csize);
-            ch.pad = 0;
+            if (m) chunk_window(m->blkwin, ch.first, ch.count, win_cap(b->K), ch.vlo, ch.vn);
             chunks.push_back(ch);
-        }
-        s.dep_off = dep;
-        dep += s.nv;
-        s.vchunk0 = (uint32_t)vchunks.size();
-        s.nvchunks = (s.nv + vsize - 1) / vsize;
-        for (uint32_t v = 0; v < s.nv; v += vsize) {
-            Chunk ch;
-            ch.body = i;
-            ch.first = v;
-            ch.count = std::min(vsize, s.nv - v);
-            ch.pad = 0;
-            vchunks.push_back(ch);
         }
     }
     hipFree(b->d_static);
     hipFree(b->d_sidx);
     hipFree(b->d_chunks);
-    hipFree(b->d_vchunks);
-    hipFree(b->d_depth);
     hipFree(b->d_partials);
     hipFree(b->d_bpos);
     hipFree(b->d_bch);
     hipFree(b->d_boff);
     hipFree(b->d_scan);
     hipFree(b->d_lists);
-    b->d_static = nullptr; b->d_sidx = nullptr; b->d_chunks = nullptr; b->d_vchunks = nullptr; b->d_depth = nullptr;
+    b->d_static = nullptr; b->d_sidx = nullptr; b->d_chunks = nullptr;
     b->d_partials = nullptr; b->d_bpos = nullptr; b->d_bch = nullptr; b->d_boff = nullptr; b->d_lists = nullptr;
     b->d_scan = nullptr;
     b->nchunks = (uint32_t)chunks.size();
-    b->nvchunks = (uint32_t)vchunks.size();
-    b->nverts = dep;
     HIPCHK(hipMalloc(&b->d_static, sizeof(BodyStatic) * std::max<size_t>(1, b->n)));
     HIPCHK(hipMemcpy(b->d_static, b->h_static.data(), sizeof(BodyStatic) * b->n, hipMemcpyHostToDevice));
     {
@@ -1857,10 +1866,6 @@ static int batch_build_chunks(sfh_batch_t* b)
     HIPCHK(hipMalloc(&b->d_chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size())));
     if (!chunks.empty())
         HIPCHK(hipMemcpy(b->d_chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&b->d_vchunks, sizeof(Chunk) * std::max<size_t>(1, vchunks.size())));
-    if (!vchunks.empty())
-        HIPCHK(hipMemcpy(b->d_vchunks, vchunks.data(), sizeof(Chunk) * vchunks.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&b->d_depth, sizeof(float) * std::max<uint64_t>(1, dep)));
     HIPCHK(hipMalloc(&b->d_partials, sizeof(double) * kChunkDoubles * std::max<size_t>(1, chunks.size())));
     HIPCHK(hipMalloc(&b->d_bpos, std::max<size_t>(1, b->n)));
     {
@@ -1868,14 +1873,13 @@ static int batch_build_chunks(sfh_batch_t* b)
         for (uint32_t i = 0; i < b->n; ++i) {
             const BodyStatic& st = b->h_static[i];
             bch[i].chunk0 = st.chunk0; bch[i].nchunks = st.nchunks;
-            bch[i].vchunk0 = st.vchunk0; bch[i].nvchunks = st.nvchunks;
         }
         HIPCHK(hipMalloc(&b->d_bch, sizeof(BodyChunks) * std::max<size_t>(1, b->n)));
         if (b->n) HIPCHK(hipMemcpy(b->d_bch, bch.data(), sizeof(BodyChunks) * b->n, hipMemcpyHostToDevice));
     }
-    HIPCHK(hipMalloc(&b->d_boff, sizeof(uint4) * std::max<size_t>(1, b->n)));
-    HIPCHK(hipMalloc(&b->d_scan, sizeof(uint4) * std::max<size_t>(1, (b->n + kScanBlock - 1) / kScanBlock)));
-    HIPCHK(hipMalloc(&b->d_lists, sizeof(uint32_t) * (2 * (size_t)b->nchunks + b->nvchunks + 8)));
+    HIPCHK(hipMalloc(&b->d_boff, sizeof(uint2) * std::max<size_t>(1, b->n)));
+    HIPCHK(hipMalloc(&b->d_scan, sizeof(uint2) * std::max<size_t>(1, (b->n + kScanBlock - 1) / kScanBlock)));
+    HIPCHK(hipMalloc(&b->d_lists, sizeof(uint32_t) * (2 * (size_t)b->nchunks + 8)));
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
     return SFH_OK;
@@ -1884,8 +1888,8 @@ static int batch_build_chunks(sfh_batch_t* b)
 static int apply_opts(sfh_batch_t* b, const sfh_batch_opts* opts)
 {
     uint32_t K = opts && opts->faces_per_lane ? opts->faces_per_lane : 16u;
-    if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16)
-        return fail(SFH_ERR_INVALID, "faces_per_lane must be 1, 2, 4, 8 or 16");
+    if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16 && K != 32)
+        return fail(SFH_ERR_INVALID, "faces_per_lane must be 1, 2, 4, 8, 16 or 32");
     b->waves = opts && opts->waves ? opts->waves : default_waves();
     if (K != b->K || !b->d_chunks) {
         b->K = K;
@@ -2226,29 +2230,31 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
     }
     uint32_t* list_sub = b->d_lists;
     uint32_t* list_cross = list_sub + b->nchunks;
-    uint32_t* list_vert = list_cross + b->nchunks;
-    uint32_t* counts = list_vert + b->nvchunks;
-    const uint32_t nblk = std::max<uint32_t>(1, (b->n + kScanBlock - 1) / kScanBlock);
-    hipLaunchKernelGGL(k_scan_part, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan);
-    hipLaunchKernelGGL(k_scan_off, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan,
-                       b->d_boff, counts);
-    const uint32_t swaves = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
-    hipLaunchKernelGGL(k_scatter, dim3((swaves + 3) / 4), dim3(256), 0, s, b->d_bpos, b->d_bch, b->d_boff, b->n,
-                       list_sub, list_cross, list_vert);
-    if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
-    uint32_t wv = std::min<uint32_t>(b->waves, std::max<uint32_t>(1, b->nvchunks));
-    hipLaunchKernelGGL(k_depths, dim3((wv + 3) / 4), dim3(256), 0, s, list_vert, counts + 2, b->d_vchunks,
-                       b->d_static, b->d_dyn, oc, b->d_depth);
-    if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
+    uint32_t* counts = list_cross + b->nchunks;
+    if (!b->fixed_pos) {
+        const uint32_t nblk = std::max<uint32_t>(1, (b->n + kScanBlock - 1) / kScanBlock);
+        hipLaunchKernelGGL(k_scan_part, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan);
+        hipLaunchKernelGGL(k_scan_off, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan,
+                           b->d_boff, counts);
+        const uint32_t swaves = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
+        hipLaunchKernelGGL(k_scatter, dim3((swaves + 3) / 4), dim3(256), 0, s, b->d_bpos, b->d_bch, b->d_boff, b->n,
+                           list_sub, list_cross);
+    }
+    if (timed) {
+        HIPCHK(hipEventRecord(b->ev[1], s));
+        HIPCHK(hipEventRecord(b->ev[2], s));
+    }
     uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
     uint32_t blocks = (waves + 3) / 4;
 #define LAUNCH_CROSS(KK, NU, NL)                                                                          \
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_cross<KK, NU, NL>), dim3(blocks), dim3(256), 0, s, list_cross,  \
-                       counts + 1, b->d_chunks, b->d_static, b->d_dyn, oc, b->d_depth, b->d_partials);
+                       counts + 1, b->d_chunks, b->d_static, b->d_dyn, oc, b->d_partials);
 #define LAUNCH_KN(KK, NU)                                                                                 \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, NU>), dim3(blocks), dim3(256), 0, s, list_sub, counts,  \
-                       b->d_chunks, b->d_static, b->d_dyn, b->d_partials);                                    \
-    if (NU || oc.nlayers > 2) {                                                                           \
+    if (b->fixed_pos != 2)                                                                                \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, NU>), dim3(blocks), dim3(256), 0, s, list_sub,     \
+                           counts, b->d_chunks, b->d_static, b->d_dyn, b->d_partials);                        \
+    if (b->fixed_pos == 1) {                                                                              \
+    } else if (NU || oc.nlayers > 2) {                                                                    \
         LAUNCH_CROSS(KK, NU, -1)                                                                          \
     } else if (oc.nlayers == 0) {                                                                         \
         LAUNCH_CROSS(KK, false, 0)                                                                        \
@@ -2273,7 +2279,7 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool ti
         }                                                                                            \
         break;
     switch (b->K) {
-        LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16)
+        LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16) LAUNCH_K(32)
         default: return fail(SFH_ERR_INVALID, "bad K");
     }
 #undef LAUNCH_K
@@ -2349,6 +2355,90 @@ extern "C" int sfh_batch_compute_masked(sfh_batch b, sfh_ocean o, const double* 
     rc = batch_launch(b, o, d_states, 0, d_out, s, timed, d_active);
     if (rc == SFH_OK && timed) b->ev_pending = true;
     return rc;
+}
+
+// ---- autotuner ------------------------------------------------------------------
+
+// Times the batch's whole step on its own inputs for each (faces per lane,
+// resident wavefronts per CU) candidate and keeps the fastest.  The step is
+// timed with HIP events on the caller's stream around `reps` eager launches
+// after one untimed launch; the face-pass time of the last launch comes from
+// the batch's own event pair.  A batch's work lists depend on the faces per
+// lane, so a candidate with another value rebuilds them (and any graph the
+// caller captured from the batch before must be captured again).
+extern "C" int sfh_batch_autotune(sfh_batch b, sfh_ocean o, const double* d_states, const uint8_t* d_active,
+                                  double* d_out, void* stream, uint32_t reps, sfh_tune_point* points,
+                                  uint32_t max_points, uint32_t* npoints)
+{
+    if (!b || !d_states || !d_out) return fail(SFH_ERR_INVALID, "null argument");
+    int rc = check_ocean(o);
+    if (rc != SFH_OK) return rc;
+    if (reps == 0) reps = 5;
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, cus = 256;
+    HIPCHK(hipGetDevice(&dev));
+    {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
+    }
+    static const uint32_t kK[] = {4, 8, 16, 32};
+    static const uint32_t kWpc[] = {8, 16, 24, 32, 48, 64, 96};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const bool timing = b->timing;
+    if (!b->ev[0])
+        for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&b->ev[i]));
+    double best = 1e30;
+    sfh_batch_opts best_opts = {b->K, b->waves};
+    uint32_t np = 0;
+    for (uint32_t K : kK) {
+        for (uint32_t wpc : kWpc) {
+            sfh_batch_opts opts = {K, wpc * (uint32_t)cus};
+            rc = apply_opts(b, &opts);
+            if (rc != SFH_OK) break;
+            rc = batch_launch(b, o, d_states, 0, d_out, s, false, d_active);   // warm-up
+            if (rc != SFH_OK) break;
+            hipEventRecord(e0, s);
+            for (uint32_t r = 0; r < reps && rc == SFH_OK; ++r)
+                rc = batch_launch(b, o, d_states, 0, d_out, s, r + 1 == reps, d_active);
+            hipEventRecord(e1, s);
+            if (rc != SFH_OK) break;
+            hipError_t e = hipEventSynchronize(e1);
+            float ms = 0.f, face = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&face, b->ev[2], b->ev[3]);
+            if (e != hipSuccess) { rc = fail(SFH_ERR_HIP, "autotune timing: %s", hipGetErrorString(e)); break; }
+            const double step = (double)ms / reps;
+            if (points && np < max_points) {
+                points[np].faces_per_lane = K;
+                points[np].waves_per_cu = wpc;
+                points[np].step_ms = step;
+                points[np].face_ms = face;
+            }
+            ++np;
+            if (step < best) {
+                best = step;
+                best_opts = opts;
+            }
+        }
+        if (rc != SFH_OK) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    b->ev_pending = false;
+    b->timing = timing;
+    const int rc2 = apply_opts(b, &best_opts);
+    if (npoints) *npoints = std::min(np, max_points);
+    return rc != SFH_OK ? rc : rc2;
+}
+
+extern "C" int sfh_batch_get_opts(sfh_batch b, sfh_batch_opts* opts)
+{
+    if (!b || !opts) return fail(SFH_ERR_INVALID, "null argument");
+    opts->faces_per_lane = b->K;
+    opts->waves = b->waves;
+    return SFH_OK;
 }
 
 // ---- tick driver ---------------------------------------------------------------
@@ -2497,111 +2587,123 @@ extern "C" int sfh_batch_kernel_ms(sfh_batch b, double* face_ms, double* total_m
 // ---- per-body evaluations ------------------------------------------------------
 
 // One per-body evaluation (a single-body batch) needs, beyond the mesh: the
-// body's static record and per-step constants, its chunk lists, the work
-// lists, the chunk partials and the vertex depths.  They are carved out of
-// one device buffer sized for meshes of up to F faces and V vertices: a
+// body's static record and per-step constants, its chunk list with the
+// chunks' vertex windows, the work lists and the chunk partials.  They are
+// carved out of one device buffer sized for meshes of up to F faces: a
 // workspace of the per-body statics (device memory from hipMalloc, grown on
-// demand) or the no-alloc entry point's file-scope array.  The static
-// record and chunk lists are written by a kernel from its arguments, so a
-// call needs no host-to-device copy of them.
+// demand) or the no-alloc entry point's file-scope array.  A per-body call
+// knows the body's position (the Surface call is the crossing loop, the
+// Submerged and aerodynamic calls the submerged one), so one kernel writes
+// the static record, the chunks and the work list from its arguments and
+// the mesh's block windows, and the step is that kernel, the prologue, one
+// face pass and the epilogue.
 static constexpr uint32_t kSingleK = 8;   // faces per lane of per-body calls
 
 struct SingleHead {
     BodyStatic st;
     BodyDyn dyn;
     BodyChunks bch;
-    uint4 boff;
-    uint4 scan;
+    uint2 boff;
+    uint2 scan;
     uint32_t sidx;
     int8_t bpos;
 };
 
 __host__ __device__ constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 __host__ __device__ constexpr uint32_t single_nchunks(uint32_t F) { return (F + 64u * kSingleK - 1) / (64u * kSingleK); }
-__host__ __device__ constexpr uint32_t single_nvchunks(uint32_t V) { return (V + 511u) / 512u; }
 
-// Byte offsets of the parts in a buffer for (F, V): head, lists, chunks,
-// vchunks, partials, depth, end.
+// Byte offsets of the parts in a buffer for F faces: head, lists, chunks,
+// partials, end.
 struct SingleLayout {
-    size_t lists, chunks, vchunks, partials, depth, end;
+    size_t lists, chunks, partials, end;
 };
 
-__host__ __device__ constexpr SingleLayout single_layout(uint32_t F, uint32_t V)
+__host__ __device__ constexpr SingleLayout single_layout(uint32_t F)
 {
-    const size_t nc = single_nchunks(F), nvc = single_nvchunks(V);
+    const size_t nc = single_nchunks(F);
     const size_t lists = align256(sizeof(SingleHead));
-    const size_t chunks = align256(lists + sizeof(uint32_t) * (2 * nc + nvc + 8));
-    const size_t vchunks = align256(chunks + sizeof(Chunk) * (nc ? nc : 1));
-    const size_t partials = align256(vchunks + sizeof(Chunk) * (nvc ? nvc : 1));
-    const size_t depth = align256(partials + sizeof(double) * kChunkDoubles * (nc ? nc : 1));
-    const size_t end = align256(depth + sizeof(float) * (V ? V : 1));
-    return SingleLayout{lists, chunks, vchunks, partials, depth, end};
+    const size_t chunks = align256(lists + sizeof(uint32_t) * (2 * nc + 8));
+    const size_t partials = align256(chunks + sizeof(Chunk) * (nc ? nc : 1));
+    const size_t end = align256(partials + sizeof(double) * kChunkDoubles * (nc ? nc : 1));
+    return SingleLayout{lists, chunks, partials, end};
 }
 
-// Writes the body's static record and its face and vertex chunk lists.
+// Writes the body's static record, its chunks with their vertex windows
+// (from the mesh's per-64-face block windows) and the work list of the pass
+// its position selects (`pos`: 1 submerged, 2 crossing) with the counts.
 __global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H, Chunk* __restrict__ chunks,
-                                                      Chunk* __restrict__ vchunks, BodyStatic st)
+                                                      uint32_t* __restrict__ lists, const uint2* __restrict__ blkwin,
+                                                      BodyStatic st, int pos)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t csize = 64u * kSingleK;
+    const uint32_t nc = st.nchunks;
     if (t == 0) {
         H->st = st;
-        H->bch.chunk0 = 0; H->bch.nchunks = st.nchunks;
-        H->bch.vchunk0 = 0; H->bch.nvchunks = st.nvchunks;
+        H->bch.chunk0 = 0;
+        H->bch.nchunks = nc;
         H->sidx = 0;
+        uint32_t* counts = lists + 2 * nc;
+        counts[0] = pos == 1 ? nc : 0u;
+        counts[1] = pos == 2 ? nc : 0u;
     }
-    if (t < st.nchunks) {
+    if (t < nc) {
         Chunk c;
-        c.body = 0; c.first = t * csize; c.count = min(csize, st.nf - t * csize); c.pad = 0;
+        c.body = 0;
+        c.first = t * csize;
+        c.count = min(csize, st.nf - t * csize);
+        uint32_t lo = 0xffffffffu, hi = 0;
+        const uint32_t nblk = (st.nf + 63) / 64;
+        for (uint32_t k = t * kSingleK; k < min((t + 1) * kSingleK, nblk); ++k) {
+            const uint2 w = blkwin[k];
+            lo = min(lo, w.x);
+            hi = max(hi, w.y);
+        }
+        const bool fits = lo <= hi && hi - lo + 1 <= win_cap(kSingleK);
+        c.vlo = fits ? lo : 0u;
+        c.vn = fits ? hi - lo + 1 : 0u;
+        c.pad[0] = c.pad[1] = c.pad[2] = 0;
         chunks[t] = c;
-    }
-    if (t < st.nvchunks) {
-        Chunk c;
-        c.body = 0; c.first = t * 512u; c.count = min(512u, st.nv - t * 512u); c.pad = 0;
-        vchunks[t] = c;
+        lists[(pos == 2 ? nc : 0u) + t] = t;
     }
 }
 
 // One evaluation of `m` with `flags` from the 38-double state d_state into
 // the output row d_out, on stream s, in the buffer `buf` laid out for the
-// mesh's (F, V).  No allocation.
+// mesh's face count.  No allocation.
 static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags, const sfh_ocean_t* o,
                          const double* d_state, double* d_out, hipStream_t s, uint32_t waves)
 {
-    const SingleLayout L = single_layout(m->nf, m->nv);
+    const SingleLayout L = single_layout(m->nf);
     SingleHead* H = (SingleHead*)buf;
     Chunk* chunks = (Chunk*)(buf + L.chunks);
-    Chunk* vchunks = (Chunk*)(buf + L.vchunks);
+    uint32_t* lists = (uint32_t*)(buf + L.lists);
     BodyStatic st = make_static(m, flags);
     st.chunk0 = 0;
     st.nchunks = single_nchunks(m->nf);
-    st.vchunk0 = 0;
-    st.nvchunks = single_nvchunks(m->nv);
     st.sidx = 0;
-    st.dep_off = 0;
-    const uint32_t nt = std::max(std::max(st.nchunks, st.nvchunks), 1u);
-    hipLaunchKernelGGL(k_single_setup, dim3((nt + 255) / 256), dim3(256), 0, s, H, chunks, vchunks, st);
+    const int pos = (flags & BF_MODE_SURF) ? 2 : 1;   // k_prologue's position for these modes
+    const uint32_t nt = std::max(st.nchunks, 1u);
+    hipLaunchKernelGGL(k_single_setup, dim3((nt + 255) / 256), dim3(256), 0, s, H, chunks, lists, m->d_blkwin, st,
+                       pos);
     // a batch view of the buffer: no allocation (empty vectors, borrowed pointers)
     sfh_batch_t b;
     b.n = 1;
     b.nunits = 1;
     b.aero = (flags & BF_MODE_AERO) != 0;
+    b.fixed_pos = pos;
     b.nchunks = st.nchunks;
-    b.nvchunks = st.nvchunks;
     b.nfaces = m->nf;
-    b.nverts = m->nv;
     b.K = kSingleK;
     b.waves = waves;
     b.d_static = &H->st;
     b.d_sidx = &H->sidx;
     b.d_chunks = chunks;
-    b.d_vchunks = vchunks;
-    b.d_depth = (float*)(buf + L.depth);
     b.d_bpos = &H->bpos;
     b.d_bch = &H->bch;
     b.d_boff = &H->boff;
     b.d_scan = &H->scan;
-    b.d_lists = (uint32_t*)(buf + L.lists);
+    b.d_lists = lists;
     b.d_dyn = &H->dyn;
     b.d_partials = (double*)(buf + L.partials);
     return batch_launch(&b, o, d_state, 1, d_out, s, false);
@@ -2672,10 +2774,10 @@ static void ws_release(Workspace* w)
     g_ws_free[w->device].push_back(w);
 }
 
-// Grows the workspace's buffer to the layout of an (F, V) mesh.
-static int ws_reserve(Workspace* w, uint32_t F, uint32_t V)
+// Grows the workspace's buffer to the layout of an F-face mesh.
+static int ws_reserve(Workspace* w, uint32_t F)
 {
-    const size_t need = single_layout(F, V).end;
+    const size_t need = single_layout(F).end;
     if (need <= w->cap) return SFH_OK;
     const size_t cap = std::max(need, 2 * w->cap);
     HIPCHK(hipStreamSynchronize(w->stream));
@@ -2695,7 +2797,7 @@ static int single_call(sfh_mesh_t* m, uint32_t flags, const sfh_ocean_t* o, cons
     Workspace* ws = nullptr;
     int rc = ws_acquire(&ws);
     if (rc != SFH_OK) return rc;
-    rc = ws_reserve(ws, m->nf, m->nv);
+    rc = ws_reserve(ws, m->nf);
     if (rc == SFH_OK) {
         double* h = ws->h_io;
         memcpy(h, T_CG, 16 * sizeof(double));
@@ -2823,14 +2925,12 @@ extern "C" int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double
 // ---- per-body call that allocates nothing ----------------------------------------
 
 // The buffer of the no-alloc entry point: the per-body layout for meshes of
-// up to kNaMaxFaces faces and kNaMaxVerts vertices, in one file-scope device
-// array.  Each GPU the library runs on holds its own copy until the process
+// up to kNaMaxFaces faces, in one file-scope device array.  Each GPU the library runs on holds its own copy until the process
 // ends; every call on a GPU shares it, so calls that overlap on one GPU (two
 // streams, or two host threads) race.
 static constexpr uint32_t kNaMaxFaces = 1u << 19;   // 524,288
-static constexpr uint32_t kNaMaxVerts = 1u << 19;
-static constexpr size_t kNaBytes = single_layout(kNaMaxFaces, kNaMaxVerts).end;
-static_assert(kNaBytes <= (5u << 19), "no-alloc scratch above 2.5 MiB");
+static constexpr size_t kNaBytes = single_layout(kNaMaxFaces).end;
+static_assert(kNaBytes <= (256u << 10), "no-alloc scratch above 256 KiB");
 
 static __device__ __align__(256) unsigned char scratch[kNaBytes];
 
@@ -2867,9 +2967,8 @@ extern "C" int sfh_forces_noalloc(const sfh_settings* settings, int mode, sfh_me
     if (mode != SFH_NOALLOC_SURFACE && mode != SFH_NOALLOC_SUBMERGED) return fail(SFH_ERR_INVALID, "mode %d", mode);
     int rc = check_ocean(ocean);
     if (rc != SFH_OK) return rc;
-    if (mesh->nf > kNaMaxFaces || mesh->nv > kNaMaxVerts)
-        return fail(SFH_ERR_SIZE, "mesh of %u faces / %u vertices exceeds the no-alloc limit of %u / %u", mesh->nf,
-                    mesh->nv, kNaMaxFaces, kNaMaxVerts);
+    if (mesh->nf > kNaMaxFaces)
+        return fail(SFH_ERR_SIZE, "mesh of %u faces exceeds the no-alloc limit of %u faces", mesh->nf, kNaMaxFaces);
     unsigned char* buf = nullptr;
     uint32_t waves = 0;
     rc = na_scratch(&buf, &waves);
@@ -2883,7 +2982,7 @@ extern "C" int sfh_forces_noalloc(const sfh_settings* settings, int mode, sfh_me
         flags |= BF_MODE_SUBM | BF_DAMPING;
     }
     // the layout for this mesh fits in the array's: every part's offset and
-    // size grows with F and V
+    // size grows with F
     return single_launch(buf, mesh, flags, ocean, d_state, d_out, (hipStream_t)stream, waves);
 }
 

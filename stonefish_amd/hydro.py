@@ -336,6 +336,23 @@ class BodyBatch:
         opts = N.sfh_batch_opts(faces_per_lane, waves)
         N.check(N.load().sfh_batch_set_opts(self.handle, C.byref(opts)))
 
+    def get_opts(self) -> tuple[int, int]:
+        """(faces_per_lane, waves) the batch runs with."""
+        opts = N.sfh_batch_opts()
+        N.check(N.load().sfh_batch_get_opts(self.handle, C.byref(opts)))
+        return int(opts.faces_per_lane), int(opts.waves)
+
+    def autotune(self, ocn: Ocean, d_states: int, d_out: int, stream: int = 0, d_active: int | None = None,
+                 reps: int = 5) -> list[dict]:
+        """sfh_batch_autotune: times the step over faces_per_lane x waves per
+        CU on these inputs and keeps the fastest; returns the measured points."""
+        pts = (N.sfh_tune_point * 64)()
+        n = C.c_uint32()
+        N.check(N.load().sfh_batch_autotune(self.handle, ocn.handle, C.c_void_p(d_states), C.c_void_p(d_active or 0),
+                                            C.c_void_p(d_out), C.c_void_p(stream), reps, pts, 64, C.byref(n)))
+        return [{"faces_per_lane": p.faces_per_lane, "waves_per_cu": p.waves_per_cu,
+                 "step_ms": round(p.step_ms, 5), "face_ms": round(p.face_ms, 5)} for p in pts[:n.value]]
+
     def compute_host(self, ocn: Ocean, states: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
         """states: (n, 22) float64 = T_CG[16], v[3], omega[3]; returns (n, 24)."""
         st = np.ascontiguousarray(states, dtype=np.float64).reshape(self.n, N.SFH_STATE_DOUBLES)
