@@ -96,7 +96,12 @@ typedef struct {
     /* Error-analysis extras (not outputs of the reference):
      * S[6]  condition scale of Fb, Tb, Fdq, Tdq, Fdf, Tdf: the sum over faces of
      *       the norms of the per-face contributions (for the flat-sea volume
-     *       buoyancy: rho |g| sum|6V|/6 and that times the largest lever arm);
+     *       buoyancy: rho |g| sum|6V|/6 and that times the largest lever arm;
+     *       for the pressure buoyancy with waves A (|depth| + the polygon's
+     *       extent from its centroid), times |r| for its moment; for the
+     *       quadratic drag the size of the term's operands, |vc|^2 A and
+     *       |r| |vc|^2 A, over the faces the flow meets or passes
+     *       tangentially);
      * raw[12] Fdq, Tdq, Fdf, Tdf before CorrectHydrodynamicForces;
      * k[4]  the scalars CorrectHydrodynamicForces multiplied them by (1 if not
      *       applied). */

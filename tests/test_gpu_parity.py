@@ -1,6 +1,6 @@
 """GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
 
-Bound per F/T vector (tests/util.py): |x - x64| <= 1e-6 |x64| + 1e-7 S with S
+Bound per F/T vector (tests/util.py): |x - x64| <= 1e-6 |x64| + 2e-7 S with S
 the sum of the norms of the per-face contributions, carried through
 CorrectHydrodynamicForces for body-level outputs; Swet and Vsub relative 1e-6;
 write masks and body/fluid positions exact.

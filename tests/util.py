@@ -3,12 +3,23 @@
 Error statement of the GPU path against the FP64 oracle, per F/T vector x
 (a sum over faces of per-face contributions x_i):
 
-    |x - x64|  <=  1e-6 |x64|  +  1e-7 S,      S = sum_i |x_i|
+    |x - x64|  <=  1e-6 |x64|  +  2e-7 S,      S = sum_i |x_i|
 
 The first term is the 1e-6 relative bound; the second is what FP32 rounding
-of each contribution (unit roundoff 6e-8) allows when the contributions cancel,
-e.g. the torque of a symmetric hull.  For sums that do not cancel (S ~ |x|)
-the bound is a 1.1e-6 relative bound.  After CorrectHydrodynamicForces
+of each contribution allows when the contributions cancel, e.g. the torque
+of a symmetric hull: a face term is a composition of several FP32 operations
+(centroid, relative velocity, normal velocity, product, cross product), whose
+rounding errors add to a few units of the unit roundoff u = 6e-8; 2e-7 is
+3.4 u.  For sums that do not cancel (S ~ |x|) the bound is a 1.2e-6
+relative bound.  For the quadratic drag S is the size
+of each face term's operands, |vc|^2 A (|r| |vc|^2 A for its moment): the
+normal velocity vc.n is a difference of products of size |vc|, so a face the
+flow passes nearly tangentially carries an FP32 rounding error of order
+u |vc|^2 A however small its own term.  For the pressure buoyancy with waves
+S is A (|d| + e) per wet face, e the wet polygon's extent from its centroid:
+a clipped polygon's points are FP32, so its centroid is known to u e, and
+near the waterline its depth d (a small difference of surface-scale values)
+is of that order (the oracle's orc_result.S).  After CorrectHydrodynamicForces
 (x_out = k(x) x with k = kappa sum_i |d_i| C_i) the bound is carried through
 the correction: |d x_out| <= (k + 2 kappa |C|) |d x_raw|.
 """
@@ -18,7 +29,7 @@ VEC = {"Fb": slice(0, 3), "Tb": slice(3, 6), "Fdq": slice(6, 9), "Tdq": slice(9,
        "Fdf": slice(12, 15), "Tdf": slice(15, 18)}
 NAMES = ("Fb", "Tb", "Fdq", "Tdq", "Fdf", "Tdf")
 RTOL = 1e-6
-STOL = 1e-7
+STOL = 2e-7
 
 
 def norm_rel(x, ref, floor):
