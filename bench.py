@@ -195,6 +195,35 @@ def _time_replays(fn, k: int) -> float:
     return (time.perf_counter() - t0) / k
 
 
+def measure_peaks(dev) -> dict:
+    """Peaks of this GPU measured in the same run: HBM bandwidth of a
+    STREAM-style device copy (2 GiB read, 2 GiB written per pass, best of 5)
+    and the device properties the FP32 peak is priced from."""
+    import torch
+    n = 512 * 1024 * 1024   # floats: 2 GiB
+    a = torch.ones(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del a, b
+    torch.cuda.empty_cache()
+    props = torch.cuda.get_device_properties(dev)
+    return {"hbm_copy_GBs": round(2 * 4 * n / best / 1e9, 1), "hbm_copy_bytes": 2 * 4 * n,
+            "hbm_peak_spec_GBs": HBM_PEAK_GBS,
+            "compute_units": int(props.multi_processor_count), "name": props.name,
+            "gcn_arch": getattr(props, "gcnArchName", ""),
+            "memory_GiB": round(props.total_memory / 2 ** 30, 1)}
+
+
 def single_body_section(args) -> dict:
     """The two single-body workloads: a 2,048-triangle capsule fully submerged
     at 5 m (1,000 steps) and a 20,480-triangle ellipsoid crossing a flat
@@ -355,12 +384,19 @@ def surrogate_section(args) -> dict:
     stream = torch.cuda.current_stream().cuda_stream
     t_exact = _time_replays(lambda: batch.replay(stream), 30)
     step = S.SurrogateStep(batch, ocean, nets, norms, [per] * 8, states)
-    # one kernel per step: a plain launch costs less than a one-node graph
-    # replay, so both are timed and the faster is reported
+    # the surrogate alone is one kernel per step: a plain launch costs less
+    # than a one-node graph replay, so both are timed
     t_eager = _time_replays(step.run, 200)
     step.capture()
     t_graph = _time_replays(step.replay, 200)
-    t_sur = min(t_eager, t_graph)
+    t_sur_only = min(t_eager, t_graph)
+    # the step that delivers outputs: surrogate, exact kernel on the bodies it
+    # flags out of distribution, merge (graph replay and eager, the faster)
+    t_fb_eager = _time_replays(step.run_with_fallback, 100)
+    step.capture(with_fallback=True)
+    t_fb_graph = _time_replays(step.replay, 100)
+    t_sur = min(t_fb_eager, t_fb_graph)
+    n_fallback = int(step.ood.sum())
     # the surrogate's error on this sample against the exact outputs
     err = torch.zeros((nb, 2), dtype=torch.float32, device="cuda")
     step.run()
@@ -391,8 +427,13 @@ def surrogate_section(args) -> dict:
         "throughput_bodies": nb,
         "exact_body_steps_per_s": round(nb / t_exact, 1),
         "surrogate_body_steps_per_s": round(nb / t_sur, 1),
-        "surrogate_us_per_step": {"launch": round(t_eager * 1e6, 2), "graph": round(t_graph * 1e6, 2)},
+        "surrogate_us_per_step": {"with_fallback_launch": round(t_fb_eager * 1e6, 2),
+                                  "with_fallback_graph": round(t_fb_graph * 1e6, 2),
+                                  "surrogate_only_launch": round(t_eager * 1e6, 2),
+                                  "surrogate_only_graph": round(t_graph * 1e6, 2)},
+        "fallback_bodies": n_fallback,
         "speedup": round(t_exact / t_sur, 2),
+        "speedup_surrogate_only": round(t_exact / t_sur_only, 2),
         "throughput_sample_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in sample_err.items()},
         "mfma": {"inference_tflops": round(inf_tflops, 1), "inference_rows": int(x.shape[0]),
                  "inference_kernel_ms": round(ms, 4), "training_tflops": round(train_tflops, 1),
@@ -422,7 +463,7 @@ def rollout_section(args, sea) -> dict:
         parts.append(H.PartDesc(mesh=meshes[id(p.mesh)], origin=p.origin, Cd=p.Cd, Cf=p.Cf))
     body = H.BodyDesc(mesh=None, aabb_min=auv.aabb_min, aabb_max=auv.aabb_max, volume=auv.volume,
                       surface=auv.surface, P_CB=auv.P_CB)
-    batch = H.CompoundBatch([H.CompoundDesc(body, parts)] * n, faces_per_lane=args.rollout_faces_per_lane)
+    batch = H.CompoundBatch([H.CompoundDesc(body, parts)] * n, faces_per_lane=args.rollout_faces_per_lane or 8)
     _, _, nfaces = batch.info()
     ocean, grid = S.make_ocean(sea)
     ds = S.HullDataset(batch, ocean, 1, n, steps=args.rollout_data_steps, holdout_steps=2, seed=31,
@@ -440,16 +481,31 @@ def rollout_section(args, sea) -> dict:
     states = torch.from_numpy(S.sample_states(rng, n, (0.0, sea.wave_len), (0.0, 2.5), v_max=1.0,
                                               w_max=0.5)).cuda()
     # exact kernel alone on the same bodies (graph replay of one evaluation),
-    # in its fastest configuration (16 faces per lane); the rollout's batch
-    # evaluates only the few out-of-distribution bodies per step and prefers
-    # shorter chunks (--rollout-faces-per-lane)
-    exact = batch if args.rollout_faces_per_lane in (0, 16) else H.CompoundBatch([H.CompoundDesc(body, parts)] * n)
+    # in the configuration the autotuner picks for the whole batch; the
+    # rollout's batch evaluates only the few out-of-distribution bodies per
+    # step and is tuned on that masked step (the plan's OOD flags of these
+    # states) instead
+    exact = H.CompoundBatch([H.CompoundDesc(body, parts)] * n)
     out = torch.zeros((n, 24), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    tune = {}
+    if not args.no_autotune:
+        pts = exact.autotune(ocean, states.data_ptr(), out.data_ptr(), stream)
+        tune["exact"] = {"chosen": exact.get_opts(), "points": pts}
+        plan = S.Plan(batch, [net], [norm], [n])
+        ft0 = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+        ood0 = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        plan.run(ocean, states, ft0, ood=ood0)
+        torch.cuda.synchronize()
+        pts = batch.autotune(ocean, states.data_ptr(), out.data_ptr(), stream, d_active=ood0.data_ptr())
+        tune["rollout_masked"] = {"chosen": batch.get_opts(), "points": pts,
+                                  "active_bodies": int(ood0.sum())}
+        plan.close()
+        torch.cuda.synchronize()
     gs = torch.cuda.Stream()
     with torch.cuda.stream(gs):
         exact.capture(ocean, states.data_ptr(), out.data_ptr(), gs.cuda_stream)
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream().cuda_stream
     t_exact = _time_replays(lambda: exact.replay(stream), 10)
     ro = S.MixedRollout(batch, ocean, net, norm, states, auv.mass, auv.inertia, dt=1e-3, audit_every=100)
     for _ in range(3):
@@ -473,7 +529,9 @@ def rollout_section(args, sea) -> dict:
         "ms_per_step": round(1e3 * t_ro, 4),
         "exact_body_steps_per_s": round(n / t_exact, 1),
         "speedup_vs_exact": round(t_exact / t_ro, 2),
-        "faces_per_lane": {"rollout": args.rollout_faces_per_lane or 16, "exact": 16},
+        "faces_per_lane": {"rollout": batch.get_opts()[0], "exact": exact.get_opts()[0]},
+        "waves": {"rollout": batch.get_opts()[1], "exact": exact.get_opts()[1]},
+        "autotune": tune,
         "audit_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in summ.items()},
         "ood_fraction_at_audits": ood_frac,
         "held_out_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in held.items()},
@@ -489,7 +547,10 @@ def main() -> int:
     ap.add_argument("--bodies", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--faces-per-lane", type=int, default=0)
-    ap.add_argument("--rollout-faces-per-lane", type=int, default=8, help="faces per lane of the rollout's batch")
+    ap.add_argument("--rollout-faces-per-lane", type=int, default=0,
+                    help="faces per lane of the rollout's batch before tuning (0: 8)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the given (or default) faces per lane and waves instead of tuning them")
     ap.add_argument("--waves", type=int, default=0, help="resident wavefronts (0 = library default)")
     ap.add_argument("--graph", type=int, default=1, help="replay a captured HIP graph per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -562,6 +623,14 @@ def main() -> int:
     states = [torch.tensor(states_h[k], dtype=torch.float64, device=dev).contiguous() for k in range(nsets)]
     out = torch.zeros((args.bodies, 24), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
+    sweep = None
+    if not args.no_autotune and not args.faces_per_lane and not args.waves:
+        t0 = time.time()
+        sweep = batch.autotune(go, states[0].data_ptr(), out.data_ptr(), stream)
+        torch.cuda.synchronize()
+        args.faces_per_lane, args.waves = batch.get_opts()
+        log(f"autotune: {len(sweep)} points in {time.time() - t0:.1f}s, chose faces_per_lane "
+            f"{args.faces_per_lane}, {args.waves} waves")
 
     graphs = None
     if args.graph:
@@ -633,13 +702,18 @@ def main() -> int:
         "config": {"workload": f"fleet of {args.bodies} bodies (ellipsoids/boxes, half-axes U[0.1,2] m, "
                                f"faces log-U[500,50000]), depth U[-1,3] m, 1024x1024 PM wave grid",
                    "bodies": args.bodies, "faces": int(nfaces), "vertices": int(nv.sum()), "chunks": nchunks,
-                   "graph": bool(args.graph), "faces_per_lane": args.faces_per_lane or 16},
+                   "graph": bool(args.graph), "faces_per_lane": batch.get_opts()[0],
+                   "waves": batch.get_opts()[1],
+                   "autotune": None if sweep is None else {
+                       "chosen": {"faces_per_lane": batch.get_opts()[0], "waves": batch.get_opts()[1]},
+                       "points": sweep}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_faces", "kernel_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),
                      "algorithmic_bytes": face_bytes, "timed_launches": count},
         "host_buffers_body_steps_per_s": round(host_io, 1),
     }
+    res["roofline"]["peaks_measured"] = measure_peaks(dev)
     if not args.no_parity:
         t0 = time.time()
         out_h = out.cpu().numpy()
