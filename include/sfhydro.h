@@ -160,9 +160,9 @@ int sfh_correct(sfh_ocean ocean, double Fdq[3], double Tdq[3], double Fdf[3], do
  *   d_out:   SFH_OUT_DOUBLES doubles laid out as a batch output row; slots
  *            the reference leaves untouched keep their contents, the
  *            written mask and position are stored in slots 20 and 21.
- * Its work buffers live in one file-scope device array of about 2.3 MB per
+ * Its work buffers live in one file-scope device array of under 256 KB per
  * GPU (held until the process ends), sized for meshes of at most 524,288
- * faces and 524,288 vertices; a larger mesh returns SFH_ERR_SIZE.  It calls
+ * faces; a larger mesh returns SFH_ERR_SIZE.  It calls
  * no hipMalloc, hipFree or allocating routine.  Every call on one GPU shares
  * that array: calls that overlap on one GPU (several streams or threads)
  * race, so the caller orders them.  mesh must not be NULL. */
