@@ -968,6 +968,7 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
         // gathers, issue together (indices past the window are clamped to
         // its last vertex; those results are not stored)
         const f3 zero = mk3(0.f, 0.f, 0.f);
+        bool wet = false;
         for (uint32_t v0 = 0; v0 < vn; v0 += 256u) {
             f3 p[4];
 #pragma unroll
@@ -977,11 +978,20 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
             for (int k = 0; k < 4; ++k) d[k] = (float)wq.depth(p[k], zero);
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (v0 + 64u * k + lane < vn) ldep[v0 + 64u * k + lane] = d[k];
+                if (v0 + 64u * k + lane < vn) {
+                    ldep[v0 + 64u * k + lane] = d[k];
+                    wet |= !(d[k] < 0.f);
+                }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (__ballot(wet) == 0ull) {
+            // every vertex of the window is dry: every face of the chunk is
+            // skipped (SolidEntity.cpp:1353), the chunk contributes nothing
+            if (lane < kChunkDoubles) gptr_w(partials)[(size_t)c * kChunkDoubles + lane] = 0.0;
+            return;
+        }
     }
     auto dep = [&](uint32_t i, f3 p) -> float {
         if (WIN) return ldep[i - vlo];
