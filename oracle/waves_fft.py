@@ -16,9 +16,16 @@ of sfh_wave_spectrum / sfh_wave_fft_*; the product never calls it.
 Pinned by tests/test_waves_fft_cpu.py: the butterfly network with exact
 twiddles equals n^2 ifft2 (numpy), a one-line spectrum gives the closed-form
 travelling cosine with the deep-water (capillary-corrected) dispersion, and
-the LCG's 24-bit uniforms.  The Elfouhaily spectrum itself is restated, not
-pinned by an independent closed form (parity unpinned beyond the agreement
-of this restatement with the library's host code and its zero regions).
+the LCG's 24-bit uniforms.  The Elfouhaily spectrum is pinned in its
+structure and energy by what the mathematics fixes (the library's samples,
+independent of this restatement's formula): its propagating half plane and
+low-k boxes are exact zeros, it is mirror symmetric across the wind, on each
+|k| ring it varies as 1 + Delta(k) cos 2 phi with one Delta per ring, it
+carries no energy four peak wavelengths out and its variance peaks near
+kp = g (Omega / U10)^2, and at full development its significant height
+lies within the JONSWAP enhancement (1.05-1.40) of the Pierson-Moskowitz
+0.21 U10^2 / g.  Its short-wave constants (alpha_m, cm, km inside B_h) are
+checked only against this restatement: parity unpinned for those.
 """
 from __future__ import annotations
 

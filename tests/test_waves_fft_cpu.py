@@ -96,3 +96,103 @@ def test_spectrum_argument_checks():
         W.reference_spectrum(1.0, 12)
     with pytest.raises(N.SfhError):
         W.reference_spectrum(1.0, 4, grid_sizes=(893.0, 0.0, 21.0, 11.0))
+
+
+# ---- the spectrum pinned by what the mathematics fixes -----------------------
+# (no use of the restatement's formula: these follow from the structure of a
+# directional wave spectrum S(k) (1 + Delta(k) cos 2 phi) / (2 pi), the
+# propagating half plane, the long-wave cut-off and the energy of a fully
+# developed sea)
+
+def _grid_S(state, passes, grid):
+    """S at the integer wavenumber indices (i, j) of one grid, from the
+    library's samples h = sqrt(S / 2) dk (|sample| = h): S = 2 (h / dk)^2."""
+    n = 1 << passes
+    s12, s34 = W.reference_spectrum(state, passes)
+    s = (s12 if grid < 2 else s34)[..., 2 * (grid % 2):2 * (grid % 2) + 2].astype(np.float64)
+    h = np.hypot(s[..., 0], s[..., 1])
+    dk = 2 * math.pi / WF.DEFAULT_GRIDS[grid]
+    idx = np.where(np.arange(n) >= n // 2, np.arange(n) - n, np.arange(n))
+    return 2.0 * (h / dk) ** 2, idx, dk
+
+
+@pytest.mark.parametrize("state", [0.5, 2.0])
+def test_spectrum_mirror_symmetric_across_the_wind(state):
+    """cos 2 phi is even in phi: S(kx, ky) = S(kx, -ky) on the propagating half
+    plane (the phases of the two samples are independent draws, their
+    magnitudes are not)."""
+    passes = 6
+    n = 1 << passes
+    for grid in range(4):
+        S, idx, _ = _grid_S(state, passes, grid)
+        for y in range(1, n // 2):
+            ym = n - y
+            np.testing.assert_allclose(S[y, :], S[ym, :], rtol=3e-6, atol=1e-12 * S.max())
+
+
+@pytest.mark.parametrize("state", [0.5, 2.0])
+def test_spectrum_angular_form(state):
+    """On one ring |k| = const the spectrum varies as 1 + Delta(k) cos 2 phi:
+    for (i, j) and (j, i) (the same |k|, phi and pi/2 - phi) the ratio
+    r = S(i, j) / S(j, i) gives Delta(k) cos 2 phi = (r - 1) / (r + 1), so every
+    pair on a ring must give the same Delta(k), with |Delta| <= 1 (a tanh).
+    A wrong angular law (cos phi, a sign, a missing factor 2) fails it."""
+    passes = 7
+    n = 1 << passes
+    S = max((_grid_S(state, passes, g)[0] for g in range(4)), key=lambda a: a.max())   # the grid holding the peak
+    floor = 1e-6 * S.max()
+    rings = {}
+    for i in range(1, n // 2):
+        for j in range(i + 1, n // 2):
+            s1, s2 = S[j, i], S[i, j]          # rows y = ky index, columns x = kx index
+            if s1 <= floor or s2 <= floor:
+                continue
+            r = s1 / s2
+            c2 = (i * i - j * j) / (i * i + j * j)   # cos 2 phi of (kx, ky) = (i, j)
+            rings.setdefault(i * i + j * j, []).append((r - 1) / (r + 1) / c2)
+    multi = {k: v for k, v in rings.items() if len(v) > 1}
+    assert len(multi) > 20
+    for v in multi.values():
+        assert max(abs(x) for x in v) <= 1.0 + 1e-5
+        assert max(v) - min(v) <= 2e-4 * max(1.0, abs(np.mean(v)))
+
+
+def test_spectrum_long_wave_cutoff_and_peak():
+    """Wavelengths four times the peak's and longer carry almost no energy
+    (the Pierson-Moskowitz factor exp(-5/4 (kp / k)^2) = 2e-9 at kp / 4
+    against the k^-4 growth of 256), and the spectrum's
+    variance peaks at |k| near kp = g (Omega / U10)^2."""
+    state = 1.5
+    wind = state * 5.0 + 2.0
+    Om = 5.0 * math.exp(-state) + 0.2
+    kp = 9.81 * (Om / wind) ** 2
+    S, idx, dk = _grid_S(state, 7, 0)
+    I, J = np.meshgrid(idx, idx, indexing="xy")
+    k = np.hypot(I, J) * dk
+    m = S > 0
+    assert S[m & (k < kp / 4)].max() < 1e-4 * S.max()
+    # variance per |k| bin (S k dk dphi summed over a ring) peaks within a factor 1.5 of kp
+    kb = np.floor(k / dk).astype(int)
+    var = np.bincount(kb[m], weights=S[m] * dk * dk)
+    kpk = (np.argmax(var) + 0.5) * dk
+    assert kp / 1.5 < kpk < kp * 1.5, (kpk, kp)
+
+
+def test_fully_developed_sea_height():
+    """At full development (inverse wave age Omega = 0.84) the sea is the
+    Pierson-Moskowitz one with Hs = 0.21 U10^2 / g; Elfouhaily's long-wave
+    part carries a JONSWAP peak enhancement (gamma = 1.7 for Omega < 1), which
+    adds energy around the peak, so the sampled spectrum's Hs = 4 sqrt(m0),
+    m0 = sum of S dk^2 over the four grids' samples, lies between 1.05 and
+    1.40 times the PM height.  A propagating half plane without its doubling
+    (Hs / sqrt 2), a missing 2 pi or a wrong power of k falls outside."""
+    Om = 0.84
+    state = -math.log((Om - 0.2) / 5.0)
+    wind = state * 5.0 + 2.0
+    s12, s34 = W.reference_spectrum(state, 7)
+    m0 = 0.0
+    for arr in (s12, s34):
+        a = arr.astype(np.float64)
+        m0 += 2.0 * float(np.sum(a ** 2))     # S dk^2 = 2 h^2 per sample
+    ratio = 4.0 * math.sqrt(m0) / (0.21 * wind ** 2 / 9.81)
+    assert 1.05 < ratio < 1.40, ratio
