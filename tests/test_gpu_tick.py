@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from oracle import hull as HO
+import hull as HO
 from stonefish_amd import hydro as H
 from stonefish_amd import meshes as M
 from stonefish_amd import transforms as T
