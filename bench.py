@@ -552,6 +552,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the given (or default) faces per lane and waves instead of tuning them")
     ap.add_argument("--waves", type=int, default=0, help="resident wavefronts (0 = library default)")
+    ap.add_argument("--split-passes", type=int, default=0,
+                    help="1: submerged and crossing chunks in face launches of their own")
     ap.add_argument("--graph", type=int, default=1, help="replay a captured HIP graph per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true")
@@ -607,7 +609,7 @@ def main() -> int:
     t0 = time.time()
     gms = [H.Mesh(b.mesh) for b in w.bodies]
     descs = [H.BodyDesc(mesh=g, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface) for g, b in zip(gms, w.bodies)]
-    batch = H.BodyBatch(descs, faces_per_lane=args.faces_per_lane, waves=args.waves)
+    batch = H.BodyBatch(descs, faces_per_lane=args.faces_per_lane, waves=args.waves, split_passes=args.split_passes)
     log(f"uploaded {nf.sum() / 1e6:.1f} M faces / {nv.sum() / 1e6:.1f} M vertices in {time.time() - t0:.1f}s")
 
     # wave field built on the GPU, fed to the depth query like the reference's fftData
@@ -624,13 +626,13 @@ def main() -> int:
     out = torch.zeros((args.bodies, 24), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
     sweep = None
-    if not args.no_autotune and not args.faces_per_lane and not args.waves:
+    if not args.no_autotune and not args.faces_per_lane and not args.waves and not args.split_passes:
         t0 = time.time()
         sweep = batch.autotune(go, states[0].data_ptr(), out.data_ptr(), stream)
         torch.cuda.synchronize()
-        args.faces_per_lane, args.waves = batch.get_opts()
+        args.faces_per_lane, args.waves, args.split_passes = batch.get_opts()
         log(f"autotune: {len(sweep)} points in {time.time() - t0:.1f}s, chose faces_per_lane "
-            f"{args.faces_per_lane}, {args.waves} waves")
+            f"{args.faces_per_lane}, {args.waves} waves, split_passes {args.split_passes}")
 
     graphs = None
     if args.graph:
@@ -638,7 +640,8 @@ def main() -> int:
         gstream = torch.cuda.Stream()
         graphs = []
         for k in range(nsets):
-            b = H.BodyBatch(descs, faces_per_lane=args.faces_per_lane, waves=args.waves)
+            b = H.BodyBatch(descs, faces_per_lane=args.faces_per_lane, waves=args.waves,
+                            split_passes=args.split_passes)
             with torch.cuda.stream(gstream):
                 b.capture(go, states[k].data_ptr(), out.data_ptr(), gstream.cuda_stream)
             graphs.append(b)
@@ -703,9 +706,10 @@ def main() -> int:
                                f"faces log-U[500,50000]), depth U[-1,3] m, 1024x1024 PM wave grid",
                    "bodies": args.bodies, "faces": int(nfaces), "vertices": int(nv.sum()), "chunks": nchunks,
                    "graph": bool(args.graph), "faces_per_lane": batch.get_opts()[0],
-                   "waves": batch.get_opts()[1],
+                   "waves": batch.get_opts()[1], "split_passes": batch.get_opts()[2],
                    "autotune": None if sweep is None else {
-                       "chosen": {"faces_per_lane": batch.get_opts()[0], "waves": batch.get_opts()[1]},
+                       "chosen": {"faces_per_lane": batch.get_opts()[0], "waves": batch.get_opts()[1],
+                                  "split_passes": batch.get_opts()[2]},
                        "points": sweep}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

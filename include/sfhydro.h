@@ -197,17 +197,19 @@ typedef struct {
 
 typedef struct {
     uint32_t faces_per_lane;  /* chunk = 64 lanes x faces_per_lane faces; 0 = default */
-    uint32_t waves;           /* resident wavefronts of the persistent launch; 0 = default */
+    uint32_t waves;           /* wavefronts of the face launch; 0 = default */
+    uint32_t split_passes;    /* 0: one face launch runs the submerged and the crossing
+                                 chunks; 1: a launch for each, each at its own occupancy */
 } sfh_batch_opts;
 
 /* Replaces the per-body pool fan-out of SimulationManager::SimulationTickCallback
  * (SimulationManager.cpp:1619-1655) + Ocean::ApplyFluidForces (Ocean.cpp:219-268)
  * + SolidEntity::ComputeHydrodynamicForces: one batch holds many bodies and
- * evaluates all of them in one pass on one stream.  Builds the face-prefix
- * work list (chunks of faces, each owned by one body). */
+ * evaluates all of them in one pass on one stream.  Cuts every body's faces
+ * into chunks (runs of consecutive faces, each owned by one body). */
 int sfh_batch_create(const sfh_body_desc* bodies, uint32_t n, const sfh_batch_opts* opts, sfh_batch* out);
 int sfh_batch_destroy(sfh_batch batch);
-/* A new faces_per_lane rebuilds the batch's work lists: graphs captured from
+/* A new faces_per_lane rebuilds the batch's chunks: graphs captured from
  * the batch before (sfh_batch_capture, or by the caller around its compute)
  * must be captured again. */
 int sfh_batch_set_opts(sfh_batch batch, const sfh_batch_opts* opts);
@@ -215,24 +217,28 @@ int sfh_batch_get_opts(sfh_batch batch, sfh_batch_opts* opts);
 
 typedef struct {
     uint32_t faces_per_lane;   /* chunk of 64 x faces_per_lane faces */
-    uint32_t waves_per_cu;     /* resident wavefronts of the face passes per compute unit */
+    uint32_t waves_per_cu;     /* wavefronts of the face launch per compute unit */
+    uint32_t split_passes;     /* as in sfh_batch_opts */
+    uint32_t pad_;
     double step_ms;            /* average time of the whole step (every launch) */
-    double face_ms;            /* of the two face passes, last timed step */
+    double face_ms;            /* of the face launches, last timed step */
 } sfh_tune_point;
 
 /* No counterpart in the reference (its only knob is the thread pool size,
  * SimulationApp.cpp:47).  Autotuner of the batched step: times the step on
  * the caller's own inputs (d_states; d_active as in sfh_batch_compute_masked,
  * or NULL) for faces_per_lane in {4, 8, 16, 32} x waves_per_cu in {8, 16,
- * 24, 32, 48, 64, 96}, reps timed launches each (0 = 5), and leaves the batch
- * set to the fastest.  d_out is written.  The measured points (up to
+ * 24, 32, 48, 64, 96, 128, 192, 256} (a face launch has at most one
+ * wavefront per chunk) x split_passes in {0, 1} (0 only for an aerodynamic
+ * batch), reps timed launches each (0 = 5), and leaves the batch set to the
+ * fastest.  d_out is written.  The measured points (up to
  * max_points) go to `points`, their number to *npoints.  Synchronous; graphs
  * captured from the batch before must be captured again. */
 int sfh_batch_autotune(sfh_batch batch, sfh_ocean ocean, const double* d_states, const uint8_t* d_active,
                        double* d_out, void* stream, uint32_t reps, sfh_tune_point* points,
                        uint32_t max_points, uint32_t* npoints);
 /* nbodies: rows of the state and output arrays (bodies, or compounds);
- * nchunks, nfaces: the face passes' work (over the parts of compounds). */
+ * nchunks, nfaces: the face pass's work (over the parts of compounds). */
 int sfh_batch_info(sfh_batch batch, uint32_t* nbodies, uint32_t* nchunks, uint64_t* nfaces);
 
 /* ---- compounds ----------------------------------------------------------- */
@@ -258,7 +264,7 @@ typedef struct {
 } sfh_compound_desc;
 
 /* Replaces Compound::ComputeHydrodynamicForces (Compound.cpp:291-437) for a
- * batch of compounds: every part is a body of the face passes (classified
+ * batch of compounds: every part is a body of the face pass (classified
  * with its compound's AABB, T_C = T_CG * T_CG2O * origin * O2C, corrected
  * with its own Cd, Cf and T_O * origin), then one lane per compound composes
  * the part results in part order.  The batch's states and outputs are per
@@ -327,8 +333,8 @@ int sfh_batch_compute(sfh_batch batch, sfh_ocean ocean, const double* d_states, 
 int sfh_batch_compute_host(sfh_batch batch, sfh_ocean ocean, const double* states, double* out);
 
 /* sfh_batch_compute restricted to the bodies with d_active[i] != 0 (device,
- * n bytes; NULL = all).  The work lists of the step hold only the active
- * bodies' faces; the output rows of inactive bodies are not written.  This is
+ * n bytes; NULL = all).  The face pass evaluates only the active bodies'
+ * chunks; the output rows of inactive bodies are not written.  This is
  * the exact fallback of the surrogate (the bodies outside its training range)
  * and its audit. */
 int sfh_batch_compute_masked(sfh_batch batch, sfh_ocean ocean, const double* d_states,
@@ -339,9 +345,9 @@ int sfh_batch_compute_masked(sfh_batch batch, sfh_ocean ocean, const double* d_s
 int sfh_batch_capture(sfh_batch batch, sfh_ocean ocean, const double* d_states, double* d_out, void* stream);
 int sfh_batch_replay(sfh_batch batch, void* stream);
 
-/* Average duration (ms) of the per-face kernel over the last `count`
- * sfh_batch_compute calls made with timing on (HIP events on the launch
- * stream). */
+/* Average durations (ms) over the last `count` sfh_batch_compute calls made
+ * with timing on (HIP events on the launch stream): face_ms of the face
+ * launches, total_ms of the whole step (every launch). */
 int sfh_batch_timing(sfh_batch batch, int enable);
 int sfh_batch_kernel_ms(sfh_batch batch, double* face_ms, double* total_ms, uint32_t* count);
 

@@ -203,7 +203,7 @@ class BodyBatch {
 public:
     explicit BodyBatch(const std::vector<sfh_body_desc>& bodies, uint32_t faces_per_lane = 0)
     {
-        sfh_batch_opts o = {faces_per_lane, 0};
+        sfh_batch_opts o = {faces_per_lane, 0, 0};
         check(sfh_batch_create(bodies.data(), (uint32_t)bodies.size(), &o, &h_));
         n_ = (uint32_t)bodies.size();
     }

@@ -82,12 +82,12 @@ class sfh_aero_compound_desc(C.Structure):
 
 
 class sfh_batch_opts(C.Structure):
-    _fields_ = [("faces_per_lane", C.c_uint32), ("waves", C.c_uint32)]
+    _fields_ = [("faces_per_lane", C.c_uint32), ("waves", C.c_uint32), ("split_passes", C.c_uint32)]
 
 
 class sfh_tune_point(C.Structure):
-    _fields_ = [("faces_per_lane", C.c_uint32), ("waves_per_cu", C.c_uint32), ("step_ms", C.c_double),
-                ("face_ms", C.c_double)]
+    _fields_ = [("faces_per_lane", C.c_uint32), ("waves_per_cu", C.c_uint32), ("split_passes", C.c_uint32),
+                ("pad_", C.c_uint32), ("step_ms", C.c_double), ("face_ms", C.c_double)]
 
 
 class sfh_wave_component(C.Structure):

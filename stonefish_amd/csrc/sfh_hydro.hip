@@ -2,21 +2,23 @@
 // hydrodynamics (SolidEntity.cpp:1239-1850, Ocean.cpp:149-201,
 // OpenGLRealOcean.cpp:280-331, Compound-free body path).
 //
-// One evaluation pass over a batch of bodies is a few launches on one stream:
-//   k_prologue  one lane per body, FP64: C transform, body/fluid classification
-//               (8 AABB corners against the surface), per-body FP32 constants of
-//               the face loop in the mesh frame, FP64 constants of the epilogue;
-//   k_scan_part / k_scan_off / k_scatter
-//               the step's work lists (chunks of inside and crossing bodies);
-//   k_faces_sub / k_faces_cross
-//               one wavefront per chunk of up to 64*K faces of one body: FP32
-//               face arithmetic with FP32 per-lane partials, FP64 wavefront
-//               reduction; one 24-double partial per chunk (the crossing pass
-//               first evaluates the FP64 depths of the chunk's vertex window
-//               into LDS);
-//   k_epilogue  one wavefront per body, FP64: sums the body's chunk partials
-//               (deterministic order), rotates to world, moment about the CoG,
-//               buoyancy, CorrectHydrodynamicForces, output write mask.
+// One evaluation pass over a batch of bodies is three launches on one stream
+// (four for compounds):
+//   k_prologue  eight lanes per body, FP64: C transform, body/fluid
+//               classification (8 AABB corners against the surface), per-body
+//               FP32 constants of the face loop in the mesh frame, FP64
+//               constants of the epilogue; the rows of bodies without face
+//               work (outside, not hydrodynamic, no faces);
+//   k_faces     one wavefront per chunk of up to 64*K faces of one body, the
+//               submerged or the crossing loop as the body's position says:
+//               FP32 face arithmetic with FP32 per-lane partials, FP64
+//               wavefront reduction into one 24-double partial per chunk (a
+//               crossing chunk first evaluates the FP64 depths of its vertex
+//               window into LDS);
+//   k_epilogue  one wavefront per body with face work, FP64: sums the body's
+//               chunk partials (fixed order), rotates to world, moment about
+//               the CoG, buoyancy, CorrectHydrodynamicForces, output write mask;
+//   k_compound  compound batches: one lane per compound composes its parts.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -419,17 +421,20 @@ __device__ int classify64_g(const double* Rcg, const double* tcg, const double* 
 // are spread over them (one memory round trip instead of eight in turn, the
 // body's critical path); they compute the rest redundantly and the first
 // writes the results.
+// The prologue also writes the rows of the bodies without face work (not
+// hydrodynamic, outside, or without faces); the epilogue writes the others'.
 static constexpr int kPrologueLanes = 8;
 
-__global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
-                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active,
-                           const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
-                           int8_t* __restrict__ bpos)
+__device__ void epilogue_finish(const BodyStatic& S, const BodyDyn& D, const OceanK& oc, const double* P,
+                                double* __restrict__ o);
+
+// Body b on its group of kPrologueLanes lanes (`sub` is the lane's index in
+// the group); returns the body's position for this step (-2: left out).
+__device__ __forceinline__ int prologue_body(const BodyStatic* __restrict__ st, const double* __restrict__ states,
+                                             int state_kind, const OceanK& oc, const uint8_t* __restrict__ active,
+                                             const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
+                                             int8_t* __restrict__ bpos, double* __restrict__ out, uint32_t b, int sub)
 {
-    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t b = gt / kPrologueLanes;
-    const int sub = (int)(gt % kPrologueLanes);
-    if (b >= n) return;   // whole groups: blockDim is a multiple of kPrologueLanes
     const BodyStatic& S = st[b];
     // A body the step skips is marked in bpos only: its static record and its
     // BodyDyn row are not touched (the later passes test bpos, and the
@@ -437,7 +442,7 @@ __global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__
     // its activity).
     if (active && !active[sidx[b]]) {
         if (sub == 0) bpos[b] = -2;
-        return;
+        return -2;
     }
     BodyDyn D;
     const double* s = states + (size_t)S.sidx * (state_kind ? 38 : 22);
@@ -538,6 +543,79 @@ __global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__
     for (int i = 0; i < 9; ++i) { D.R[i] = R[i]; D.Rcg[i] = Rcg[i]; }
     for (int i = 0; i < 3; ++i) D.tcg[i] = tcg[i];
     if (sub == 0) dyn[b] = D;
+
+    double* o = out + (size_t)b * SFH_OUT_DOUBLES;
+    if (pos == -1) {          // not a hydrodynamic body: nothing assigned
+        if (sub == 0) { o[20] = 0.0; o[21] = -1.0; }
+    } else if (pos == 0) {    // OUTSIDE: everything zero (SolidEntity.cpp:1810-1821)
+        for (int k = sub; k < 20; k += kPrologueLanes) o[k] = 0.0;
+        if (sub == 0) { o[20] = 255.0; o[21] = 0.0; }
+    } else if (pos > 0 && sub == 0 && S.nchunks == 0) {   // a mesh without faces: empty sums
+        double P[23];
+        for (int k = 0; k < 23; ++k) P[k] = 0.0;
+        epilogue_finish(S, D, oc, P, o);
+    }
+    return pos;
+}
+
+// The step's work list: one array of nc entries (nc = the batch's chunks),
+// the chunks of bodies in position 1 (submerged loop) filled from the front
+// and those of bodies in position 2 (crossing loop) from the back, with the
+// two counts in counts[0..1] (zero before the prologue; the epilogue zeroes
+// them for the next step).  Each wavefront of the prologue sums its bodies'
+// chunk counts and reserves its ranges with one atomic add per list; a
+// body's lanes then write its chunk ids.  The order of the entries depends on
+// the order of the atomics, the results do not: a chunk's partial has its
+// own slot and the epilogue sums a body's partials in chunk order.
+__device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_t* __restrict__ counts,
+                                            uint32_t nc, uint32_t c_sub, uint32_t c_cross, uint32_t chunk0,
+                                            int sub)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t v0 = sub == 0 ? c_sub : 0u, v1 = sub == 0 ? c_cross : 0u;
+    uint32_t i0 = v0, i1 = v1;   // inclusive scan over the wavefront's lanes
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y0 = __shfl_up(i0, off, 64), y1 = __shfl_up(i1, off, 64);
+        if (lane >= off) { i0 += y0; i1 += y1; }
+    }
+    uint32_t base0 = 0, base1 = 0;
+    if (lane == 63) {
+        if (i0) base0 = atomicAdd(counts, i0);
+        if (i1) base1 = atomicAdd(counts + 1, i1);
+    }
+    base0 = __shfl(base0, 63, 64);
+    base1 = __shfl(base1, 63, 64);
+    const int lead = lane & ~(kPrologueLanes - 1);
+    const uint32_t o0 = base0 + __shfl(i0 - v0, lead, 64);
+    const uint32_t o1 = base1 + __shfl(i1 - v1, lead, 64);
+    // the two parts hold at most the batch's nc chunks together; the bounds
+    // tests only guard the array against counts that were not zeroed
+    for (uint32_t i = (uint32_t)sub; i < c_sub; i += kPrologueLanes)
+        if (o0 + i < nc) list[o0 + i] = chunk0 + i;
+    for (uint32_t i = (uint32_t)sub; i < c_cross; i += kPrologueLanes)
+        if (o1 + i < nc) list[nc - 1u - (o1 + i)] = chunk0 + i;
+}
+
+// Every lane of the grid reaches the list build (lanes past the last body
+// contribute nothing), so the wavefront-wide scan sees whole wavefronts.
+__global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
+                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active,
+                           const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
+                           int8_t* __restrict__ bpos, double* __restrict__ out, uint32_t* __restrict__ list,
+                           uint32_t nc, uint32_t* __restrict__ counts)
+{
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = gt / kPrologueLanes;
+    const int sub = (int)(gt % kPrologueLanes);
+    int pos = -3;
+    if (b < n) pos = prologue_body(st, states, state_kind, oc, active, sidx, dyn, bpos, out, b, sub);
+    uint32_t c_sub = 0, c_cross = 0, chunk0 = 0;
+    if (pos == 1 || pos == 2) {
+        chunk0 = st[b].chunk0;
+        if (pos == 1) c_sub = st[b].nchunks;
+        else c_cross = st[b].nchunks;
+    }
+    list_append(list, counts, nc, c_sub, c_cross, chunk0, sub);
 }
 
 // Jets and pipes at a mesh-frame point (Jet.cpp:54-75, Pipe.cpp:57-78): the
@@ -578,10 +656,10 @@ struct CurFlow {
 };
 
 // ---------------------------------------------------------------------------
-// work lists
+// chunks
 // ---------------------------------------------------------------------------
 
-// A work item of the face passes: a run of consecutive faces of one body and
+// A work item of the face pass: a run of consecutive faces of one body and
 // the window of vertex ids they reference (vertices are numbered in order of
 // first use, so a run of faces of a coherent mesh references a compact
 // window).  vn = 0: the window is larger than the crossing pass keeps in LDS.
@@ -594,155 +672,6 @@ struct Chunk {
 // Vertex-window capacity (vertices) of a crossing-pass wavefront for chunks
 // of 64 K faces; the window's FP64 depths are kept in LDS as floats.
 __host__ __device__ constexpr uint32_t win_cap(uint32_t K) { return K <= 16 ? 64u * K : 1536u; }
-
-// Per-step work lists.  k_prologue writes every body's position to bpos; a
-// two-pass prefix sum in body order turns the positions into each body's
-// offsets in the two lists (chunks of inside bodies, chunks of crossing
-// bodies): k_scan_part sums blocks of
-// kScanBlock bodies, k_scan_off adds the sums of the blocks before its own
-// and scans within the block; k_scatter (one wavefront per body,
-// grid-strided) writes the chunk ids.  The passes then take list entries
-// w, w + W, w + 2W, ... (W wavefronts), so every wavefront gets the same
-// number of chunks of its kind.
-static constexpr int kScanThreads = 256;
-static constexpr int kScanPer = 8;                              // consecutive bodies per thread
-static constexpr uint32_t kScanBlock = kScanThreads * kScanPer; // bodies per workgroup
-
-struct BodyChunks {
-    uint32_t chunk0, nchunks;
-};
-
-__device__ __forceinline__ void body_counts(const int8_t* __restrict__ bpos, const BodyChunks* __restrict__ bch,
-                                            uint32_t b, uint32_t& c0, uint32_t& c1)
-{
-    const int p = bpos[b];
-    const BodyChunks bc = bch[b];
-    c0 = (p == 1) ? bc.nchunks : 0u;
-    c1 = (p == 2) ? bc.nchunks : 0u;
-}
-
-// Exclusive scan of two counters over the threads of a workgroup (in thread
-// order); the workgroup totals land in tot.
-template <int NT>
-__device__ __forceinline__ void block_scan2(uint32_t (&v)[2], uint32_t (&tot)[2])
-{
-    __shared__ uint32_t wsum[2][NT / 64];
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint32_t inc[2] = {v[0], v[1]};
-    for (int off = 1; off < 64; off <<= 1)
-        for (int k = 0; k < 2; ++k) {
-            const uint32_t y = __shfl_up(inc[k], off, 64);
-            if ((int)lane >= off) inc[k] += y;
-        }
-    if (lane == 63)
-        for (int k = 0; k < 2; ++k) wsum[k][w] = inc[k];
-    __syncthreads();
-    if (t == 0) {
-        uint32_t run[2] = {0, 0};
-        for (int i = 0; i < NT / 64; ++i)
-            for (int k = 0; k < 2; ++k) { const uint32_t x = wsum[k][i]; wsum[k][i] = run[k]; run[k] += x; }
-        for (int k = 0; k < 2; ++k) tot[k] = run[k];
-    }
-    __syncthreads();
-    for (int k = 0; k < 2; ++k) v[k] = wsum[k][w] + inc[k] - v[k];
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_part(const int8_t* __restrict__ bpos,
-                                                            const BodyChunks* __restrict__ bch, uint32_t n,
-                                                            uint2* __restrict__ part)
-{
-    __shared__ uint32_t red[2][kScanThreads / 64];
-    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
-    uint32_t c[2] = {0, 0};
-    for (int j = 0; j < kScanPer; ++j) {
-        const uint32_t b = b0 + j;
-        if (b < n) {
-            uint32_t x0, x1;
-            body_counts(bpos, bch, b, x0, x1);
-            c[0] += x0; c[1] += x1;
-        }
-    }
-    for (int off = 32; off >= 1; off >>= 1)
-        for (int k = 0; k < 2; ++k) c[k] += __shfl_xor(c[k], off, 64);
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int k = 0; k < 2; ++k) red[k][w] = c[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s[2] = {0, 0};
-        for (int i = 0; i < kScanThreads / 64; ++i)
-            for (int k = 0; k < 2; ++k) s[k] += red[k][i];
-        part[blockIdx.x] = make_uint2(s[0], s[1]);
-    }
-}
-
-// The block's base offset is the sum of the block sums before it (each
-// workgroup adds them itself: a few hundred at most, so no third launch is
-// needed for a scan of the block sums); the last block writes the totals.
-__global__ __launch_bounds__(kScanThreads) void k_scan_off(const int8_t* __restrict__ bpos,
-                                                           const BodyChunks* __restrict__ bch, uint32_t n,
-                                                           const uint2* __restrict__ part, uint2* __restrict__ boff,
-                                                           uint32_t* __restrict__ counts)
-{
-    __shared__ uint32_t red[2][kScanThreads / 64];
-    const uint32_t b0 = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
-    uint32_t pre[2] = {0, 0};
-    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) {
-        const uint2 p = part[j];
-        pre[0] += p.x; pre[1] += p.y;
-    }
-    for (int off = 32; off >= 1; off >>= 1)
-        for (int k = 0; k < 2; ++k) pre[k] += __shfl_xor(pre[k], off, 64);
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 2; ++k) red[k][threadIdx.x >> 6] = pre[k];
-    __syncthreads();
-    uint32_t base[2] = {0, 0};
-    for (int i = 0; i < kScanThreads / 64; ++i) {
-        base[0] += red[0][i]; base[1] += red[1][i];
-    }
-    uint32_t c[2] = {0, 0};
-    for (int j = 0; j < kScanPer; ++j) {
-        const uint32_t b = b0 + j;
-        if (b < n) {
-            uint32_t x0, x1;
-            body_counts(bpos, bch, b, x0, x1);
-            c[0] += x0; c[1] += x1;
-        }
-    }
-    uint32_t tot[2];
-    block_scan2<kScanThreads>(c, tot);
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-        counts[0] = base[0] + tot[0];
-        counts[1] = base[1] + tot[1];
-    }
-    uint32_t o[2] = {base[0] + c[0], base[1] + c[1]};
-    for (int j = 0; j < kScanPer; ++j) {
-        const uint32_t b = b0 + j;
-        if (b < n) {
-            uint32_t x0, x1;
-            body_counts(bpos, bch, b, x0, x1);
-            boff[b] = make_uint2(o[0], o[1]);
-            o[0] += x0; o[1] += x1;
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_scatter(const int8_t* __restrict__ bpos, const BodyChunks* __restrict__ bch,
-                                                 const uint2* __restrict__ boff, uint32_t n,
-                                                 uint32_t* __restrict__ list_sub, uint32_t* __restrict__ list_cross)
-{
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t b = wave; b < n; b += nwaves) {
-        const int p = bpos[b];
-        if (p != 1 && p != 2) continue;
-        const BodyChunks bc = bch[b];
-        const uint2 o = boff[b];
-        uint32_t* L = (p == 1) ? list_sub + o.x : list_cross + o.y;
-        for (uint32_t i = lane; i < bc.nchunks; i += 64) L[i] = bc.chunk0 + i;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // k_faces: one wavefront per chunk of up to 64*K faces of one body
@@ -849,72 +778,57 @@ __device__ __forceinline__ int xreduce12_slot(int lane)
     return 6 * ((lane >> 5) & 1) + 3 * ((lane >> 4) & 1) + g;
 }
 
-// Submerged loop (bodies fully below the surface): per-face arithmetic in FP32
-// with FP32 per-lane partial sums over the chunk's K faces per lane, reduced
-// across the wavefront and across chunks in FP64.  `list` holds the ids of the
-// chunks of this step's inside bodies.
-template <int K, bool NU, bool AERO = false>
-__global__ __launch_bounds__(256, NU ? 1 : 8) void k_faces_sub(const uint32_t* __restrict__ list,
-                                                   const uint32_t* __restrict__ count,
-                                                   const Chunk* __restrict__ chunks,
-                                                   const BodyStatic* __restrict__ st,
-                                                   const BodyDyn* __restrict__ dyn, double* __restrict__ partials)
+// Submerged loop (bodies fully below the surface), one chunk (faces
+// [f0, f0 + nfc) of body S): per-face arithmetic in FP32 with FP32 per-lane
+// partial sums over the lane's K faces, reduced across the wavefront in FP64
+// into the chunk's partial (12 doubles: Fdq Mdq Fdf Mdf).
+template <int K, bool NU, bool AERO>
+__device__ __forceinline__ void sub_chunk(uint32_t c, uint32_t f0, uint32_t nfc, const BodyStatic& S,
+                                          const BodyDyn& D, double* __restrict__ partials, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t n = count[0];
-    for (uint32_t li = wave; li < n; li += nwaves) {
-        const uint32_t c = __builtin_amdgcn_readfirstlane(list[li]);
-        const Chunk ch = chunks[c];
-        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
-        const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
-        const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
-        const BodyStatic& S = st[b];
-        const SFH_GLOBAL float* __restrict__ verts = gptr(S.verts);
-        const SFH_GLOBAL uint32_t* __restrict__ faces = gptr(S.faces) + 3 * (size_t)f0;
-        const FaceBody fb = dyn[b].fb;
-        const CurFlow flow{&dyn[b].cur};
-        AccDrag<float> acc;
-        acc_zero(acc);
-        // Two-stage software pipeline over the lane's K faces: the indices of
-        // face j+2 and the vertices of face j+1 are in flight while face j is
-        // computed.  Face numbers past the chunk are clamped to its last face
-        // so every load stays in bounds; their results are not used.
-        const uint32_t flast = nfc - 1;
-        uint32_t fc0 = min((uint32_t)lane, flast), fc1 = min(64u + lane, flast);
-        uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
-        uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
-        f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
-        for (int j = 0; j < K; ++j) {
-            const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
-            f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
-            uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
-            if (64u * j + lane < nfc) {
-                if (AERO) {
-                    if (NU)
-                        face_aero(fb, p1, p2, p3, flow, acc);
-                    else
-                        face_aero(fb, p1, p2, p3, NoFlow(), acc);
-                } else if (NU) {
-                    face_submerged(fb, p1, p2, p3, flow, acc);
-                } else {
-                    face_submerged(fb, p1, p2, p3, acc);
-                }
+    const SFH_GLOBAL float* __restrict__ verts = gptr(S.verts);
+    const SFH_GLOBAL uint32_t* __restrict__ faces = gptr(S.faces) + 3 * (size_t)f0;
+    const FaceBody fb = D.fb;
+    const CurFlow flow{&D.cur};
+    AccDrag<float> acc;
+    acc_zero(acc);
+    // Two-stage software pipeline over the lane's K faces: the indices of
+    // face j+2 and the vertices of face j+1 are in flight while face j is
+    // computed.  Face numbers past the chunk are clamped to its last face
+    // so every load stays in bounds; their results are not used.
+    const uint32_t flast = nfc - 1;
+    uint32_t fc0 = min((uint32_t)lane, flast), fc1 = min(64u + lane, flast);
+    uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
+    uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
+    f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
+    for (int j = 0; j < K; ++j) {
+        const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
+        f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
+        uint32_t c0 = faces[3 * fn2], c1 = faces[3 * fn2 + 1], c2 = faces[3 * fn2 + 2];
+        if (64u * j + lane < nfc) {
+            if (AERO) {
+                if (NU)
+                    face_aero(fb, p1, p2, p3, flow, acc);
+                else
+                    face_aero(fb, p1, p2, p3, NoFlow(), acc);
+            } else if (NU) {
+                face_submerged(fb, p1, p2, p3, flow, acc);
+            } else {
+                face_submerged(fb, p1, p2, p3, acc);
             }
-            p1 = q1; p2 = q2; p3 = q3;
-            b0 = c0; b1 = c1; b2 = c2;
         }
-        double vals[12];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            vals[k] = acc.Fdq[k]; vals[3 + k] = acc.Mdq[k];
-            vals[6 + k] = acc.Fdf[k]; vals[9 + k] = acc.Mdf[k];
-        }
-        xreduce12(vals, lane);
-        const int q = xreduce12_slot(lane);
-        if (q >= 0) gptr_w(partials)[(size_t)c * kChunkDoubles + q] = vals[0];
+        p1 = q1; p2 = q2; p3 = q3;
+        b0 = c0; b1 = c1; b2 = c2;
     }
+    double vals[12];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        vals[k] = acc.Fdq[k]; vals[3 + k] = acc.Mdq[k];
+        vals[6 + k] = acc.Fdf[k]; vals[9 + k] = acc.Mdf[k];
+    }
+    xreduce12(vals, lane);
+    const int q = xreduce12_slot(lane);
+    if (q >= 0) gptr_w(partials)[(size_t)c * kChunkDoubles + q] = vals[0];
 }
 
 // Contributions of a wet (possibly clipped) face: pressure buoyancy at the
@@ -1055,43 +969,8 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
     if (q >= 0) gptr_w(partials)[(size_t)c * kChunkDoubles + q] = vals[0];
 }
 
-// Surface loop over the chunks of this step's crossing bodies (`list`).
-template <int K, bool NU, int NL>
-__global__ __launch_bounds__(256, NU ? 1 : 5) void k_faces_cross(const uint32_t* __restrict__ list,
-                                                     const uint32_t* __restrict__ count,
-                                                     const Chunk* __restrict__ chunks,
-                                                     const BodyStatic* __restrict__ st,
-                                                     const BodyDyn* __restrict__ dyn, OceanK oc,
-                                                     double* __restrict__ partials)
-{
-    __shared__ uint16_t cutq[4][64 * K];
-    __shared__ float ldep[4][win_cap(K)];
-    const int lane = threadIdx.x & 63;
-    const int wl = threadIdx.x >> 6;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t n = count[0];
-    for (uint32_t li = wave; li < n; li += nwaves) {
-        const uint32_t c = __builtin_amdgcn_readfirstlane(list[li]);
-        const Chunk ch = chunks[c];
-        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
-        const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
-        const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
-        const uint32_t vlo = __builtin_amdgcn_readfirstlane(ch.vlo);
-        const uint32_t vn = __builtin_amdgcn_readfirstlane(ch.vn);
-        const BodyDyn& D = dyn[b];
-        const BodyStatic& S = st[b];
-        WaveQ<NL> wq;
-        wq.load(D, oc);
-        if (vn > 0 && vn <= win_cap(K))
-            cross_chunk<K, NU, NL, true>(c, f0, nfc, vlo, vn, S, D, oc, wq, ldep[wl], cutq[wl], partials, lane);
-        else
-            cross_chunk<K, NU, NL, false>(c, f0, nfc, vlo, vn, S, D, oc, wq, ldep[wl], cutq[wl], partials, lane);
-    }
-}
-
 // ---------------------------------------------------------------------------
-// k_epilogue: one wavefront per body
+// epilogue: one wavefront per body
 // ---------------------------------------------------------------------------
 
 // CorrectHydrodynamicForces (SolidEntity.cpp:1263-1287): M = inverse(basis of
@@ -1153,37 +1032,14 @@ __device__ void correct_aero64(uint32_t type, const double* params, const double
     for (int k = 0; k < 3; ++k) { F[k] *= cor; T[k] *= cor; }
 }
 
-__device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t b,
-                              const OceanK& oc, const double* __restrict__ partials, double* __restrict__ out)
+// Finishes the row o of a body in position 1 or 2 from the sums P of its
+// chunk partials (one lane): drag in world axes with moments about the CoG,
+// buoyancy, CorrectHydrodynamicForces, the write mask.
+__device__ void epilogue_finish(const BodyStatic& S, const BodyDyn& D, const OceanK& oc, const double* P,
+                                double* __restrict__ o)
 {
-    const int lane = threadIdx.x;
-    const BodyStatic& S = st[b];
-    const BodyDyn& D = dyn[b];
-    double* o = out + (size_t)b * SFH_OUT_DOUBLES;
     const int pos = D.pos;
     const uint32_t flags = D.flags;
-    if (pos < 0) {   // not a hydrodynamic body: nothing assigned
-        if (lane == 0) { o[20] = 0.0; o[21] = -1.0; }
-        return;
-    }
-    if (pos == 0) {  // OUTSIDE: everything zero (SolidEntity.cpp:1810-1821)
-        if (lane < 20) o[lane] = 0.0;
-        if (lane == 0) { o[20] = 255.0; o[21] = 0.0; }
-        return;
-    }
-    // sum the body's chunk partials: lanes stride over chunks, then a wave
-    // reduction (fixed order, so the result is deterministic)
-    double P[23];
-#pragma unroll
-    for (int k = 0; k < 23; ++k) P[k] = 0.0;
-    for (uint32_t c = lane; c < S.nchunks; c += 64) {
-        const double* p = partials + (size_t)(S.chunk0 + c) * kChunkDoubles;
-#pragma unroll
-        for (int k = 0; k < 23; ++k) P[k] += p[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 23; ++k) P[k] = wave_sum_f64(P[k]);
-    if (lane != 0) return;
     const bool body_level = !(flags & (BF_MODE_SURF | BF_MODE_SUBM | BF_MODE_AERO));
     const bool damping = flags & BF_DAMPING;
     const bool buoyant = flags & BF_BUOYANT;
@@ -1279,26 +1135,104 @@ __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* 
     o[21] = (double)pos;
 }
 
-// One wavefront per body, bodies strided over the grid; inactive bodies of a
-// masked step are skipped after one byte load.
+// Row of body b (position 1 or 2) on one wavefront: lanes stride over the
+// body's chunk partials and a wavefront reduction sums them in a fixed order,
+// so the row does not depend on which wavefront runs this; lane 0 finishes.
+__device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t b,
+                              const OceanK& oc, const double* partials, double* __restrict__ out, int lane)
+{
+    const BodyStatic& S = st[b];
+    double P[23];
+#pragma unroll
+    for (int k = 0; k < 23; ++k) P[k] = 0.0;
+    for (uint32_t c = lane; c < S.nchunks; c += 64) {
+        const double* p = partials + (size_t)(S.chunk0 + c) * kChunkDoubles;
+#pragma unroll
+        for (int k = 0; k < 23; ++k) P[k] += p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 23; ++k) P[k] = wave_sum_f64(P[k]);
+    if (lane == 0) epilogue_finish(S, dyn[b], oc, P, out + (size_t)b * SFH_OUT_DOUBLES);
+}
+
+// One wavefront per body, bodies strided over the grid: lane k reads the
+// position of the k-th of the wavefront's next 64 bodies in one load and the
+// wavefront finishes the rows of those with face work (positions 1 and 2;
+// the prologue wrote the others', and those of bodies without faces), in
+// order.  A masked step with few active bodies thus costs two loads per 64
+// bodies.
+// It also zeroes the work list's counts for the next step's prologue.
 __global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
                                                  const int8_t* __restrict__ bpos, uint32_t n, OceanK oc,
-                                                 const double* __restrict__ partials, double* __restrict__ out)
+                                                 const double* __restrict__ partials, double* __restrict__ out,
+                                                 uint32_t* __restrict__ counts)
 {
-    // The wavefront's bodies are blockIdx.x + k gridDim.x; lane k reads the
-    // position of the k-th of the next 64 in one load and only the bodies the
-    // step evaluates (position != -2) are visited, in the same order.  A
-    // masked step (few active bodies) thus costs two loads per 64 bodies
-    // instead of a chain of dependent ones.
     const uint32_t G = gridDim.x;
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0 && lane < 2) counts[lane] = 0u;
     for (uint32_t base = blockIdx.x; base < n; base += 64u * G) {
-        const uint32_t bl = base + (uint32_t)threadIdx.x * G;
-        const bool act = bl < n && bpos[bl] != -2;
-        uint64_t m = __ballot(act);
+        const uint32_t bl = base + (uint32_t)lane * G;
+        int p = -3;
+        if (bl < n) p = bpos[bl];
+        uint64_t m = __ballot(p == 1 || p == 2);
         while (m) {
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             m &= m - 1ull;
-            epilogue_body(st, dyn, base + k * G, oc, partials, out);
+            const uint32_t b = base + k * G;
+            if (st[b].nchunks) epilogue_body(st, dyn, b, oc, partials, out, lane);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_faces: one wavefront per chunk
+// ---------------------------------------------------------------------------
+
+// Wavefront w of W takes the entries w, w + W, w + 2W, ... of its PASS's
+// part of the step's work list: kPassSub, the submerged chunks (the list's
+// front, counts[0] entries); kPassCross, the crossing chunks (its back,
+// counts[1] entries); kPassBoth, the crossing chunks then the submerged ones
+// (the longer work first), in one launch; kPassAero, the front with the
+// aerodynamic loop.  kPassSub and kPassCross are two launches, each at its
+// own occupancy.
+static constexpr int kPassSub = 1, kPassCross = 2, kPassBoth = 3, kPassAero = 4;
+
+template <int K, bool NU, int NL, int PASS>
+__global__ __launch_bounds__(256, NU ? 1 : ((PASS & kPassCross) ? 5 : 8)) void k_faces(
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ counts, uint32_t nc,
+    const Chunk* __restrict__ chunks, const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, OceanK oc,
+    double* __restrict__ partials)
+{
+    constexpr bool CROSS = (PASS & kPassCross) != 0;
+    __shared__ uint16_t cutq[4][CROSS ? 64 * K : 1];
+    __shared__ float ldep[4][CROSS ? win_cap(K) : 1];
+    const int lane = threadIdx.x & 63;
+    const int wl = threadIdx.x >> 6;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t n_cross = CROSS ? min(counts[1], nc) : 0u;
+    const uint32_t n_sub = PASS == kPassCross ? 0u : min(counts[0], nc - min(counts[1], nc));
+    const uint32_t* __restrict__ cross_list = list + (nc - n_cross);
+    for (uint32_t i = wave; i < n_cross + n_sub; i += nwaves) {
+        const bool is_cross = i < n_cross;
+        const uint32_t c = __builtin_amdgcn_readfirstlane(is_cross ? cross_list[i] : list[i - n_cross]);
+        const Chunk ch = chunks[c];
+        const uint32_t b = __builtin_amdgcn_readfirstlane(ch.body);
+        const uint32_t f0 = __builtin_amdgcn_readfirstlane(ch.first);
+        const uint32_t nfc = __builtin_amdgcn_readfirstlane(ch.count);
+        const BodyStatic& S = st[b];
+        const BodyDyn& D = dyn[b];
+        if (!CROSS || !is_cross) {
+            sub_chunk<K, NU, PASS == kPassAero>(c, f0, nfc, S, D, partials, lane);
+        } else {
+            const uint32_t vlo = __builtin_amdgcn_readfirstlane(ch.vlo);
+            const uint32_t vn = __builtin_amdgcn_readfirstlane(ch.vn);
+            WaveQ<NL> wq;
+            wq.load(D, oc);
+            if (vn > 0 && vn <= win_cap(K))
+                cross_chunk<K, NU, NL, true>(c, f0, nfc, vlo, vn, S, D, oc, wq, ldep[wl], cutq[wl], partials, lane);
+            else
+                cross_chunk<K, NU, NL, false>(c, f0, nfc, vlo, vn, S, D, oc, wq, ldep[wl], cutq[wl], partials, lane);
         }
     }
 }
@@ -1501,10 +1435,9 @@ struct sfh_ocean_t {
 struct sfh_batch_t {
     std::vector<sfh_mesh_t*> meshes;   // one reference per body with a mesh
     std::vector<sfh_mesh_t*> meshes_by_body;
-    uint32_t n = 0;         // bodies of the face passes (the parts, for a compound batch)
+    uint32_t n = 0;         // bodies of the face pass (the parts, for a compound batch)
     uint32_t nunits = 0;    // rows of the state and output arrays: bodies, or compounds
     bool compound = false;
-    int fixed_pos = 0;      // per-body calls: 1 or 2, every chunk in that pass's list (no per-step lists)
     bool aero = false;      // every body runs the aerodynamic face loop (sfh_forces_aero)
     CompStatic* d_comp = nullptr;
     double* d_part_out = nullptr;
@@ -1513,6 +1446,7 @@ struct sfh_batch_t {
     uint64_t nfaces = 0;
     uint32_t K = 8;
     uint32_t waves = 0;
+    uint32_t split = 0;     // 1: the submerged and the crossing loops in launches of their own
     std::vector<BodyStatic> h_static;
     BodyStatic* d_static = nullptr;
     uint32_t* d_sidx = nullptr;     // per body: BodyStatic::sidx, compact (the masked prologue's first read)
@@ -1520,10 +1454,7 @@ struct sfh_batch_t {
     double* d_hout = nullptr;
     Chunk* d_chunks = nullptr;
     int8_t* d_bpos = nullptr;       // per body: this step's position (-2 inactive, -1 not hydrodynamic, 0, 1, 2)
-    BodyChunks* d_bch = nullptr;    // per body: its chunk range
-    uint2* d_boff = nullptr;        // per body: this step's offsets in the two lists
-    uint2* d_scan = nullptr;        // per block of kScanBlock bodies: sums, then exclusive offsets
-    uint32_t* d_lists = nullptr;    // list_sub [nchunks], list_cross [nchunks], counts [4]
+    uint32_t* d_lists = nullptr;    // the step's work list [nchunks], then its two counts
     BodyDyn* d_dyn = nullptr;
     double* d_partials = nullptr;
     hipGraph_t graph = nullptr;
@@ -1786,9 +1717,6 @@ static void batch_free(sfh_batch_t* b)
     b->d_hstates = b->d_hout = nullptr;
     hipFree(b->d_chunks);
     hipFree(b->d_bpos);
-    hipFree(b->d_bch);
-    hipFree(b->d_boff);
-    hipFree(b->d_scan);
     hipFree(b->d_lists);
     hipFree(b->d_dyn);
     hipFree(b->d_partials);
@@ -1830,9 +1758,8 @@ static void chunk_window(const std::vector<uint2>& blk, uint32_t first, uint32_t
     }
 }
 
-// (Re)builds the work lists: every body's faces cut into chunks of 64*K
-// consecutive faces (the face-prefix work list of the face passes), each
-// with its vertex window.
+// (Re)builds the chunks: every body's faces cut into chunks of 64*K
+// consecutive faces, each with its vertex window.
 static int batch_build_chunks(sfh_batch_t* b)
 {
     std::vector<Chunk> chunks;
@@ -1857,13 +1784,9 @@ static int batch_build_chunks(sfh_batch_t* b)
     hipFree(b->d_chunks);
     hipFree(b->d_partials);
     hipFree(b->d_bpos);
-    hipFree(b->d_bch);
-    hipFree(b->d_boff);
-    hipFree(b->d_scan);
     hipFree(b->d_lists);
     b->d_static = nullptr; b->d_sidx = nullptr; b->d_chunks = nullptr;
-    b->d_partials = nullptr; b->d_bpos = nullptr; b->d_bch = nullptr; b->d_boff = nullptr; b->d_lists = nullptr;
-    b->d_scan = nullptr;
+    b->d_partials = nullptr; b->d_bpos = nullptr; b->d_lists = nullptr;
     b->nchunks = (uint32_t)chunks.size();
     HIPCHK(hipMalloc(&b->d_static, sizeof(BodyStatic) * std::max<size_t>(1, b->n)));
     HIPCHK(hipMemcpy(b->d_static, b->h_static.data(), sizeof(BodyStatic) * b->n, hipMemcpyHostToDevice));
@@ -1878,18 +1801,8 @@ static int batch_build_chunks(sfh_batch_t* b)
         HIPCHK(hipMemcpy(b->d_chunks, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&b->d_partials, sizeof(double) * kChunkDoubles * std::max<size_t>(1, chunks.size())));
     HIPCHK(hipMalloc(&b->d_bpos, std::max<size_t>(1, b->n)));
-    {
-        std::vector<BodyChunks> bch(b->n);
-        for (uint32_t i = 0; i < b->n; ++i) {
-            const BodyStatic& st = b->h_static[i];
-            bch[i].chunk0 = st.chunk0; bch[i].nchunks = st.nchunks;
-        }
-        HIPCHK(hipMalloc(&b->d_bch, sizeof(BodyChunks) * std::max<size_t>(1, b->n)));
-        if (b->n) HIPCHK(hipMemcpy(b->d_bch, bch.data(), sizeof(BodyChunks) * b->n, hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMalloc(&b->d_boff, sizeof(uint2) * std::max<size_t>(1, b->n)));
-    HIPCHK(hipMalloc(&b->d_scan, sizeof(uint2) * std::max<size_t>(1, (b->n + kScanBlock - 1) / kScanBlock)));
-    HIPCHK(hipMalloc(&b->d_lists, sizeof(uint32_t) * (2 * (size_t)b->nchunks + 8)));
+    HIPCHK(hipMalloc(&b->d_lists, sizeof(uint32_t) * ((size_t)b->nchunks + 2)));
+    HIPCHK(hipMemset(b->d_lists + b->nchunks, 0, 2 * sizeof(uint32_t)));
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
     return SFH_OK;
@@ -1900,7 +1813,9 @@ static int apply_opts(sfh_batch_t* b, const sfh_batch_opts* opts)
     uint32_t K = opts && opts->faces_per_lane ? opts->faces_per_lane : 16u;
     if (K != 1 && K != 2 && K != 4 && K != 8 && K != 16 && K != 32)
         return fail(SFH_ERR_INVALID, "faces_per_lane must be 1, 2, 4, 8, 16 or 32");
+    if (opts && opts->split_passes > 1) return fail(SFH_ERR_INVALID, "split_passes must be 0 or 1");
     b->waves = opts && opts->waves ? opts->waves : default_waves();
+    b->split = opts ? opts->split_passes : 0u;
     if (K != b->K || !b->d_chunks) {
         b->K = K;
         return batch_build_chunks(b);
@@ -2231,91 +2146,78 @@ extern "C" int sfh_batch_info(sfh_batch b, uint32_t* nbodies, uint32_t* nchunks,
     return SFH_OK;
 }
 
-// Launches the work-list build, the depth pass and the two face passes.
-static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s, bool timed)
+// Launches the face pass over every chunk of the batch.
+static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s)
 {
-    if (b->nchunks == 0) {
-        if (timed) { HIPCHK(hipEventRecord(b->ev[1], s)); HIPCHK(hipEventRecord(b->ev[2], s)); }
-        return SFH_OK;
+    if (b->nchunks == 0) return SFH_OK;
+    const uint32_t waves = std::min<uint32_t>(std::max<uint32_t>(b->waves, 1u), b->nchunks);   // grid-strided
+    const uint32_t blocks = (waves + 3) / 4;
+#define LAUNCH_F(KK, NU, NL, PASS)                                                                              \
+    {                                                                                                           \
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces<KK, NU, NL, PASS>), dim3(blocks), dim3(256), 0, s,          \
+                           b->d_lists, b->d_lists + b->nchunks, b->nchunks, b->d_chunks, b->d_static, b->d_dyn, \
+                           oc, b->d_partials);                                                                  \
     }
-    uint32_t* list_sub = b->d_lists;
-    uint32_t* list_cross = list_sub + b->nchunks;
-    uint32_t* counts = list_cross + b->nchunks;
-    if (!b->fixed_pos) {
-        const uint32_t nblk = std::max<uint32_t>(1, (b->n + kScanBlock - 1) / kScanBlock);
-        hipLaunchKernelGGL(k_scan_part, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan);
-        hipLaunchKernelGGL(k_scan_off, dim3(nblk), dim3(kScanThreads), 0, s, b->d_bpos, b->d_bch, b->n, b->d_scan,
-                           b->d_boff, counts);
-        const uint32_t swaves = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
-        hipLaunchKernelGGL(k_scatter, dim3((swaves + 3) / 4), dim3(256), 0, s, b->d_bpos, b->d_bch, b->d_boff, b->n,
-                           list_sub, list_cross);
+#define LAUNCH_X(KK, NU, NL)                                                                                    \
+    {                                                                                                           \
+        if (b->split) {                                                                                         \
+            LAUNCH_F(KK, NU, 0, kPassSub)                                                                       \
+            LAUNCH_F(KK, NU, NL, kPassCross)                                                                    \
+        } else {                                                                                                \
+            LAUNCH_F(KK, NU, NL, kPassBoth)                                                                     \
+        }                                                                                                       \
     }
-    if (timed) {
-        HIPCHK(hipEventRecord(b->ev[1], s));
-        HIPCHK(hipEventRecord(b->ev[2], s));
-    }
-    uint32_t waves = std::min<uint32_t>(b->waves, b->nchunks);
-    uint32_t blocks = (waves + 3) / 4;
-#define LAUNCH_CROSS(KK, NU, NL)                                                                          \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_cross<KK, NU, NL>), dim3(blocks), dim3(256), 0, s, list_cross,  \
-                       counts + 1, b->d_chunks, b->d_static, b->d_dyn, oc, b->d_partials);
-#define LAUNCH_KN(KK, NU)                                                                                 \
-    if (b->fixed_pos != 2)                                                                                \
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, NU>), dim3(blocks), dim3(256), 0, s, list_sub,     \
-                           counts, b->d_chunks, b->d_static, b->d_dyn, b->d_partials);                        \
-    if (b->fixed_pos == 1) {                                                                              \
-    } else if (NU || oc.nlayers > 2) {                                                                    \
-        LAUNCH_CROSS(KK, NU, -1)                                                                          \
-    } else if (oc.nlayers == 0) {                                                                         \
-        LAUNCH_CROSS(KK, false, 0)                                                                        \
-    } else if (oc.nlayers == 1) {                                                                         \
-        LAUNCH_CROSS(KK, false, 1)                                                                        \
-    } else {                                                                                              \
-        LAUNCH_CROSS(KK, false, 2)                                                                        \
-    }
-#define LAUNCH_K(KK)                                                                                 \
-    case KK:                                                                                         \
-        if (b->aero) {                                                                               \
-            if (oc.ncur)                                                                             \
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, true, true>), dim3(blocks), dim3(256), 0, s, \
-                                   list_sub, counts, b->d_chunks, b->d_static, b->d_dyn, b->d_partials);  \
-            else                                                                                     \
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_faces_sub<KK, false, true>), dim3(blocks), dim3(256), 0, s, \
-                                   list_sub, counts, b->d_chunks, b->d_static, b->d_dyn, b->d_partials);  \
-        } else if (oc.ncur) {                                                                        \
-            LAUNCH_KN(KK, true)                                                                      \
-        } else {                                                                                     \
-            LAUNCH_KN(KK, false)                                                                     \
-        }                                                                                            \
+#define LAUNCH_K(KK)                                                                                            \
+    case KK:                                                                                                    \
+        if (b->aero) {                                                                                          \
+            if (oc.ncur) LAUNCH_F(KK, true, 0, kPassAero) else LAUNCH_F(KK, false, 0, kPassAero)                \
+        } else if (oc.ncur) LAUNCH_X(KK, true, -1)                                                              \
+        else if (oc.nlayers > 2) LAUNCH_X(KK, false, -1)                                                        \
+        else if (oc.nlayers == 0) LAUNCH_X(KK, false, 0)                                                        \
+        else if (oc.nlayers == 1) LAUNCH_X(KK, false, 1)                                                        \
+        else LAUNCH_X(KK, false, 2)                                                                             \
         break;
     switch (b->K) {
         LAUNCH_K(1) LAUNCH_K(2) LAUNCH_K(4) LAUNCH_K(8) LAUNCH_K(16) LAUNCH_K(32)
         default: return fail(SFH_ERR_INVALID, "bad K");
     }
 #undef LAUNCH_K
-#undef LAUNCH_KN
-#undef LAUNCH_CROSS
+#undef LAUNCH_X
+#undef LAUNCH_F
     return SFH_OK;
 }
 
+// One evaluation: the prologue, the face pass, the epilogue and, for a
+// compound batch, the composition of the parts' rows.  timed: events before
+// the prologue (ev[0]), around the face launches (ev[1], ev[2]) and after the
+// last launch (ev[3]).
 static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_states, int kind, double* d_out,
                         hipStream_t s, bool timed, const uint8_t* active = nullptr)
 {
     if (b->n == 0) return SFH_OK;
+    // every launch of the step runs or none does: the epilogue zeroes the
+    // counts the prologue's list build adds to
+    if (b->K != 1 && b->K != 2 && b->K != 4 && b->K != 8 && b->K != 16 && b->K != 32)
+        return fail(SFH_ERR_INVALID, "bad K");
     OceanK oc = make_oceank(o);
+    double* rows = b->compound ? b->d_part_out : d_out;
     const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + 127) / 128);
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
     hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, active,
-                       b->d_sidx, b->d_dyn, b->d_bpos);
-    int rc = launch_faces(b, oc, s, timed);
+                       b->d_sidx, b->d_dyn, b->d_bpos, rows, b->d_lists, b->nchunks, b->d_lists + b->nchunks);
+    if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
+    int rc = launch_faces(b, oc, s);
     if (rc != SFH_OK) return rc;
-    if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
-    const uint32_t eblocks = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
-    hipLaunchKernelGGL(k_epilogue, dim3(eblocks), dim3(64), 0, s, b->d_static, b->d_dyn, b->d_bpos, b->n, oc,
-                       b->d_partials, b->compound ? b->d_part_out : d_out);
+    if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
+    if (b->nchunks) {
+        const uint32_t eblocks = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
+        hipLaunchKernelGGL(k_epilogue, dim3(eblocks), dim3(64), 0, s, b->d_static, b->d_dyn, b->d_bpos, b->n, oc,
+                           b->d_partials, rows, b->d_lists + b->nchunks);
+    }
     if (b->compound && b->nunits)
         hipLaunchKernelGGL(k_compound, dim3((b->nunits + 127) / 128), dim3(128), 0, s, b->d_comp, b->d_static,
                            b->d_dyn, b->nunits, oc, active, b->d_part_out, d_out);
+    if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
     HIPCHK(hipGetLastError());
     return SFH_OK;
 }
@@ -2331,7 +2233,7 @@ static void harvest_timing(sfh_batch_t* b)
     if (!b->ev_pending) return;
     float t_face = 0.f, t_tot = 0.f;
     if (hipEventSynchronize(b->ev[3]) == hipSuccess &&
-        hipEventElapsedTime(&t_face, b->ev[2], b->ev[3]) == hipSuccess &&
+        hipEventElapsedTime(&t_face, b->ev[1], b->ev[2]) == hipSuccess &&
         hipEventElapsedTime(&t_tot, b->ev[0], b->ev[3]) == hipSuccess) {
         b->face_ms_sum += t_face;
         b->total_ms_sum += t_tot;
@@ -2370,10 +2272,10 @@ extern "C" int sfh_batch_compute_masked(sfh_batch b, sfh_ocean o, const double* 
 // ---- autotuner ------------------------------------------------------------------
 
 // Times the batch's whole step on its own inputs for each (faces per lane,
-// resident wavefronts per CU) candidate and keeps the fastest.  The step is
-// timed with HIP events on the caller's stream around `reps` eager launches
-// after one untimed launch; the face-pass time of the last launch comes from
-// the batch's own event pair.  A batch's work lists depend on the faces per
+// resident wavefronts per CU, one or two face launches) candidate and keeps
+// the fastest.  The step is timed with HIP events on the caller's stream
+// around `reps` eager launches after one untimed launch; the time of the face
+// launches of the last launch comes from the batch's own event pair.  A batch's chunks depend on the faces per
 // lane, so a candidate with another value rebuilds them (and any graph the
 // caller captured from the batch before must be captured again).
 extern "C" int sfh_batch_autotune(sfh_batch b, sfh_ocean o, const double* d_states, const uint8_t* d_active,
@@ -2392,7 +2294,8 @@ extern "C" int sfh_batch_autotune(sfh_batch b, sfh_ocean o, const double* d_stat
         if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
     }
     static const uint32_t kK[] = {4, 8, 16, 32};
-    static const uint32_t kWpc[] = {8, 16, 24, 32, 48, 64, 96};
+    static const uint32_t kWpc[] = {8, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+    const uint32_t nsplit = b->aero ? 1u : 2u;   // an aerodynamic batch has one loop
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -2400,11 +2303,12 @@ extern "C" int sfh_batch_autotune(sfh_batch b, sfh_ocean o, const double* d_stat
     if (!b->ev[0])
         for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&b->ev[i]));
     double best = 1e30;
-    sfh_batch_opts best_opts = {b->K, b->waves};
+    sfh_batch_opts best_opts = {b->K, b->waves, b->split};
     uint32_t np = 0;
     for (uint32_t K : kK) {
         for (uint32_t wpc : kWpc) {
-            sfh_batch_opts opts = {K, wpc * (uint32_t)cus};
+          for (uint32_t split = 0; split < nsplit && rc == SFH_OK; ++split) {
+            sfh_batch_opts opts = {K, wpc * (uint32_t)cus, split};
             rc = apply_opts(b, &opts);
             if (rc != SFH_OK) break;
             rc = batch_launch(b, o, d_states, 0, d_out, s, false, d_active);   // warm-up
@@ -2417,12 +2321,14 @@ extern "C" int sfh_batch_autotune(sfh_batch b, sfh_ocean o, const double* d_stat
             hipError_t e = hipEventSynchronize(e1);
             float ms = 0.f, face = 0.f;
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            if (e == hipSuccess) e = hipEventElapsedTime(&face, b->ev[2], b->ev[3]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&face, b->ev[1], b->ev[2]);
             if (e != hipSuccess) { rc = fail(SFH_ERR_HIP, "autotune timing: %s", hipGetErrorString(e)); break; }
             const double step = (double)ms / reps;
             if (points && np < max_points) {
                 points[np].faces_per_lane = K;
                 points[np].waves_per_cu = wpc;
+                points[np].split_passes = split;
+                points[np].pad_ = 0;
                 points[np].step_ms = step;
                 points[np].face_ms = face;
             }
@@ -2431,6 +2337,8 @@ extern "C" int sfh_batch_autotune(sfh_batch b, sfh_ocean o, const double* d_stat
                 best = step;
                 best_opts = opts;
             }
+          }
+          if (rc != SFH_OK) break;
         }
         if (rc != SFH_OK) break;
     }
@@ -2448,6 +2356,7 @@ extern "C" int sfh_batch_get_opts(sfh_batch b, sfh_batch_opts* opts)
     if (!b || !opts) return fail(SFH_ERR_INVALID, "null argument");
     opts->faces_per_lane = b->K;
     opts->waves = b->waves;
+    opts->split_passes = b->split;
     return SFH_OK;
 }
 
@@ -2597,24 +2506,20 @@ extern "C" int sfh_batch_kernel_ms(sfh_batch b, double* face_ms, double* total_m
 // ---- per-body evaluations ------------------------------------------------------
 
 // One per-body evaluation (a single-body batch) needs, beyond the mesh: the
-// body's static record and per-step constants, its chunk list with the
-// chunks' vertex windows, the work lists and the chunk partials.  They are
+// body's static record and per-step constants, its chunks with their vertex
+// windows, the work list and the chunk partials.  They are
 // carved out of one device buffer sized for meshes of up to F faces: a
 // workspace of the per-body statics (device memory from hipMalloc, grown on
-// demand) or the no-alloc entry point's file-scope array.  A per-body call
-// knows the body's position (the Surface call is the crossing loop, the
-// Submerged and aerodynamic calls the submerged one), so one kernel writes
-// the static record, the chunks and the work list from its arguments and
-// the mesh's block windows, and the step is that kernel, the prologue, one
-// face pass and the epilogue.
+// demand) or the no-alloc entry point's file-scope array.  One kernel writes
+// the static record and the chunks from its arguments and the mesh's block
+// windows, and the step is that kernel, the prologue (which gives the body
+// the position its call names: the Surface call the crossing loop, the
+// Submerged and aerodynamic calls the submerged one) and the face pass.
 static constexpr uint32_t kSingleK = 8;   // faces per lane of per-body calls
 
 struct SingleHead {
     BodyStatic st;
     BodyDyn dyn;
-    BodyChunks bch;
-    uint2 boff;
-    uint2 scan;
     uint32_t sidx;
     int8_t bpos;
 };
@@ -2622,8 +2527,8 @@ struct SingleHead {
 __host__ __device__ constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 __host__ __device__ constexpr uint32_t single_nchunks(uint32_t F) { return (F + 64u * kSingleK - 1) / (64u * kSingleK); }
 
-// Byte offsets of the parts in a buffer for F faces: head, lists, chunks,
-// partials, end.
+// Byte offsets of the parts in a buffer for F faces: head, work list (with
+// its counts), chunks, partials, end.
 struct SingleLayout {
     size_t lists, chunks, partials, end;
 };
@@ -2632,30 +2537,27 @@ __host__ __device__ constexpr SingleLayout single_layout(uint32_t F)
 {
     const size_t nc = single_nchunks(F);
     const size_t lists = align256(sizeof(SingleHead));
-    const size_t chunks = align256(lists + sizeof(uint32_t) * (2 * nc + 8));
+    const size_t chunks = align256(lists + sizeof(uint32_t) * (nc + 2));
     const size_t partials = align256(chunks + sizeof(Chunk) * (nc ? nc : 1));
     const size_t end = align256(partials + sizeof(double) * kChunkDoubles * (nc ? nc : 1));
     return SingleLayout{lists, chunks, partials, end};
 }
 
-// Writes the body's static record, its chunks with their vertex windows
-// (from the mesh's per-64-face block windows) and the work list of the pass
-// its position selects (`pos`: 1 submerged, 2 crossing) with the counts.
+// Writes the body's static record and its chunks with their vertex windows
+// (from the mesh's per-64-face block windows), and zeroes the work list's
+// counts (the buffer may hold another call's).
 __global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H, Chunk* __restrict__ chunks,
-                                                      uint32_t* __restrict__ lists, const uint2* __restrict__ blkwin,
-                                                      BodyStatic st, int pos)
+                                                      uint32_t* __restrict__ counts, const uint2* __restrict__ blkwin,
+                                                      BodyStatic st)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t csize = 64u * kSingleK;
     const uint32_t nc = st.nchunks;
     if (t == 0) {
         H->st = st;
-        H->bch.chunk0 = 0;
-        H->bch.nchunks = nc;
         H->sidx = 0;
-        uint32_t* counts = lists + 2 * nc;
-        counts[0] = pos == 1 ? nc : 0u;
-        counts[1] = pos == 2 ? nc : 0u;
+        counts[0] = 0u;
+        counts[1] = 0u;
     }
     if (t < nc) {
         Chunk c;
@@ -2674,7 +2576,6 @@ __global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H
         c.vn = fits ? hi - lo + 1 : 0u;
         c.pad[0] = c.pad[1] = c.pad[2] = 0;
         chunks[t] = c;
-        lists[(pos == 2 ? nc : 0u) + t] = t;
     }
 }
 
@@ -2692,16 +2593,14 @@ static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags
     st.chunk0 = 0;
     st.nchunks = single_nchunks(m->nf);
     st.sidx = 0;
-    const int pos = (flags & BF_MODE_SURF) ? 2 : 1;   // k_prologue's position for these modes
     const uint32_t nt = std::max(st.nchunks, 1u);
-    hipLaunchKernelGGL(k_single_setup, dim3((nt + 255) / 256), dim3(256), 0, s, H, chunks, lists, m->d_blkwin, st,
-                       pos);
+    hipLaunchKernelGGL(k_single_setup, dim3((nt + 255) / 256), dim3(256), 0, s, H, chunks, lists + st.nchunks,
+                       m->d_blkwin, st);
     // a batch view of the buffer: no allocation (empty vectors, borrowed pointers)
     sfh_batch_t b;
     b.n = 1;
     b.nunits = 1;
     b.aero = (flags & BF_MODE_AERO) != 0;
-    b.fixed_pos = pos;
     b.nchunks = st.nchunks;
     b.nfaces = m->nf;
     b.K = kSingleK;
@@ -2710,9 +2609,6 @@ static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags
     b.d_sidx = &H->sidx;
     b.d_chunks = chunks;
     b.d_bpos = &H->bpos;
-    b.d_bch = &H->bch;
-    b.d_boff = &H->boff;
-    b.d_scan = &H->scan;
     b.d_lists = lists;
     b.d_dyn = &H->dyn;
     b.d_partials = (double*)(buf + L.partials);
@@ -2946,7 +2842,7 @@ static __device__ __align__(256) unsigned char scratch[kNaBytes];
 
 static std::mutex g_na_mu;
 static void* g_na_ptr[64];       // per device: the address of `scratch` (hipGetSymbolAddress)
-static uint32_t g_na_waves[64];  // per device: resident wavefronts of the face passes
+static uint32_t g_na_waves[64];  // per device: resident wavefronts of the face pass
 
 // Looks up this device's copy of `scratch` once (with the device current):
 // from sfh_init, or from the first no-alloc call on a device sfh_init did not see.

@@ -309,12 +309,12 @@ class BodyBatch:
     """Many bodies evaluated in one pass (SolidEntity::ComputeHydrodynamicForces
     for each, with Ocean::ApplyFluidForces' settings {damping, buoyancy} = {true, true})."""
 
-    def __init__(self, bodies: list[BodyDesc], faces_per_lane: int = 0, waves: int = 0):
+    def __init__(self, bodies: list[BodyDesc], faces_per_lane: int = 0, waves: int = 0, split_passes: int = 0):
         lib = N.load()
         arr = (N.sfh_body_desc * max(1, len(bodies)))()
         for i, b in enumerate(bodies):
             _fill_body(arr[i], b)
-        opts = N.sfh_batch_opts(faces_per_lane, waves)
+        opts = N.sfh_batch_opts(faces_per_lane, waves, split_passes)
         h = C.c_void_p()
         N.check(lib.sfh_batch_create(arr, len(bodies), C.byref(opts), C.byref(h)))
         self.handle = h
@@ -332,25 +332,26 @@ class BodyBatch:
         N.check(N.load().sfh_batch_info(self.handle, C.byref(nb), C.byref(nc), C.byref(nf)))
         return nb.value, nc.value, nf.value
 
-    def set_opts(self, faces_per_lane: int = 0, waves: int = 0) -> None:
-        opts = N.sfh_batch_opts(faces_per_lane, waves)
+    def set_opts(self, faces_per_lane: int = 0, waves: int = 0, split_passes: int = 0) -> None:
+        opts = N.sfh_batch_opts(faces_per_lane, waves, split_passes)
         N.check(N.load().sfh_batch_set_opts(self.handle, C.byref(opts)))
 
-    def get_opts(self) -> tuple[int, int]:
-        """(faces_per_lane, waves) the batch runs with."""
+    def get_opts(self) -> tuple[int, int, int]:
+        """(faces_per_lane, waves, split_passes) the batch runs with."""
         opts = N.sfh_batch_opts()
         N.check(N.load().sfh_batch_get_opts(self.handle, C.byref(opts)))
-        return int(opts.faces_per_lane), int(opts.waves)
+        return int(opts.faces_per_lane), int(opts.waves), int(opts.split_passes)
 
     def autotune(self, ocn: Ocean, d_states: int, d_out: int, stream: int = 0, d_active: int | None = None,
                  reps: int = 5) -> list[dict]:
         """sfh_batch_autotune: times the step over faces_per_lane x waves per
-        CU on these inputs and keeps the fastest; returns the measured points."""
-        pts = (N.sfh_tune_point * 64)()
+        CU x split_passes on these inputs and keeps the fastest; returns the
+        measured points."""
+        pts = (N.sfh_tune_point * 128)()
         n = C.c_uint32()
         N.check(N.load().sfh_batch_autotune(self.handle, ocn.handle, C.c_void_p(d_states), C.c_void_p(d_active or 0),
-                                            C.c_void_p(d_out), C.c_void_p(stream), reps, pts, 64, C.byref(n)))
-        return [{"faces_per_lane": p.faces_per_lane, "waves_per_cu": p.waves_per_cu,
+                                            C.c_void_p(d_out), C.c_void_p(stream), reps, pts, 128, C.byref(n)))
+        return [{"faces_per_lane": p.faces_per_lane, "waves_per_cu": p.waves_per_cu, "split_passes": p.split_passes,
                  "step_ms": round(p.step_ms, 5), "face_ms": round(p.face_ms, 5)} for p in pts[:n.value]]
 
     def compute_host(self, ocn: Ocean, states: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
