@@ -562,32 +562,41 @@ __device__ __forceinline__ int prologue_body(const BodyStatic* __restrict__ st, 
 // the chunks of bodies in position 1 (submerged loop) filled from the front
 // and those of bodies in position 2 (crossing loop) from the back, with the
 // two counts in counts[0..1] (zero before the prologue; the epilogue zeroes
-// them for the next step).  Each wavefront of the prologue sums its bodies'
-// chunk counts and reserves its ranges with one atomic add per list; a
-// body's lanes then write its chunk ids.  The order of the entries depends on
-// the order of the atomics, the results do not: a chunk's partial has its
-// own slot and the epilogue sums a body's partials in chunk order.
+// them for the next step).  Each workgroup of the prologue sums its bodies'
+// chunk counts (a scan over each wavefront's lanes, then over the
+// wavefronts) and reserves its ranges with one atomic add per list; a body's
+// lanes then write its chunk ids.  The order of the entries depends on the
+// order of the atomics, the results do not: a chunk's partial has its own
+// slot and the epilogue sums a body's partials in chunk order.
+static constexpr int kPrologueBlock = 256;
+
 __device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_t* __restrict__ counts,
                                             uint32_t nc, uint32_t c_sub, uint32_t c_cross, uint32_t chunk0,
                                             int sub)
 {
+    __shared__ uint32_t wsum[2][kPrologueBlock / 64];
+    __shared__ uint32_t wbase[2][kPrologueBlock / 64];
     const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
     const uint32_t v0 = sub == 0 ? c_sub : 0u, v1 = sub == 0 ? c_cross : 0u;
     uint32_t i0 = v0, i1 = v1;   // inclusive scan over the wavefront's lanes
     for (int off = 1; off < 64; off <<= 1) {
         const uint32_t y0 = __shfl_up(i0, off, 64), y1 = __shfl_up(i1, off, 64);
         if (lane >= off) { i0 += y0; i1 += y1; }
     }
-    uint32_t base0 = 0, base1 = 0;
-    if (lane == 63) {
-        if (i0) base0 = atomicAdd(counts, i0);
-        if (i1) base1 = atomicAdd(counts + 1, i1);
+    if (lane == 63) { wsum[0][w] = i0; wsum[1][w] = i1; }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int k = threadIdx.x;
+        uint32_t run = 0;
+        for (int j = 0; j < kPrologueBlock / 64; ++j) { wbase[k][j] = run; run += wsum[k][j]; }
+        const uint32_t base = run ? atomicAdd(counts + k, run) : 0u;
+        for (int j = 0; j < kPrologueBlock / 64; ++j) wbase[k][j] += base;
     }
-    base0 = __shfl(base0, 63, 64);
-    base1 = __shfl(base1, 63, 64);
+    __syncthreads();
     const int lead = lane & ~(kPrologueLanes - 1);
-    const uint32_t o0 = base0 + __shfl(i0 - v0, lead, 64);
-    const uint32_t o1 = base1 + __shfl(i1 - v1, lead, 64);
+    const uint32_t o0 = wbase[0][w] + __shfl(i0 - v0, lead, 64);
+    const uint32_t o1 = wbase[1][w] + __shfl(i1 - v1, lead, 64);
     // the two parts hold at most the batch's nc chunks together; the bounds
     // tests only guard the array against counts that were not zeroed
     for (uint32_t i = (uint32_t)sub; i < c_sub; i += kPrologueLanes)
@@ -597,8 +606,8 @@ __device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_
 }
 
 // Every lane of the grid reaches the list build (lanes past the last body
-// contribute nothing), so the wavefront-wide scan sees whole wavefronts.
-__global__ __launch_bounds__(128) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
+// contribute nothing), so the scans see whole wavefronts and workgroups.
+__global__ __launch_bounds__(kPrologueBlock) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
                            uint32_t n, OceanK oc, const uint8_t* __restrict__ active,
                            const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
                            int8_t* __restrict__ bpos, double* __restrict__ out, uint32_t* __restrict__ list,
@@ -676,13 +685,6 @@ __host__ __device__ constexpr uint32_t win_cap(uint32_t K) { return K <= 16 ? 64
 // ---------------------------------------------------------------------------
 // k_faces: one wavefront per chunk of up to 64*K faces of one body
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ double wave_sum_f64(double x)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
 
 // Transposed FP64 reduction of 24 per-lane values across a wavefront.  Each
 // step exchanges half of the remaining values with the partner lane, so the
@@ -1136,50 +1138,75 @@ __device__ void epilogue_finish(const BodyStatic& S, const BodyDyn& D, const Oce
 }
 
 // Row of body b (position 1 or 2) on one wavefront: lanes stride over the
-// body's chunk partials and a wavefront reduction sums them in a fixed order,
-// so the row does not depend on which wavefront runs this; lane 0 finishes.
+// body's chunk partials, a transposed reduction across the wavefront (a fixed
+// order, so the row does not depend on which wavefront runs this) leaves the
+// 23 totals on 23 lanes, which pass them to lane 0 through `ps` (the
+// wavefront's LDS slice); lane 0 finishes.
 __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t b,
-                              const OceanK& oc, const double* partials, double* __restrict__ out, int lane)
+                              const OceanK& oc, const double* partials, double* __restrict__ out, double* ps,
+                              int lane)
 {
     const BodyStatic& S = st[b];
-    double P[23];
+    double v[24];
 #pragma unroll
-    for (int k = 0; k < 23; ++k) P[k] = 0.0;
+    for (int k = 0; k < 24; ++k) v[k] = 0.0;
     for (uint32_t c = lane; c < S.nchunks; c += 64) {
-        const double* p = partials + (size_t)(S.chunk0 + c) * kChunkDoubles;
+        const double2* p = (const double2*)(partials + (size_t)(S.chunk0 + c) * kChunkDoubles);
 #pragma unroll
-        for (int k = 0; k < 23; ++k) P[k] += p[k];
+        for (int k = 0; k < 12; ++k) {
+            const double2 x = p[k];
+            v[2 * k] += x.x;
+            v[2 * k + 1] += x.y;
+        }
     }
+    v[23] = 0.0;
+    xreduce24(v, lane);
+    const int q = xreduce24_slot(lane);
+    if (q >= 0) ps[q] = v[0];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) {
+        double P[23];
 #pragma unroll
-    for (int k = 0; k < 23; ++k) P[k] = wave_sum_f64(P[k]);
-    if (lane == 0) epilogue_finish(S, dyn[b], oc, P, out + (size_t)b * SFH_OUT_DOUBLES);
+        for (int k = 0; k < 23; ++k) P[k] = ps[k];
+        epilogue_finish(S, dyn[b], oc, P, out + (size_t)b * SFH_OUT_DOUBLES);
+    }
+    // ps is rewritten by the wavefront's next body
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront per body, bodies strided over the grid: lane k reads the
-// position of the k-th of the wavefront's next 64 bodies in one load and the
-// wavefront finishes the rows of those with face work (positions 1 and 2;
-// the prologue wrote the others', and those of bodies without faces), in
-// order.  A masked step with few active bodies thus costs two loads per 64
-// bodies.
+// kEpilogueWaves wavefronts per workgroup; wavefront w of W takes the bodies
+// w, w + W, w + 2W, ...: lane k reads the position of the k-th of the
+// wavefront's next 64 bodies in one load and the wavefront finishes the rows
+// of those with face work (positions 1 and 2; the prologue wrote the others',
+// and those of bodies without faces), in order.  A masked step with few
+// active bodies thus costs two loads per 64 bodies.
 // It also zeroes the work list's counts for the next step's prologue.
-__global__ __launch_bounds__(64) void k_epilogue(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn,
-                                                 const int8_t* __restrict__ bpos, uint32_t n, OceanK oc,
-                                                 const double* __restrict__ partials, double* __restrict__ out,
-                                                 uint32_t* __restrict__ counts)
+static constexpr int kEpilogueWaves = 4;
+
+__global__ __launch_bounds__(64 * kEpilogueWaves) void k_epilogue(
+    const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, const int8_t* __restrict__ bpos, uint32_t n,
+    OceanK oc, const double* __restrict__ partials, double* __restrict__ out, uint32_t* __restrict__ counts)
 {
-    const uint32_t G = gridDim.x;
-    const int lane = threadIdx.x;
-    if (blockIdx.x == 0 && lane < 2) counts[lane] = 0u;
-    for (uint32_t base = blockIdx.x; base < n; base += 64u * G) {
-        const uint32_t bl = base + (uint32_t)lane * G;
+    __shared__ double ps[kEpilogueWaves][24];
+    const int lane = threadIdx.x & 63;
+    const int wl = threadIdx.x >> 6;
+    const uint32_t W = gridDim.x * kEpilogueWaves;
+    const uint32_t w = blockIdx.x * kEpilogueWaves + wl;
+    if (blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = 0u;
+    for (uint32_t base = w; base < n; base += 64u * W) {
+        const uint32_t bl = base + (uint32_t)lane * W;
         int p = -3;
         if (bl < n) p = bpos[bl];
         uint64_t m = __ballot(p == 1 || p == 2);
         while (m) {
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             m &= m - 1ull;
-            const uint32_t b = base + k * G;
-            if (st[b].nchunks) epilogue_body(st, dyn, b, oc, partials, out, lane);
+            const uint32_t b = base + k * W;
+            if (st[b].nchunks) epilogue_body(st, dyn, b, oc, partials, out, ps[wl], lane);
         }
     }
 }
@@ -2201,17 +2228,18 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
         return fail(SFH_ERR_INVALID, "bad K");
     OceanK oc = make_oceank(o);
     double* rows = b->compound ? b->d_part_out : d_out;
-    const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + 127) / 128);
+    const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + kPrologueBlock - 1) / kPrologueBlock);
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
-    hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(128), 0, s, b->d_static, d_states, kind, b->n, oc, active,
+    hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(kPrologueBlock), 0, s, b->d_static, d_states, kind, b->n, oc, active,
                        b->d_sidx, b->d_dyn, b->d_bpos, rows, b->d_lists, b->nchunks, b->d_lists + b->nchunks);
     if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
     int rc = launch_faces(b, oc, s);
     if (rc != SFH_OK) return rc;
     if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
     if (b->nchunks) {
-        const uint32_t eblocks = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
-        hipLaunchKernelGGL(k_epilogue, dim3(eblocks), dim3(64), 0, s, b->d_static, b->d_dyn, b->d_bpos, b->n, oc,
+        const uint32_t ewaves = std::min<uint32_t>(b->n, std::max<uint32_t>(1, b->waves / 2));
+        const uint32_t eblocks = (ewaves + kEpilogueWaves - 1) / kEpilogueWaves;
+        hipLaunchKernelGGL(k_epilogue, dim3(eblocks), dim3(64 * kEpilogueWaves), 0, s, b->d_static, b->d_dyn, b->d_bpos, b->n, oc,
                            b->d_partials, rows, b->d_lists + b->nchunks);
     }
     if (b->compound && b->nunits)
