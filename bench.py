@@ -554,7 +554,8 @@ def main() -> int:
     ap.add_argument("--waves", type=int, default=0, help="resident wavefronts (0 = library default)")
     ap.add_argument("--split-passes", type=int, default=0,
                     help="1: submerged and crossing chunks in face launches of their own")
-    ap.add_argument("--graph", type=int, default=1, help="replay a captured HIP graph per step")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="also time replays of captured HIP graphs of the step (reported beside the eager rate)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single-body workloads")
@@ -634,9 +635,26 @@ def main() -> int:
         log(f"autotune: {len(sweep)} points in {time.time() - t0:.1f}s, chose faces_per_lane "
             f"{args.faces_per_lane}, {args.waves} waves, split_passes {args.split_passes}")
 
-    graphs = None
+    def timed(step) -> float:
+        for k in range(args.warmup):
+            step(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(k)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # the timed steps: eager launches of the step (prologue, face launch,
+    # epilogue) on one stream, the state set changing every step
+    elapsed = timed(lambda k: batch.compute(go, states[k % nsets].data_ptr(), out.data_ptr(), stream))
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = args.bodies * args.steps / elapsed
+
+    graph_rate = None
     if args.graph:
-        # one captured graph per state set (kernel arguments are baked into a graph)
+        # the same steps as replays of captured HIP graphs, one per state set
+        # (kernel arguments are baked into a graph), reported beside the rate
         gstream = torch.cuda.Stream()
         graphs = []
         for k in range(nsets):
@@ -646,23 +664,9 @@ def main() -> int:
                 b.capture(go, states[k].data_ptr(), out.data_ptr(), gstream.cuda_stream)
             graphs.append(b)
         torch.cuda.synchronize()
-
-    def step(k: int) -> None:
-        if graphs is not None:
-            graphs[k % nsets].replay(stream)
-        else:
-            batch.compute(go, states[k % nsets].data_ptr(), out.data_ptr(), stream)
-
-    for k in range(args.warmup):
-        step(k)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(k)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = args.bodies * args.steps / elapsed
+        g_el = timed(lambda k: graphs[k % nsets].replay(stream))
+        graph_rate = {"value": round(args.bodies * args.steps / g_el, 1), "ms_per_step": round(1e3 * g_el / args.steps, 4)}
+        del graphs
 
     # per-kernel timing with HIP events on the launch stream (eager launches)
     batch.timing(True)
@@ -705,7 +709,7 @@ def main() -> int:
         "config": {"workload": f"fleet of {args.bodies} bodies (ellipsoids/boxes, half-axes U[0.1,2] m, "
                                f"faces log-U[500,50000]), depth U[-1,3] m, 1024x1024 PM wave grid",
                    "bodies": args.bodies, "faces": int(nfaces), "vertices": int(nv.sum()), "chunks": nchunks,
-                   "graph": bool(args.graph), "faces_per_lane": batch.get_opts()[0],
+                   "launch": "eager", "faces_per_lane": batch.get_opts()[0],
                    "waves": batch.get_opts()[1], "split_passes": batch.get_opts()[2],
                    "autotune": None if sweep is None else {
                        "chosen": {"faces_per_lane": batch.get_opts()[0], "waves": batch.get_opts()[1],
@@ -716,6 +720,7 @@ def main() -> int:
                      "kernel": "k_faces", "kernel_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),
                      "algorithmic_bytes": face_bytes, "timed_launches": count},
         "host_buffers_body_steps_per_s": round(host_io, 1),
+        "graph_replay": graph_rate,
     }
     res["roofline"]["peaks_measured"] = measure_peaks(dev)
     if not args.no_parity:
