@@ -377,12 +377,19 @@ def surrogate_section(args) -> dict:
     rng = np.random.default_rng(99)
     states = torch.from_numpy(S.sample_states(rng, nb, (0.0, w.sea.wave_len), (-1.0, 3.0))).cuda()
     out = torch.zeros((nb, 24), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    # the exact step in the configuration the autotuner picks for it, eager
+    # launches and graph replay (the faster)
+    tune = {}
+    if not args.no_autotune:
+        batch.autotune(ocean, states.data_ptr(), out.data_ptr(), stream)
+        tune["exact"] = batch.get_opts()
     gs = torch.cuda.Stream()
     with torch.cuda.stream(gs):
         batch.capture(ocean, states.data_ptr(), out.data_ptr(), gs.cuda_stream)
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream().cuda_stream
-    t_exact = _time_replays(lambda: batch.replay(stream), 30)
+    t_exact = min(_time_replays(lambda: batch.replay(stream), 30),
+                  _time_replays(lambda: batch.compute(ocean, states.data_ptr(), out.data_ptr(), stream), 30))
     step = S.SurrogateStep(batch, ocean, nets, norms, [per] * 8, states)
     # the surrogate alone is one kernel per step: a plain launch costs less
     # than a one-node graph replay, so both are timed
@@ -391,7 +398,15 @@ def surrogate_section(args) -> dict:
     t_graph = _time_replays(step.replay, 200)
     t_sur_only = min(t_eager, t_graph)
     # the step that delivers outputs: surrogate, exact kernel on the bodies it
-    # flags out of distribution, merge (graph replay and eager, the faster)
+    # flags out of distribution, merge (graph replay and eager, the faster);
+    # the masked exact step is tuned on this step's flags
+    if not args.no_autotune:
+        step.run()
+        torch.cuda.synchronize()
+        mask = step.ood.clone()
+        batch.autotune(ocean, states.data_ptr(), out.data_ptr(), stream, d_active=mask.data_ptr())
+        tune["fallback"] = batch.get_opts()
+        torch.cuda.synchronize()
     t_fb_eager = _time_replays(step.run_with_fallback, 100)
     step.capture(with_fallback=True)
     t_fb_graph = _time_replays(step.replay, 100)
@@ -432,6 +447,7 @@ def surrogate_section(args) -> dict:
                                   "surrogate_only_launch": round(t_eager * 1e6, 2),
                                   "surrogate_only_graph": round(t_graph * 1e6, 2)},
         "fallback_bodies": n_fallback,
+        "autotune": tune,
         "speedup": round(t_exact / t_sur, 2),
         "speedup_surrogate_only": round(t_exact / t_sur_only, 2),
         "throughput_sample_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in sample_err.items()},
@@ -483,8 +499,9 @@ def rollout_section(args, sea) -> dict:
     # exact kernel alone on the same bodies (graph replay of one evaluation),
     # in the configuration the autotuner picks for the whole batch; the
     # rollout's batch evaluates only the few out-of-distribution bodies per
-    # step and is tuned on that masked step (the plan's OOD flags of these
-    # states) instead
+    # step and is tuned on that masked step: the rollout's states after a few
+    # steps with 1% of the compounds active (a seeded draw; the audits find
+    # 0.5-1% out of distribution, and the first steps' states are all inside)
     exact = H.CompoundBatch([H.CompoundDesc(body, parts)] * n)
     out = torch.zeros((n, 24), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
@@ -492,25 +509,22 @@ def rollout_section(args, sea) -> dict:
     if not args.no_autotune:
         pts = exact.autotune(ocean, states.data_ptr(), out.data_ptr(), stream)
         tune["exact"] = {"chosen": exact.get_opts(), "points": pts}
-        plan = S.Plan(batch, [net], [norm], [n])
-        ft0 = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
-        ood0 = torch.zeros(n, dtype=torch.uint8, device="cuda")
-        plan.run(ocean, states, ft0, ood=ood0)
-        torch.cuda.synchronize()
-        pts = batch.autotune(ocean, states.data_ptr(), out.data_ptr(), stream, d_active=ood0.data_ptr())
-        tune["rollout_masked"] = {"chosen": batch.get_opts(), "points": pts,
-                                  "active_bodies": int(ood0.sum())}
-        plan.close()
-        torch.cuda.synchronize()
     gs = torch.cuda.Stream()
     with torch.cuda.stream(gs):
         exact.capture(ocean, states.data_ptr(), out.data_ptr(), gs.cuda_stream)
     torch.cuda.synchronize()
     t_exact = _time_replays(lambda: exact.replay(stream), 10)
-    ro = S.MixedRollout(batch, ocean, net, norm, states, auv.mass, auv.inertia, dt=1e-3, audit_every=100)
+    ro = S.MixedRollout(batch, ocean, net, norm, states, auv.mass, auv.inertia, dt=1e-3, audit_every=100,
+                        audit_batch=exact)
     for _ in range(3):
         ro.step()
     torch.cuda.synchronize()
+    if not args.no_autotune:
+        mask = torch.from_numpy((np.random.default_rng(5).random(n) < 0.01).astype(np.uint8)).cuda()
+        st_now = ro.states.clone()
+        pts = batch.autotune(ocean, st_now.data_ptr(), out.data_ptr(), stream, d_active=mask.data_ptr())
+        torch.cuda.synchronize()
+        tune["rollout_masked"] = {"chosen": batch.get_opts(), "points": pts, "active_bodies": int(mask.sum())}
     ro.capture()
     ro.audits.clear()
     ro.k = 0
