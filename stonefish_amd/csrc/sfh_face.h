@@ -206,24 +206,30 @@ struct Wet {
 // Tetra term of the submerged-volume integral for a triangle with normal
 // vector n (|n| = twice its area) through the point base + q:
 // 6V = (base - o + q) . n, and the centroid term (3(base - o) + s)/4 * 6V where
-// s is the sum of the triangle's offsets from base.
-template <typename T>
+// s is the sum of the triangle's offsets from base.  CB false: the centroid
+// term is not wanted (a surface with waves takes the buoyancy from the
+// pressure integral, the volume only for Vsub).
+template <typename T, bool CB = true>
 SFH_HD void tetra_term(f3 bo, f3 q, f3 n, f3 s, Acc<T>& acc)
 {
     float v6 = dot(bo + q, n);
-    f3 g = (bo * 3.0f + s) * 0.25f;
-    add3(acc.CB, g * v6);
+    if (CB) {
+        f3 g = (bo * 3.0f + s) * 0.25f;
+        add3(acc.CB, g * v6);
+    }
     acc.V6 += (T)v6;
 }
 
 // tetra_term with q = 0 (the polygon's first point is its base vertex):
 // the same values without the three additions of zero.
-template <typename T>
+template <typename T, bool CB = true>
 SFH_HD void tetra_term0(f3 bo, f3 n, f3 s, Acc<T>& acc)
 {
     float v6 = dot(bo, n);
-    f3 g = (bo * 3.0f + s) * 0.25f;
-    add3(acc.CB, g * v6);
+    if (CB) {
+        f3 g = (bo * 3.0f + s) * 0.25f;
+        add3(acc.CB, g * v6);
+    }
     acc.V6 += (T)v6;
 }
 
@@ -234,7 +240,7 @@ SFH_HD void tetra_term0(f3 bo, f3 n, f3 s, Acc<T>& acc)
 // Every polygon is carried as a wet base vertex plus offsets built from mesh
 // edges: a clip point is base + (edge * t), so edges, normals and areas of thin
 // wet slivers keep FP32 precision relative to the sliver, not to the body.
-template <typename T>
+template <typename T, bool CB = true>
 SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1, float d2, Acc<T>& acc)
 {
     Wet r;
@@ -248,7 +254,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
             f3 q2 = (p2 - p3) * (d2 / (fabsf(d1) + d2));
             fv1 = q2 - q1; fv2 = q1 * -1.0f;
             f3 n = cross(fv1, fv2);
-            tetra_term(base - B.o, q1, n, q1 + q2, acc);
+            tetra_term<T, CB>(base - B.o, q1, n, q1 + q2, acc);
             csum = q1 + q2;
         } else if (d2 < 0.f) {          // B: only p2 wet (1399-1433)
             base = p2;
@@ -256,7 +262,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
             f3 q3 = (p3 - p2) * (d1 / (fabsf(d2) + d1));
             fv1 = q1 * -1.0f; fv2 = q3 - q1;
             f3 n = cross(fv1, fv2);
-            tetra_term(base - B.o, q1, n, q1 + q3, acc);
+            tetra_term<T, CB>(base - B.o, q1, n, q1 + q3, acc);
             csum = q1 + q3;
         } else {                        // C: only p1 dry, quad p1' p2 p3 p4 (1434-1482)
             base = p2;
@@ -264,8 +270,8 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
             f3 q1 = (p1 - p2) * (d1 / (fabsf(d0) + d1));
             f3 q4 = e23 + (p1 - p3) * (d2 / (fabsf(d0) + d2));
             f3 bo = base - B.o;
-            tetra_term(bo, q1, cross(q1 * -1.0f, e23 - q1), q1 + e23, acc);      // (p1', p2, p3)
-            tetra_term(bo, q1, cross(e23 - q1, q4 - q1), q1 + e23 + q4, acc);   // (p1', p3, p4)
+            tetra_term<T, CB>(bo, q1, cross(q1 * -1.0f, e23 - q1), q1 + e23, acc);      // (p1', p2, p3)
+            tetra_term<T, CB>(bo, q1, cross(e23 - q1, q4 - q1), q1 + e23 + q4, acc);   // (p1', p3, p4)
             fv1 = q1 * -1.0f; fv2 = q4 - q1; fv3 = e23 * -1.0f; fv4 = q4 - e23;
             csum = q1 + e23 + q4;
             quad = true;
@@ -277,7 +283,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
             f3 q3 = (p3 - p1) * (d0 / (fabsf(d2) + d0));
             fv1 = q2; fv2 = q3;
             f3 n = cross(fv1, fv2);
-            tetra_term0(base - B.o, n, q2 + q3, acc);
+            tetra_term0<T, CB>(base - B.o, n, q2 + q3, acc);
             csum = q2 + q3;
         } else {                        // E: only p2 dry, quad p1 p2' p4 p3 (1521-1568)
             base = p1;
@@ -285,8 +291,8 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
             f3 q2 = (p2 - p1) * (d0 / (fabsf(d1) + d0));
             f3 q4 = q3 + (p2 - p3) * (d2 / (fabsf(d1) + d2));
             f3 bo = base - B.o;
-            tetra_term0(bo, cross(q2, q3), q2 + q3, acc);     // (p1, p2', p3)
-            tetra_term(bo, q2, cross(q4 - q2, q3 - q2), q2 + q4 + q3, acc);     // (p2', p4, p3)
+            tetra_term0<T, CB>(bo, cross(q2, q3), q2 + q3, acc);     // (p1, p2', p3)
+            tetra_term<T, CB>(bo, q2, cross(q4 - q2, q3 - q2), q2 + q4 + q3, acc);     // (p2', p4, p3)
             fv1 = q2; fv2 = q3; fv3 = q2 - q3; fv4 = q4 - q3;
             csum = q2 + q3 + q4;
             quad = true;
@@ -297,8 +303,8 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
         f3 q3 = q2 + (p3 - p2) * (d1 / (fabsf(d2) + d1));
         f3 q4 = (p3 - p1) * (d0 / (fabsf(d2) + d0));
         f3 bo = base - B.o;
-        tetra_term0(bo, cross(q2, q3), q2 + q3, acc);         // (p1, p2, p3')
-        tetra_term0(bo, cross(q3, q4), q3 + q4, acc);         // (p1, p3', p4)
+        tetra_term0<T, CB>(bo, cross(q2, q3), q2 + q3, acc);         // (p1, p2, p3')
+        tetra_term0<T, CB>(bo, cross(q3, q4), q3 + q4, acc);         // (p1, p3', p4)
         fv1 = q2; fv2 = q4; fv3 = q2 - q3; fv4 = q4 - q3;
         csum = q2 + q3 + q4;
         quad = true;
@@ -306,7 +312,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
         base = p1;
         fv1 = p2 - p1; fv2 = p3 - p1;
         f3 n = cross(fv1, fv2);
-        tetra_term0(base - B.o, n, fv1 + fv2, acc);
+        tetra_term0<T, CB>(base - B.o, n, fv1 + fv2, acc);
         csum = fv1 + fv2;
     }
     f3 n = cross(fv1, fv2);
@@ -330,7 +336,7 @@ SFH_HD Wet face_clip(const FaceBody& B, f3 p1, f3 p2, f3 p3, float d0, float d1,
 
 // All-wet face (case G of face_clip, every vertex depth >= 0): the same
 // operations in the same order, without the case analysis.
-template <typename T>
+template <typename T, bool CB = true>
 SFH_HD Wet face_allwet(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
 {
     Wet r;
@@ -338,7 +344,7 @@ SFH_HD Wet face_allwet(const FaceBody& B, f3 p1, f3 p2, f3 p3, Acc<T>& acc)
     const f3 base = p1;
     const f3 fv1 = p2 - p1, fv2 = p3 - p1;
     f3 n = cross(fv1, fv2);
-    tetra_term0(base - B.o, n, fv1 + fv2, acc);
+    tetra_term0<T, CB>(base - B.o, n, fv1 + fv2, acc);
     const f3 csum = fv1 + fv2;
     n = cross(fv1, fv2);
     float n2 = dot(n, n);

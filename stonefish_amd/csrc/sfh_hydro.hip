@@ -231,9 +231,10 @@ __device__ __forceinline__ float depth_vertex(const BodyDyn& D, const OceanK& oc
 
 // Depth of a wet polygon's centroid for the pressure buoyancy
 // (SolidEntity.cpp:1650-1658), with the body's wave constants hoisted out of
-// the face loop for a chunk: NL layers known at compile time (0, 1, 2),
-// their constants read once into registers; NL < 0 reads the layer count at
-// run time.  The centroid is formed in FP64 from its FP32 wet base vertex
+// the face loop for a chunk: NL layers known at compile time (0, 1, 2) on a
+// power-of-two grid (the texel wrap is a mask), their constants read once
+// into registers; NL < 0 reads the layer count at run time and wraps any
+// grid size.  The centroid is formed in FP64 from its FP32 wet base vertex
 // and offset (exact: both are FP32 numbers at most a face apart), and its
 // depth is the FP64 query of depth64.  Near the waterline a depth is a
 // small difference of body-scale values, so a centroid rounded to FP32 at
@@ -283,15 +284,8 @@ struct WaveQ {
             const double u = fu[l] + ru[l][0] * x + ru[l][1] * y + ru[l][2] * z;
             const double v = fv[l] + rv[l][0] * x + rv[l][1] * y + rv[l][2] * z;
             const double fi = floor(u), fj = floor(v);
-            int i, j;
-            if (nmask >= 0) {
-                i = (ku[l] + (int)fi) & nmask;
-                j = (kv[l] + (int)fj) & nmask;
-            } else {
-                i = wrap_n(ku[l] + (int)fi, n);
-                j = wrap_n(kv[l] + (int)fj, n);
-            }
-            const f32x4 t = quad[l][(size_t)j * n + i];
+            const int i = (ku[l] + (int)fi) & nmask, j = (kv[l] + (int)fj) & nmask;
+            const f32x4 t = quad[l][(uint32_t)j * (uint32_t)n + (uint32_t)i];
             d += texel_mix(u - fi, v - fj, make_float4(t.x, t.y, t.z, t.w));
         }
         return d;
@@ -879,11 +873,11 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
     const uint32_t flags = D.flags;
     const bool damping = flags & BF_DAMPING;
     const bool wave_buoy = (flags & BF_BUOYANT) && oc.nlayers > 0;
+    const f3 zero = mk3(0.f, 0.f, 0.f);
     if (WIN) {
         // four vertices per lane in flight: their loads, then their texel
         // gathers, issue together (indices past the window are clamped to
         // its last vertex; those results are not stored)
-        const f3 zero = mk3(0.f, 0.f, 0.f);
         bool wet = false;
         for (uint32_t v0 = 0; v0 < vn; v0 += 256u) {
             f3 p[4];
@@ -911,8 +905,11 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
     }
     auto dep = [&](uint32_t i, f3 p) -> float {
         if (WIN) return ldep[i - vlo];
-        return (float)wq.depth(p, mk3(0.f, 0.f, 0.f));   // = depth64 at p
+        return (float)wq.depth(p, zero);   // = depth64 at p
     };
+    // the surface's centroid terms are wanted on a flat surface only (with
+    // waves the buoyancy comes from the pressure integral)
+    constexpr bool CBON = NL <= 0;
     Acc<float> acc;
     acc_zero(acc);
     const uint32_t flast = nfc - 1;
@@ -930,7 +927,7 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
         const bool allwet = !(d0 < 0.f) && !(d1 < 0.f) && !(d2 < 0.f);
         const bool cut = valid && !allwet && !(d0 < 0.f && d1 < 0.f && d2 < 0.f);
         if (valid && allwet)
-            wet_contrib<NU, NL>(face_allwet(fb, p1, p2, p3, acc), fb, D, wq, wave_buoy, damping, acc);
+            wet_contrib<NU, NL>(face_allwet<float, CBON>(fb, p1, p2, p3, acc), fb, D, wq, wave_buoy, damping, acc);
         const uint64_t m = __ballot(cut);
         if (cut) cutq[ncut + (uint32_t)__popcll(m & below)] = (uint16_t)(64u * j + lane);
         ncut += (uint32_t)__popcll(m);
@@ -946,7 +943,7 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
             const uint32_t fo = cutq[qb + lane];
             const uint32_t i0 = faces[3 * fo], i1 = faces[3 * fo + 1], i2 = faces[3 * fo + 2];
             const f3 v0 = vert(verts, i0), v1 = vert(verts, i1), v2 = vert(verts, i2);
-            const Wet wt = face_clip(fb, v0, v1, v2, dep(i0, v0), dep(i1, v1), dep(i2, v2), acc);
+            const Wet wt = face_clip<float, CBON>(fb, v0, v1, v2, dep(i0, v0), dep(i1, v1), dep(i2, v2), acc);
             wet_contrib<NU, NL>(wt, fb, D, wq, wave_buoy, damping, acc);
         }
     }
@@ -2199,7 +2196,7 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s)
         if (b->aero) {                                                                                          \
             if (oc.ncur) LAUNCH_F(KK, true, 0, kPassAero) else LAUNCH_F(KK, false, 0, kPassAero)                \
         } else if (oc.ncur) LAUNCH_X(KK, true, -1)                                                              \
-        else if (oc.nlayers > 2) LAUNCH_X(KK, false, -1)                                                        \
+        else if (oc.nlayers > 2 || (oc.nlayers > 0 && oc.nmask < 0)) LAUNCH_X(KK, false, -1)                   \
         else if (oc.nlayers == 0) LAUNCH_X(KK, false, 0)                                                        \
         else if (oc.nlayers == 1) LAUNCH_X(KK, false, 1)                                                        \
         else LAUNCH_X(KK, false, 2)                                                                             \
