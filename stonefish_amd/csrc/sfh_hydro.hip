@@ -797,6 +797,7 @@ __device__ __forceinline__ void sub_chunk(uint32_t c, uint32_t f0, uint32_t nfc,
     uint32_t a0 = faces[3 * fc0], a1 = faces[3 * fc0 + 1], a2 = faces[3 * fc0 + 2];
     uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
     f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
+#pragma unroll 2
     for (int j = 0; j < K; ++j) {
         const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
         f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
@@ -918,6 +919,9 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
     uint32_t b0 = faces[3 * fc1], b1 = faces[3 * fc1 + 1], b2 = faces[3 * fc1 + 2];
     f3 p1 = vert(verts, a0), p2 = vert(verts, a1), p3 = vert(verts, a2);
     uint32_t ncut = 0;
+    // unrolled by two: the pipeline's registers alternate instead of being
+    // copied at the end of every face
+#pragma unroll 2
     for (int j = 0; j < K; ++j) {
         const uint32_t fn2 = min(64u * (j + 2) + lane, flast);
         f3 q1 = vert(verts, b0), q2 = vert(verts, b1), q3 = vert(verts, b2);
