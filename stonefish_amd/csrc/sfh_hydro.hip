@@ -272,6 +272,43 @@ struct WaveQ {
         }
     }
 
+    // depth() in two halves (NL >= 0): issue() forms the vertical part and the
+    // texel fractions and requests the texels, finish() mixes them in; the
+    // same operations in the same order as depth(), so a caller can put
+    // other work between the texel requests and their use.
+    struct Pending {
+        double d, a[M], b[M];
+        f32x4 t[M];
+    };
+
+    __device__ __forceinline__ Pending issue(f3 base, f3 off) const
+    {
+        Pending q;
+        const double x = (double)base.x + (double)off.x, y = (double)base.y + (double)off.y,
+                     z = (double)base.z + (double)off.z;
+        q.d = tz + rz[0] * x + rz[1] * y + rz[2] * z;
+#pragma unroll
+        for (int l = 0; l < (NL > 0 ? NL : 0); ++l) {
+            const double u = fu[l] + ru[l][0] * x + ru[l][1] * y + ru[l][2] * z;
+            const double v = fv[l] + rv[l][0] * x + rv[l][1] * y + rv[l][2] * z;
+            const double fi = floor(u), fj = floor(v);
+            const int i = (ku[l] + (int)fi) & nmask, j = (kv[l] + (int)fj) & nmask;
+            q.t[l] = quad[l][(uint32_t)j * (uint32_t)n + (uint32_t)i];
+            q.a[l] = u - fi;
+            q.b[l] = v - fj;
+        }
+        return q;
+    }
+
+    __device__ __forceinline__ double finish(const Pending& q) const
+    {
+        double d = q.d;
+#pragma unroll
+        for (int l = 0; l < (NL > 0 ? NL : 0); ++l)
+            d += texel_mix(q.a[l], q.b[l], make_float4(q.t[l].x, q.t[l].y, q.t[l].z, q.t[l].w));
+        return d;
+    }
+
     // depth64 at the mesh-frame point base + off (FP32 parts, summed in FP64)
     __device__ __forceinline__ double depth(f3 base, f3 off) const
     {
@@ -836,6 +873,21 @@ __device__ __forceinline__ void wet_contrib(const Wet& wt, const FaceBody& fb, c
 {
     if (!wt.ok) return;
     f3 cpt = wt.base + wt.off;
+    if (NL > 0 && !NU) {
+        // the centroid's texels are requested first and mixed in after the
+        // drag, whose arithmetic covers their latency
+        typename WaveQ<NL>::Pending q;
+        if (wave_buoy) q = wq.issue(wt.base, wt.off);
+        if (damping) face_drag(fb, cpt, wt.n1, wt.A, acc);
+        if (wave_buoy) {
+            const float dc = (float)wq.finish(q);
+            f3 fbi = wt.n1 * (-wt.A * dc);
+            add3(acc.Fb, fbi);
+            add3(acc.Mb, cross(cpt, fbi));
+        }
+        acc.Swet += wt.A;
+        return;
+    }
     if (NL != 0 && wave_buoy) {
         const float dc = (float)wq.depth(wt.base, wt.off);
         f3 fbi = wt.n1 * (-wt.A * dc);
