@@ -10,10 +10,13 @@
 //   dH        dH = dZ W                                     (A = dZ,   Bt = W^T)
 //   dW        dW = dZ^T H, split over the batch             (A = dZ^T, Bt = H^T)
 //
-// One wavefront computes one 32x32 tile with v_mfma_f32_32x32x16_bf16; the
-// epilogues fuse bias, GELU (exact erf) and its derivative, the loss and its
-// gradient, BF16 rounding and the transposed stores.  Weight-gradient partials
-// of the batch splits are summed in a fixed order by the Adam kernel.
+// Tiles are 32x32 v_mfma_f32_32x32x16_bf16 accumulators: the products of the
+// 256-wide layers run 64x64 per workgroup with the operands staged in LDS,
+// the 32-wide ones one tile per wavefront.  The epilogues fuse bias, GELU
+// (exact erf) and its derivative, the loss and its gradient, BF16 rounding
+// and the transposed stores; the weight-gradient products also sum dZ's
+// columns (the bias gradient).  Weight- and bias-gradient partials of the
+// batch splits are summed in a fixed order by the Adam kernel.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -59,7 +62,13 @@ extern "C" const char* sfm_last_error(void) { return g_err; }
 static constexpr int kDims[SFM_LAYERS + 1] = {SFM_IN_PAD, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_OUT_PAD};
 static constexpr int kReal[SFM_LAYERS + 1] = {SFM_IN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_HIDDEN, SFM_OUT};
 static constexpr int kSplitK = 256;   // batch rows per weight-gradient split
-static constexpr int kMaxSplits = 16; // batch splits of a weight-gradient product
+static constexpr int kMaxSplits = 64; // batch splits of a weight-gradient product
+
+// Batch splits of layer l's weight-gradient product at most: the 256 x 256
+// products run 16 workgroups per split, the 32-wide ones 2, so the latter
+// take more splits to fill the chip; every split's partial sum is read again
+// by the Adam launch.
+static constexpr int max_splits(int l) { return kDims[l] == SFM_HIDDEN && kDims[l + 1] == SFM_HIDDEN ? 32 : kMaxSplits; }
 
 // ---------------------------------------------------------------------------
 // numerics
@@ -87,74 +96,30 @@ struct GemmArgs {
     float* loss;            //                              (OUT-train)
     float scale;            // 1 / (rows * SFM_OUT)         (OUT-train)
     float* part;            // [ksplit][rows x cols] FP32  (WGRAD)
+    float* bpart;           // [ksplit][rows] FP32 row sums of A over each split (WGRAD, optional: bias gradient)
 };
 
-
-// One wavefront computes a (32 TM) x (32 TN) block of C as TM x TN MFMA
-// tiles: per k-step TM + TN 16-byte operand loads feed TM * TN MFMAs.
-template <int EPI, int TM, int TN>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
+__device__ __forceinline__ float sum8(const bf16x8& v)
 {
-    const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int blocks_n = g.cols / (32 * TN);
-    const int bm = wave / blocks_n, bn = wave - bm * blocks_n;
-    if (bm * 32 * TM >= g.rows) return;
-    const int split = blockIdx.y;
-    const int kper = g.K / g.ksplit;
-    const int k0 = split * kper;
-    const int kk = (lane >> 5) * 8;
-    const uint16_t* ap[TM];
-    const uint16_t* bp[TN];
-#pragma unroll
-    for (int i = 0; i < TM; ++i) ap[i] = g.A + (size_t)((bm * TM + i) * 32 + (lane & 31)) * g.lda + k0 + kk;
-#pragma unroll
-    for (int j = 0; j < TN; ++j) bp[j] = g.Bt + (size_t)((bn * TN + j) * 32 + (lane & 31)) * g.ldb + k0 + kk;
-    f32x16 acc[TM][TN];
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = f32x16{};
-    // The operands of the next kPF k-steps are in flight while the current
-    // one multiplies (a ring of kPF register stages, refilled right after use):
-    // a k-step waits for a load issued kPF steps earlier, not for the last one.
-    constexpr int kPF = 4;
-    bf16x8 ra[kPF][TM], rb[kPF][TN];
-#pragma unroll
-    for (int s = 0; s < kPF; ++s)
-        if (16 * s < kper) {
-#pragma unroll
-            for (int i = 0; i < TM; ++i) ra[s][i] = ld8g(ap[i] + 16 * s);
-#pragma unroll
-            for (int j = 0; j < TN; ++j) rb[s][j] = ld8g(bp[j] + 16 * s);
-        }
-    for (int k0 = 0; k0 < kper; k0 += 16 * kPF) {
-#pragma unroll
-        for (int s = 0; s < kPF; ++s) {
-            const int k = k0 + 16 * s;
-            if (k < kper) {
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int j = 0; j < TN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[s][i], rb[s][j], acc[i][j], 0, 0, 0);
-                if (k + 16 * kPF < kper) {
-#pragma unroll
-                    for (int i = 0; i < TM; ++i) ra[s][i] = ld8g(ap[i] + k + 16 * kPF);
-#pragma unroll
-                    for (int j = 0; j < TN; ++j) rb[s][j] = ld8g(bp[j] + k + 16 * kPF);
-                }
-            }
-        }
-    }
+    return (((float)v[0] + (float)v[1]) + ((float)v[2] + (float)v[3])) +
+           (((float)v[4] + (float)v[5]) + ((float)v[6] + (float)v[7]));
+}
+
+
+// The epilogue of a wavefront's TM x TN tiles of C: tile (ti, tj) covers rows
+// (rt0 + ti) * 32 .. + 31 and columns (ct0 + tj) * 32 .. + 31.
+template <int EPI, int TM, int TN>
+__device__ __forceinline__ void gemm_store(const GemmArgs& g, const f32x16 (&acc)[TM][TN], int rt0, int ct0, int split,
+                                           int lane)
+{
     float sq = 0.f;
 #pragma unroll
     for (int ti = 0; ti < TM; ++ti) {
 #pragma unroll
         for (int tj = 0; tj < TN; ++tj) {
             const f32x16& A = acc[ti][tj];
-            const int n = (bn * TN + tj) * 32 + (lane & 31);
-            const int mb = (bm * TM + ti) * 32;
+            const int n = (ct0 + tj) * 32 + (lane & 31);
+            const int mb = (rt0 + ti) * 32;
             if (EPI == EPI_WGRAD) {
                 float* out = g.part + (size_t)split * g.rows * g.cols;
 #pragma unroll
@@ -206,6 +171,184 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
         for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off, 64);
         if (lane == 0) atomicAdd(g.loss, sq * g.scale);
     }
+}
+
+// One wavefront computes a (32 TM) x (32 TN) block of C as TM x TN MFMA
+// tiles: per k-step TM + TN 16-byte operand loads feed TM * TN MFMAs.
+template <int EPI, int TM, int TN>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int blocks_n = g.cols / (32 * TN);
+    const int bm = wave / blocks_n, bn = wave - bm * blocks_n;
+    if (bm * 32 * TM >= g.rows) return;
+    const int split = blockIdx.y;
+    const int kper = g.K / g.ksplit;
+    const int k0 = split * kper;
+    const int kk = (lane >> 5) * 8;
+    const uint16_t* ap[TM];
+    const uint16_t* bp[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) ap[i] = g.A + (size_t)((bm * TM + i) * 32 + (lane & 31)) * g.lda + k0 + kk;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) bp[j] = g.Bt + (size_t)((bn * TN + j) * 32 + (lane & 31)) * g.ldb + k0 + kk;
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x16{};
+    // The operands of the next kPF k-steps are in flight while the current
+    // one multiplies (a ring of kPF register stages, refilled right after use):
+    // a k-step waits for a load issued kPF steps earlier, not for the last one.
+    constexpr int kPF = 4;
+    bf16x8 ra[kPF][TM], rb[kPF][TN];
+#pragma unroll
+    for (int s = 0; s < kPF; ++s)
+        if (16 * s < kper) {
+#pragma unroll
+            for (int i = 0; i < TM; ++i) ra[s][i] = ld8g(ap[i] + 16 * s);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) rb[s][j] = ld8g(bp[j] + 16 * s);
+        }
+    // the bias gradient rides along the weight gradient: the wavefronts of the
+    // first column of tiles also sum their rows of A (each lane 8 of the 16 k
+    // of a k-step, the two half-waves combined at the end)
+    const bool rsum = EPI == EPI_WGRAD && g.bpart && bn == 0;
+    float rs[TM] = {};
+    for (int k0 = 0; k0 < kper; k0 += 16 * kPF) {
+#pragma unroll
+        for (int s = 0; s < kPF; ++s) {
+            const int k = k0 + 16 * s;
+            if (k < kper) {
+                if (rsum) {
+#pragma unroll
+                    for (int i = 0; i < TM; ++i) rs[i] += sum8(ra[s][i]);
+                }
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[s][i], rb[s][j], acc[i][j], 0, 0, 0);
+                if (k + 16 * kPF < kper) {
+#pragma unroll
+                    for (int i = 0; i < TM; ++i) ra[s][i] = ld8g(ap[i] + k + 16 * kPF);
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) rb[s][j] = ld8g(bp[j] + k + 16 * kPF);
+                }
+            }
+        }
+    }
+    if (rsum) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            const float v = rs[i] + __shfl_xor(rs[i], 32, 64);
+            if (lane < 32) g.bpart[(size_t)split * g.rows + (bm * TM + i) * 32 + lane] = v;
+        }
+    }
+    gemm_store<EPI, TM, TN>(g, acc, bm * TM, bn * TN, split, lane);
+}
+
+// Workgroup-tiled product for the 256-wide layers: the four wavefronts of a
+// workgroup, in a 2 x 2 arrangement, cover (64 WM) x (64 WN) of C, each
+// wavefront WM x WN 32x32 tiles.  K advances in chunks of kKC: the workgroup
+// copies the chunk's rows of A and of Bt into LDS once (the next chunk's
+// global loads in flight while the current one multiplies; two LDS buffers)
+// and the wavefronts read their MFMA operands from there, so an operand row
+// comes from L2 once per workgroup instead of once per wavefront that uses it.
+static constexpr int kKC = 64;            // K per chunk
+static constexpr int kLdsRow = kKC + 8;   // bf16 per LDS row: 16 bytes of padding against bank conflicts
+
+template <int EPI, int WM, int WN>
+__global__ __launch_bounds__(256) void k_gemm_lds(GemmArgs g)
+{
+    constexpr int BM = 64 * WM, BN = 64 * WN;
+    constexpr int SEG = kKC / 8;                 // 16-byte segments per row and chunk
+    constexpr int SA = BM * SEG / 256, SB = BN * SEG / 256;
+    static_assert(SA * 256 == BM * SEG && SB * 256 == BN * SEG, "tile rows must fill the workgroup's copies");
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * kLdsRow];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
+    const int blocks_n = g.cols / BN;
+    const int bm = blockIdx.x / blocks_n, bn = blockIdx.x - bm * blocks_n;
+    const int split = blockIdx.y;
+    const int kper = g.K / g.ksplit;
+    const int kbase = split * kper;
+    const int nch = kper / kKC;
+    bf16x8 ra[SA], rb[SB];
+    auto gload = [&](int c) {
+        const int k0 = kbase + c * kKC;
+#pragma unroll
+        for (int i = 0; i < SA; ++i) {
+            const int idx = t + 256 * i, row = idx / SEG, seg = idx % SEG;
+            ra[i] = ld8g(g.A + (size_t)(bm * BM + row) * g.lda + k0 + seg * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < SB; ++i) {
+            const int idx = t + 256 * i, row = idx / SEG, seg = idx % SEG;
+            rb[i] = ld8g(g.Bt + (size_t)(bn * BN + row) * g.ldb + k0 + seg * 8);
+        }
+    };
+    // bias gradient (see k_gemm): the workgroups of the first column of
+    // tiles sum the rows of A they copy, each thread its 8-value segments
+    const bool rsum = EPI == EPI_WGRAD && g.bpart && bn == 0;
+    float rs[SA] = {};
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < SA; ++i) {
+            const int idx = t + 256 * i, row = idx / SEG, seg = idx % SEG;
+            *reinterpret_cast<bf16x8*>(&As[buf][row * kLdsRow + seg * 8]) = ra[i];
+            if (rsum) rs[i] += sum8(ra[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < SB; ++i) {
+            const int idx = t + 256 * i, row = idx / SEG, seg = idx % SEG;
+            *reinterpret_cast<bf16x8*>(&Bs[buf][row * kLdsRow + seg * 8]) = rb[i];
+        }
+    };
+    f32x16 acc[WM][WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[i][j] = f32x16{};
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nch) gload(c + 1);
+#pragma unroll
+        for (int s = 0; s < kKC / 16; ++s) {
+            const int kk = 16 * s + (lane >> 5) * 8;
+            bf16x8 fa[WM], fb[WN];
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+                fa[i] = *reinterpret_cast<const bf16x8*>(&As[buf][((wm * WM + i) * 32 + (lane & 31)) * kLdsRow + kk]);
+#pragma unroll
+            for (int j = 0; j < WN; ++j)
+                fb[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][((wn * WN + j) * 32 + (lane & 31)) * kLdsRow + kk]);
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        if (c + 1 < nch) lstore(buf ^ 1);
+        __syncthreads();
+    }
+    if (rsum) {   // the SEG threads of a row are consecutive lanes
+        static_assert(SEG == 8, "row segments per chunk");
+#pragma unroll
+        for (int i = 0; i < SA; ++i) {
+            float v = rs[i];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            const int idx = t + 256 * i;
+            if (idx % SEG == 0) g.bpart[(size_t)split * g.rows + bm * BM + idx / SEG] = v;
+        }
+    }
+    gemm_store<EPI, WM, WN>(g, acc, (bm * 2 + wm) * WM, (bn * 2 + wn) * WN, split, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -262,27 +405,6 @@ __global__ void k_transpose(const uint16_t* __restrict__ x, int rows, int cols, 
     xt[(size_t)c * rows + r] = x[i];
 }
 
-// Row sums of xt [n x m] (BF16): one 256-thread workgroup per row, each
-// thread summing 4 consecutive values at a time, then a fixed-order tree
-// (deterministic).  m is a multiple of 256.
-__global__ __launch_bounds__(256) void k_rowsum(const uint16_t* __restrict__ xt, int m, float* __restrict__ out)
-{
-    __shared__ float red[4];
-    const int row = blockIdx.x, t = threadIdx.x;
-    const uint16_t* p = xt + (size_t)row * m;
-    float s = 0.f;
-    for (int i = 4 * t; i < m; i += 1024) {
-        const uint2 v = *reinterpret_cast<const uint2*>(p + i);
-        s += __uint_as_float(v.x << 16) + __uint_as_float(v.x & 0xffff0000u) + __uint_as_float(v.y << 16) +
-             __uint_as_float(v.y & 0xffff0000u);
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((t & 63) == 0) red[t >> 6] = s;
-    __syncthreads();
-    if (t == 0) out[row] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
 // Adam on n parameters of a [rows x cols] matrix (cols = 1 for biases): the
 // gradient is the sum of nsplit partials (fixed order); keeps the FP32 master
 // copy and refreshes the BF16 copy and its transpose.
@@ -314,7 +436,8 @@ __global__ __launch_bounds__(256) void k_adam(AdamSet S, float lr, float b1, flo
     const int i = ((int)blockIdx.x - g.block0) * (int)blockDim.x + (int)threadIdx.x;
     if (i >= n) return;
     float gr = 0.f;
-    for (int s = 0; s < g.nsplit; ++s) gr += g.part[(size_t)s * n + i];
+#pragma unroll 8
+    for (int s = 0; s < g.nsplit; ++s) gr += g.part[(size_t)s * n + i];   // loads in flight 8 at a time
     g.gsave[i] = gr;
     const float m = b1 * g.mo[i] + (1.f - b1) * gr;
     const float v = b2 * g.ve[i] + (1.f - b2) * gr * gr;
@@ -373,8 +496,8 @@ struct sfm_net_t {
     uint16_t* Ht[SFM_LAYERS] = {};   // transposed inputs
     float* Z[SFM_LAYERS] = {};       // pre-activations of layers 0..3 outputs (Z[l] = output of layer l)
     uint16_t *dZ[2] = {}, *dZt[2] = {};
-    float* part[SFM_LAYERS] = {};    // per layer: weight-gradient partials of the batch splits (<= 16)
-    float* dbias[SFM_LAYERS] = {};   // per layer: bias gradient
+    float* part[SFM_LAYERS] = {};    // per layer: weight-gradient partials of the batch splits (<= max_splits)
+    float* dbias[SFM_LAYERS] = {};   // per layer: bias-gradient partials of the same splits
     uint64_t step = 0;
     bool timing = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -457,8 +580,8 @@ extern "C" int sfm_create(uint32_t max_batch, uint64_t seed, sfm_net* out)
         ok = alloc((void**)&n->dZ[i], 2 * (size_t)max_batch * SFM_HIDDEN) &&
              alloc((void**)&n->dZt[i], 2 * (size_t)max_batch * SFM_HIDDEN);
     for (int l = 0; l < SFM_LAYERS && ok; ++l)
-        ok = alloc((void**)&n->part[l], 4 * (size_t)kMaxSplits * kDims[l + 1] * kDims[l]) &&
-             alloc((void**)&n->dbias[l], 4 * (size_t)kDims[l + 1]);
+        ok = alloc((void**)&n->part[l], 4 * (size_t)max_splits(l) * kDims[l + 1] * kDims[l]) &&
+             alloc((void**)&n->dbias[l], 4 * (size_t)max_splits(l) * kDims[l + 1]);
     if (!ok) { net_free(n); delete n; return fail(SFM_ERR_HIP, "allocation failed"); }
     // Kaiming-uniform weights on the real fan-in, zero biases, padding zero
     uint64_t s = seed;
@@ -507,12 +630,20 @@ extern "C" int sfm_layer_io(sfm_net n, int l, float* w, float* b, int set)
     return SFM_OK;
 }
 
-// One tile per wavefront: on these shapes (K <= 1024) more resident
-// wavefronts hide the operand latency better than 2 x 2 blocks (measured:
-// 8.6 s vs 9.7 s for 20,000 steps of 16,384).
+// The products whose rows and columns are multiples of 64 and whose K (per
+// batch split) is a multiple of kKC run workgroup-tiled through LDS, 64 x 64
+// per workgroup (the 256-wide layers' forward, backward and weight-gradient
+// products: 1,024 workgroups for 16,384 x 256, where 128 x 128 tiles leave
+// 256 and the operand latency exposed; measured 97 vs 85 TFLOP/s training);
+// the 32-wide ones (the first layer's K, the output layer) one 32 x 32 tile
+// per wavefront (k_gemm).
 template <int EPI>
 static void launch_gemm(const GemmArgs& g, hipStream_t s)
 {
+    if ((g.K / g.ksplit) % kKC == 0 && g.rows % 64 == 0 && g.cols % 64 == 0) {
+        hipLaunchKernelGGL((k_gemm_lds<EPI, 1, 1>), dim3((g.rows / 64) * (g.cols / 64), g.ksplit), dim3(256), 0, s, g);
+        return;
+    }
     const bool m2 = false, n2 = false;
     const int blocks = (g.rows / (m2 ? 64 : 32)) * (g.cols / (n2 ? 64 : 32));
     const dim3 grid((blocks + 3) / 4, g.ksplit);
@@ -681,16 +812,17 @@ extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target,
         Layer& L = n->L[l];
         // dW = dZ^T H, split over the batch
         {
-            // batch splits of the weight-gradient product: at most 16, each a
-            // multiple of 256 rows, summed in a fixed order by k_adam
+            // batch splits of the weight-gradient product: at most
+            // max_splits(l), each a multiple of 256 rows; their partial sums
+            // (weights and biases) are summed in a fixed order by k_adam
             int splits = 1;
-            for (int d = kMaxSplits; d >= 1; --d)
+            for (int d = max_splits(l); d >= 1; --d)
                 if ((M / kSplitK) % d == 0) { splits = d; break; }
             GemmArgs g = gemm(n->dZt[cur], M, n->Ht[l], M, dout, din, M);
             g.ksplit = splits;
             g.part = n->part[l];
+            g.bpart = n->dbias[l];
             launch_gemm<EPI_WGRAD>(g, s);
-            hipLaunchKernelGGL(k_rowsum, dim3(dout), dim3(256), 0, s, n->dZt[cur], M, n->dbias[l]);
             // dH = dZ W  (with the pre-update weights), then dZ of the layer below
             if (l > 0) {
                 GemmArgs h = gemm(n->dZ[cur], dout, L.Wbt, dout, M, din, dout);
@@ -706,7 +838,7 @@ extern "C" int sfm_train_step(sfm_net n, const uint16_t* x, const float* target,
             sw = AdamSeg{L.W, L.mW, L.vW, n->part[l], L.gW, L.Wb, L.Wbt, L.Wp, splits, dout, din, nblk};
             nblk += (dout * din + 255) / 256;
             AdamSeg& sb = adam.seg[adam.nseg++];
-            sb = AdamSeg{L.b, L.mb, L.vb, n->dbias[l], L.gb, nullptr, nullptr, nullptr, 1, dout, 1, nblk};
+            sb = AdamSeg{L.b, L.mb, L.vb, n->dbias[l], L.gb, nullptr, nullptr, nullptr, splits, dout, 1, nblk};
             nblk += (dout + 255) / 256;
         }
         cur ^= 1;

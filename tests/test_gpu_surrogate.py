@@ -76,11 +76,12 @@ def test_fused_group_and_layer_paths_agree(gpu):
         n.close()
 
 
-def test_gradients_match_reference(gpu):
-    net = _net(1024)
+@pytest.mark.parametrize("batch", [512, 16384])   # 2 and 32/64 batch splits of the weight gradients
+def test_gradients_match_reference(gpu, batch):
+    net = _net(max(1024, batch))
     Ws, bs = R.init_weights(21)
     _load(net, Ws, bs)
-    x, t = R.random_inputs(22, 512)
+    x, t = R.random_inputs(22, batch)
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
     net.train_step(x.to(torch.bfloat16).cuda(), t.cuda().contiguous(), loss, lr=0.0)
     torch.cuda.synchronize()
