@@ -553,6 +553,47 @@ def rollout_section(args, sea) -> dict:
     }
 
 
+def compound_exact_section(args, sea) -> dict:
+    """The rollout's exact step of the AUV compounds alone, for counter runs:
+    `args.rollout_bodies` compounds on the fleet's wave field in the
+    autotuner's configuration, `args.steps` eager steps after the warm-up,
+    then 10 steps timed with HIP events on the launch stream."""
+    import torch
+    from stonefish_amd import hydro as H
+    from stonefish_amd import surrogate as S
+    from stonefish_amd import workloads as WL
+    n = args.rollout_bodies
+    auv = WL.auv_compound()
+    meshes, parts = {}, []
+    for p in auv.parts:
+        if id(p.mesh) not in meshes:
+            meshes[id(p.mesh)] = H.Mesh(p.mesh)
+        parts.append(H.PartDesc(mesh=meshes[id(p.mesh)], origin=p.origin, Cd=p.Cd, Cf=p.Cf))
+    body = H.BodyDesc(mesh=None, aabb_min=auv.aabb_min, aabb_max=auv.aabb_max, volume=auv.volume,
+                      surface=auv.surface, P_CB=auv.P_CB)
+    batch = H.CompoundBatch([H.CompoundDesc(body, parts)] * n)
+    _, nchunks, nfaces = batch.info()
+    ocean, grid = S.make_ocean(sea)
+    rng = np.random.default_rng(77)
+    states = torch.from_numpy(S.sample_states(rng, n, (0.0, sea.wave_len), (0.0, 2.5), v_max=1.0,
+                                              w_max=0.5)).cuda()
+    out = torch.zeros((n, 24), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    if not args.no_autotune:
+        batch.autotune(ocean, states.data_ptr(), out.data_ptr(), stream)
+    t = _time_replays(lambda: batch.compute(ocean, states.data_ptr(), out.data_ptr(), stream), max(1, args.steps))
+    batch.timing(True)
+    for k in range(10):
+        batch.compute(ocean, states.data_ptr(), out.data_ptr(), stream)
+        batch.kernel_ms()
+    torch.cuda.synchronize()
+    face_ms, total_ms, _ = batch.kernel_ms()
+    batch.timing(False)
+    return {"workload": f"{n} AUV compounds ({nfaces} faces, {nchunks} chunks), exact step",
+            "ms_per_step": round(1e3 * t, 4), "face_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),
+            "faces": int(nfaces), "opts": batch.get_opts()}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -576,6 +617,8 @@ def main() -> int:
     ap.add_argument("--single-only", action="store_true", help="run only the single-body workloads")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate and rollout sections")
     ap.add_argument("--rollout-only", action="store_true", help="run only the mixed rollout section (profiling)")
+    ap.add_argument("--compound-exact-only", action="store_true",
+                    help="run only the exact step of the rollout's compounds (profiling)")
     ap.add_argument("--surrogate-only", action="store_true", help="run only the surrogate section (profiling)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB device copy (reference dispatch for HBM byte counters)")
@@ -611,6 +654,10 @@ def main() -> int:
         del a, b
     if args.rollout_only:
         print(json.dumps({"rollout": rollout_section(args, WL.fleet(1, seed=args.seed).sea)}), flush=True)
+        return 0
+    if args.compound_exact_only:
+        print(json.dumps({"compound_exact": compound_exact_section(args, WL.fleet(1, seed=args.seed).sea)}),
+              flush=True)
         return 0
     if args.single_only:
         print(json.dumps({"single_body": single_body_section(args)}), flush=True)
