@@ -362,16 +362,19 @@ __device__ __forceinline__ void gl_basis(const double* m, double* B)  // OpenGL 
 
 __device__ __forceinline__ void mat_t_vec(const double* B, const double* x, double* y)  // B^T x
 {
+#pragma clang fp contract(on)   // FMAs only within an expression: the same rounding wherever this is inlined
     for (int i = 0; i < 3; ++i) y[i] = B[i] * x[0] + B[3 + i] * x[1] + B[6 + i] * x[2];
 }
 
 __device__ __forceinline__ void mat_vec(const double* B, const double* x, double* y)  // B x
 {
+#pragma clang fp contract(on)   // FMAs only within an expression: the same rounding wherever this is inlined
     for (int i = 0; i < 3; ++i) y[i] = B[3 * i] * x[0] + B[3 * i + 1] * x[1] + B[3 * i + 2] * x[2];
 }
 
 __device__ __forceinline__ void cross64(const double* a, const double* b, double* c)
 {
+#pragma clang fp contract(on)   // FMAs only within an expression: the same rounding wherever this is inlined
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];
@@ -1033,6 +1036,7 @@ __device__ __forceinline__ void cross_chunk(uint32_t c, uint32_t f0, uint32_t nf
 __device__ void correct64(double rho, const double* BO, const double* Cd, const double* Cf,
                           double* Fdq, double* Tdq, double* Fdf, double* Tdf)
 {
+#pragma clang fp contract(on)   // FMAs only within an expression: the same rounding wherever this is inlined
 #define CF(r1, c1, r2, c2) (BO[3 * (r1) + (c1)] * BO[3 * (r2) + (c2)] - BO[3 * (r1) + (c2)] * BO[3 * (r2) + (c1)])
     double co0 = CF(1, 1, 2, 2), co1 = CF(1, 2, 2, 0), co2 = CF(1, 0, 2, 1);
     double det = BO[0] * co0 + BO[1] * co1 + BO[2] * co2;
@@ -1066,6 +1070,7 @@ __device__ void correct64(double rho, const double* BO, const double* Cd, const 
 // Minv: the inverse basis, rows, from the host; params: fdApproxParams.
 __device__ void correct_aero64(uint32_t type, const double* params, const double* Minv, double* F, double* T)
 {
+#pragma clang fp contract(on)   // FMAs only within an expression: the same rounding wherever this is inlined
     double d[3];
     mat_vec(Minv, F, d);
     const double l2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -1093,6 +1098,7 @@ __device__ void correct_aero64(uint32_t type, const double* params, const double
 __device__ void epilogue_finish(const BodyStatic& S, const BodyDyn& D, const OceanK& oc, const double* P,
                                 double* __restrict__ o)
 {
+#pragma clang fp contract(on)   // FMAs only within an expression: the same rounding wherever this is inlined
     const int pos = D.pos;
     const uint32_t flags = D.flags;
     const bool body_level = !(flags & (BF_MODE_SURF | BF_MODE_SUBM | BF_MODE_AERO));
@@ -1190,16 +1196,12 @@ __device__ void epilogue_finish(const BodyStatic& S, const BodyDyn& D, const Oce
     o[21] = (double)pos;
 }
 
-// Row of body b (position 1 or 2) on one wavefront: lanes stride over the
-// body's chunk partials, a transposed reduction across the wavefront (a fixed
-// order, so the row does not depend on which wavefront runs this) leaves the
-// 23 totals on 23 lanes, which pass them to lane 0 through `ps` (the
-// wavefront's LDS slice); lane 0 finishes.
-__device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, uint32_t b,
-                              const OceanK& oc, const double* partials, double* __restrict__ out, double* ps,
-                              int lane)
+// Sums of the chunk partials of body b on one wavefront: lanes stride over
+// the body's chunks, a transposed reduction across the wavefront (a fixed
+// order, so the sums do not depend on which wavefront runs this) leaves the
+// 23 totals on 23 lanes, which store them to `ps` (24 doubles of LDS).
+__device__ void epilogue_sums(const BodyStatic& S, const double* partials, double* ps, int lane)
 {
-    const BodyStatic& S = st[b];
     double v[24];
 #pragma unroll
     for (int k = 0; k < 24; ++k) v[k] = 0.0;
@@ -1216,16 +1218,10 @@ __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* 
     xreduce24(v, lane);
     const int q = xreduce24_slot(lane);
     if (q >= 0) ps[q] = v[0];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane == 0) {
-        double P[23];
-#pragma unroll
-        for (int k = 0; k < 23; ++k) P[k] = ps[k];
-        epilogue_finish(S, dyn[b], oc, P, out + (size_t)b * SFH_OUT_DOUBLES);
-    }
-    // ps is rewritten by the wavefront's next body
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1233,18 +1229,23 @@ __device__ void epilogue_body(const BodyStatic* __restrict__ st, const BodyDyn* 
 
 // kEpilogueWaves wavefronts per workgroup; wavefront w of W takes the bodies
 // w, w + W, w + 2W, ...: lane k reads the position of the k-th of the
-// wavefront's next 64 bodies in one load and the wavefront finishes the rows
-// of those with face work (positions 1 and 2; the prologue wrote the others',
-// and those of bodies without faces), in order.  A masked step with few
-// active bodies thus costs two loads per 64 bodies.
+// wavefront's next 64 bodies in one load, and the wavefront sums the chunk
+// partials of those with face work (positions 1 and 2; the prologue wrote the
+// others' rows, and those of bodies without faces), in order, into LDS slots;
+// every kEpilogueBatch bodies (and at the end of the 64) lane j finishes the
+// row of the body in slot j — world rotation, moment about the CoG, buoyancy,
+// CorrectHydrodynamicForces, the write mask — so a wavefront with several
+// bodies runs the finishing arithmetic once per batch, not once per body.
+// A masked step with few active bodies costs two loads per 64 bodies.
 // It also zeroes the work list's counts for the next step's prologue.
 static constexpr int kEpilogueWaves = 4;
+static constexpr int kEpilogueBatch = 16;
 
-__global__ __launch_bounds__(64 * kEpilogueWaves) void k_epilogue(
+__global__ __launch_bounds__(64 * kEpilogueWaves, 4) void k_epilogue(
     const BodyStatic* __restrict__ st, const BodyDyn* __restrict__ dyn, const int8_t* __restrict__ bpos, uint32_t n,
     OceanK oc, const double* __restrict__ partials, double* __restrict__ out, uint32_t* __restrict__ counts)
 {
-    __shared__ double ps[kEpilogueWaves][24];
+    __shared__ double ps[kEpilogueWaves][kEpilogueBatch][24];
     const int lane = threadIdx.x & 63;
     const int wl = threadIdx.x >> 6;
     const uint32_t W = gridDim.x * kEpilogueWaves;
@@ -1254,12 +1255,38 @@ __global__ __launch_bounds__(64 * kEpilogueWaves) void k_epilogue(
         const uint32_t bl = base + (uint32_t)lane * W;
         int p = -3;
         if (bl < n) p = bpos[bl];
-        uint64_t m = __ballot(p == 1 || p == 2);
+        uint64_t m = __ballot((p == 1 || p == 2) && st[bl].nchunks);
+        uint32_t mine = 0;    // lane j < nb: the body in slot j
+        uint32_t first = 0;   // the body in slot 0 (wave-uniform)
+        int nb = 0;
         while (m) {
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             m &= m - 1ull;
             const uint32_t b = base + k * W;
-            if (st[b].nchunks) epilogue_body(st, dyn, b, oc, partials, out, ps[wl], lane);
+            epilogue_sums(st[b], partials, ps[wl][nb], lane);
+            if (nb == 0) first = b;
+            if (lane == nb) mine = b;
+            if (++nb == kEpilogueBatch || !m) {
+                wave_lds_sync();
+                // a batch of one keeps the body's index wave-uniform (its
+                // constants come in through scalar loads); the finishing
+                // arithmetic rounds the same either way
+                if (nb == 1) {
+                    if (lane == 0) {
+                        double P[23];
+#pragma unroll
+                        for (int q = 0; q < 23; ++q) P[q] = ps[wl][0][q];
+                        epilogue_finish(st[first], dyn[first], oc, P, out + (size_t)first * SFH_OUT_DOUBLES);
+                    }
+                } else if (lane < nb) {
+                    double P[23];
+#pragma unroll
+                    for (int q = 0; q < 23; ++q) P[q] = ps[wl][lane][q];
+                    epilogue_finish(st[mine], dyn[mine], oc, P, out + (size_t)mine * SFH_OUT_DOUBLES);
+                }
+                wave_lds_sync();   // the slots are rewritten by the next batch
+                nb = 0;
+            }
         }
     }
 }
