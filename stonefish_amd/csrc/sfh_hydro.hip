@@ -478,6 +478,14 @@ __device__ __forceinline__ int prologue_body(const BodyStatic* __restrict__ st, 
         if (sub == 0) bpos[b] = -2;
         return -2;
     }
+    // The body's BodyDyn record is written by the group's lane 0 as its parts
+    // are computed (a record held whole until one store at the end keeps ~1 KB
+    // live in registers); the wave layers below nlayers, and the currents'
+    // table only when there are varying currents (cur.n says how many), so a
+    // body on one layer without jets or pipes writes 360 of its 944 bytes.
+    // D keeps what the finish of a body without faces reads.
+    BodyDyn* __restrict__ dd = dyn + b;
+    const bool w0 = sub == 0;
     BodyDyn D;
     const double* s = states + (size_t)S.sidx * (state_kind ? 38 : 22);
     double Rcg[9], tcg[3], R[9], tc[3];
@@ -524,59 +532,74 @@ __device__ __forceinline__ int prologue_body(const BodyStatic* __restrict__ st, 
     }
     D.pos = pos;
     D.flags = flags;
-    if (sub == 0) bpos[b] = (int8_t)pos;
+    if (w0) {
+        bpos[b] = (int8_t)pos;
+        dd->pos = pos;
+        dd->flags = flags;
+        dd->cpos = D.cpos;
+        dd->pad_ = 0;
+    }
 
     // mesh-frame FP32 constants
     double x[3], y[3], vl[3], wl[3], ul[3], wd[3];
     mat_t_vec(R, v, vl);
     mat_t_vec(R, w, wl);
     mat_t_vec(R, oc.u, ul);
-    D.fb.w = to_f3(wl);
+    FaceBody fb;
+    fb.w = to_f3(wl);
     for (int i = 0; i < 3; ++i) x[i] = tc[i] - tcg[i];
     mat_t_vec(R, x, y);
     D.dt[0] = y[0]; D.dt[1] = y[1]; D.dt[2] = y[2];
     cross64(wl, D.dt, wd);
     for (int i = 0; i < 3; ++i) y[i] = vl[i] + wd[i] - ul[i];
-    D.fb.vb = to_f3(y);
+    fb.vb = to_f3(y);
     x[0] = tcg[0] - tc[0]; x[1] = tcg[1] - tc[1]; x[2] = -tc[2];
     mat_t_vec(R, x, y);
-    D.fb.o = to_f3(y);
-    D.rz[0] = R[6]; D.rz[1] = R[7]; D.rz[2] = R[8];
-    D.tz = tc[2];
+    fb.o = to_f3(y);
+    if (w0) {
+        dd->fb = fb;
+        dd->rz[0] = R[6]; dd->rz[1] = R[7]; dd->rz[2] = R[8];
+        dd->tz = tc[2];
+        for (int i = 0; i < 3; ++i) dd->dt[i] = D.dt[i];
+    }
 
     for (int l = 0; l < kMaxLayers; ++l) {
+        if (l >= oc.nlayers) break;
         LayerK L;
-        L.fu = L.fv = 0.0; L.ku = L.kv = 0;
-        for (int k = 0; k < 3; ++k) L.ru[k] = L.rv[k] = 0.0;
-        if (l < oc.nlayers) {
-            const double s = oc.scale[l];
-            double bu = tc[0] * s - 0.5, bv = tc[1] * s - 0.5;
-            double fu = floor(bu), fv = floor(bv);
-            L.fu = bu - fu;
-            L.fv = bv - fv;
-            L.ku = wrap_texel(fu, oc);
-            L.kv = wrap_texel(fv, oc);
-            for (int k = 0; k < 3; ++k) { L.ru[k] = R[k] * s; L.rv[k] = R[3 + k] * s; }
-        }
-        D.layer[l] = L;
+        const double s = oc.scale[l];
+        double bu = tc[0] * s - 0.5, bv = tc[1] * s - 0.5;
+        double fu = floor(bu), fv = floor(bv);
+        L.fu = bu - fu;
+        L.fv = bv - fv;
+        L.ku = wrap_texel(fu, oc);
+        L.kv = wrap_texel(fv, oc);
+        for (int k = 0; k < 3; ++k) { L.ru[k] = R[k] * s; L.rv[k] = R[3 + k] * s; }
+        if (w0) dd->layer[l] = L;
     }
-    D.cur.n = oc.ncur;
-    D.cur.pad_ = 0;
-    for (int i = 0; i < kMaxCur; ++i) {
-        D.cur.type[i] = 0;
-        for (int k = 0; k < 3; ++k) D.cur.c[i][k] = D.cur.a[i][k] = 0.0;
-        for (int k = 0; k < 5; ++k) D.cur.p[i][k] = 0.0;
-        if (i >= oc.ncur) continue;
-        double d[3];
-        for (int k = 0; k < 3; ++k) d[k] = oc.cur_c[i][k] - tc[k];
-        mat_t_vec(R, d, D.cur.c[i]);
-        mat_t_vec(R, oc.cur_a[i], D.cur.a[i]);
-        D.cur.type[i] = oc.cur_type[i];
-        for (int k = 0; k < 5; ++k) D.cur.p[i][k] = oc.cur_p[i][k];
+    if (w0) {
+        dd->cur.n = oc.ncur;
+        dd->cur.pad_ = 0;
+        for (int i = 0; i < kMaxCur && oc.ncur > 0; ++i) {
+            CurBody& C = dd->cur;
+            C.type[i] = 0;
+            for (int k = 0; k < 3; ++k) C.c[i][k] = C.a[i][k] = 0.0;
+            for (int k = 0; k < 5; ++k) C.p[i][k] = 0.0;
+            if (i >= oc.ncur) continue;
+            double d[3], cl[3], al[3];
+            for (int k = 0; k < 3; ++k) d[k] = oc.cur_c[i][k] - tc[k];
+            mat_t_vec(R, d, cl);
+            mat_t_vec(R, oc.cur_a[i], al);
+            for (int k = 0; k < 3; ++k) { C.c[i][k] = cl[k]; C.a[i][k] = al[k]; }
+            C.type[i] = oc.cur_type[i];
+            for (int k = 0; k < 5; ++k) C.p[i][k] = oc.cur_p[i][k];
+        }
     }
     for (int i = 0; i < 9; ++i) { D.R[i] = R[i]; D.Rcg[i] = Rcg[i]; }
     for (int i = 0; i < 3; ++i) D.tcg[i] = tcg[i];
-    if (sub == 0) dyn[b] = D;
+    if (w0) {
+        for (int i = 0; i < 9; ++i) { dd->R[i] = R[i]; dd->Rcg[i] = Rcg[i]; }
+        for (int i = 0; i < 3; ++i) dd->tcg[i] = tcg[i];
+    }
 
     double* o = out + (size_t)b * SFH_OUT_DOUBLES;
     if (pos == -1) {          // not a hydrodynamic body: nothing assigned
