@@ -362,7 +362,9 @@ def surrogate_section(args) -> dict:
     t0 = time.time()
     cfg = S.TrainConfig(steps=args.sur_train_steps, batch=args.sur_batch, lr=args.sur_lr, warmup=200,
                         t_weight=args.sur_t_weight)
-    r = S.run_training_workload(8, args.sur_per_hull, args.sur_steps, args.sur_holdout, cfg, seed=17, log=log)
+    gate_tau = args.sur_gate_tau if args.sur_gate_tau > 0 else None
+    r = S.run_training_workload(8, args.sur_per_hull, args.sur_steps, args.sur_holdout, cfg, seed=17, log=log,
+                                gate_tau=gate_tau)
     hulls, nets, norms, w, ocean = r["hulls"], r["nets"], r["norms"], r["workload"], r["ocean"]
     ds = r["dataset"]
     log(f"surrogate training workload in {time.time() - t0:.1f}s")
@@ -418,6 +420,45 @@ def surrogate_section(args) -> dict:
     step.plan.merge(out, step.ft.clone(), err=err)
     torch.cuda.synchronize()
     sample_err = S.error_summary(err.cpu().numpy(), step.ood.cpu().numpy().astype(bool), step.pos.cpu().numpy())
+    gated = None
+    if gate_tau is not None:
+        # the same step with the error-aware gate: the barely wet crossing
+        # bodies whose training error exceeded tau go to the exact kernel too
+        gstep = S.SurrogateStep(batch, ocean, nets, r["norms_gated"], [per] * 8, states)
+        if not args.no_autotune:
+            gstep.run()
+            torch.cuda.synchronize()
+            gmask = gstep.ood.clone()
+            batch.autotune(ocean, states.data_ptr(), out.data_ptr(), stream, d_active=gmask.data_ptr())
+            tune["fallback_gated"] = batch.get_opts()
+            torch.cuda.synchronize()
+        tg_eager = _time_replays(gstep.run_with_fallback, 100)
+        gstep.capture(with_fallback=True)
+        tg_graph = _time_replays(gstep.replay, 100)
+        tg = min(tg_eager, tg_graph)
+        gerr = torch.zeros((nb, 2), dtype=torch.float32, device="cuda")
+        gstep.run()
+        gstep.plan.merge(out, gstep.ft.clone(), err=gerr)
+        torch.cuda.synchronize()
+        ge, go, gp = gerr.cpu().numpy(), gstep.ood.cpu().numpy().astype(bool), gstep.pos.cpu().numpy()
+        rnd = lambda d: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in d.items()}   # noqa: E731
+        keys_d = ("F_median", "F_p90", "F_p99", "T_median", "T_p90", "T_p99", "F_within_1pct", "T_within_1pct")
+        gh = [h["gated"] for h in hulls]
+        n_g = sum(g["delivered"]["n"] for g in gh)
+        gated = {
+            "tau": gate_tau,
+            "depth_below_m": [round(g["depth_below"], 3) for g in gh],
+            "held_out_flagged_fraction": round(sum(g["n_ood"] for g in gh) / max(1, n_g), 4),
+            "held_out_network_error": {k: round(float(sum(g[k] * g["n"] for g in gh) / max(1, sum(g["n"] for g in gh))), 5)
+                                       for k in keys_d},
+            "held_out_delivered_error": {k: round(float(sum(g["delivered"][k] * g["delivered"]["n"] for g in gh)
+                                                        / max(1, n_g)), 5) for k in keys_d},
+            "fallback_bodies": int(go.sum()),
+            "us_per_step": {"with_fallback_launch": round(tg_eager * 1e6, 2), "with_fallback_graph": round(tg_graph * 1e6, 2)},
+            "speedup": round(t_exact / tg, 2),
+            "throughput_sample_network_error": rnd(S.error_summary(ge, go, gp)),
+            "throughput_sample_delivered_error": rnd(S.delivered_summary(ge, go, gp)),
+        }
     # MFMA rate of the fused inference at a large batch (all hulls' held-out inputs, one launch)
     xs = [ds.inputs(h, norms[h], range(ds.train_steps, ds.steps))[0] for h in range(8)]
     x = torch.cat(xs).contiguous()
@@ -451,6 +492,7 @@ def surrogate_section(args) -> dict:
         "speedup": round(t_exact / t_sur, 2),
         "speedup_surrogate_only": round(t_exact / t_sur_only, 2),
         "throughput_sample_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in sample_err.items()},
+        "gated": gated,
         "mfma": {"inference_tflops": round(inf_tflops, 1), "inference_rows": int(x.shape[0]),
                  "inference_kernel_ms": round(ms, 4), "training_tflops": round(train_tflops, 1),
                  "peak_tflops_bf16_dense": 2500.0, "inference_frac": round(inf_tflops / 2500.0, 4),
@@ -546,7 +588,9 @@ def rollout_section(args, sea) -> dict:
         "faces_per_lane": {"rollout": batch.get_opts()[0], "exact": exact.get_opts()[0]},
         "waves": {"rollout": batch.get_opts()[1], "exact": exact.get_opts()[1]},
         "autotune": tune,
-        "audit_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in summ.items()},
+        "audit_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in summ.items() if k != "delivered"},
+        "audit_delivered_error": {k: (round(v, 5) if isinstance(v, float) else v)
+                                  for k, v in summ.get("delivered", {}).items()},
         "ood_fraction_at_audits": ood_frac,
         "held_out_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in held.items()},
         "seconds": round(time.time() - t0, 1),
@@ -629,6 +673,9 @@ def main() -> int:
     ap.add_argument("--sur-lr", type=float, default=5e-3, help="peak Adam learning rate (cosine schedule)")
     ap.add_argument("--sur-batch", type=int, default=16384)
     ap.add_argument("--sur-t-weight", type=float, default=1.0, help="torque outputs' weight in the training loss")
+    ap.add_argument("--sur-gate-tau", type=float, default=0.1,
+                    help="error-aware gate: crossing bodies in the depth band whose training error exceeded this "
+                         "fall back to the exact kernel (0: off)")
     ap.add_argument("--rollout-bodies", type=int, default=16384)
     ap.add_argument("--rollout-steps", type=int, default=300)
     ap.add_argument("--rollout-data-steps", type=int, default=14)

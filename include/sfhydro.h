@@ -454,6 +454,13 @@ typedef struct {
     float in_hi[SFH_SUR_FEATURES];      /* the body out-of-distribution               */
     float out_shift[6];                 /* CG-frame F, T = network output * scale + shift */
     float out_scale[6];
+    float gate_band;                    /* error-aware gate (0: off): a body crossing the   */
+                                        /* surface whose depth feature (feature 9, the CB's */
+                                        /* depth below the local surface) is below          */
+                                        /* in_lo[9] + gate_band is flagged like an          */
+                                        /* out-of-distribution one, so the exact kernel     */
+                                        /* answers for the barely wet bodies whose small    */
+                                        /* forces the network fits worst                    */
 } sfh_surrogate_norm;
 
 /* Features of bodies first .. first+count-1 of the batch (states: the whole
@@ -463,7 +470,8 @@ typedef struct {
  *   d_x        rows x 32 BF16 network inputs (rows >= count, a multiple of 32;
  *              rows past count are zero); needs norm
  *   d_ood      count bytes, 1 where a feature is outside [in_lo, in_hi] or not
- *              finite and the body is in the water; needs norm
+ *              finite, or the gate flags the body (gate_band), and the body is
+ *              in the water; needs norm
  *   d_pos      count int8: the body's position (CheckBodyFluidPosition on its
  *              AABB: 0 outside, 1 inside, 2 crossing; -1 not hydrodynamic)
  * and, from the exact outputs d_exact (the batch's n x SFH_OUT_DOUBLES array;

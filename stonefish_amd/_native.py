@@ -106,6 +106,7 @@ class sfh_surrogate_norm(C.Structure):
         ("in_hi", C.c_float * 13),
         ("out_shift", C.c_float * 6),
         ("out_scale", C.c_float * 6),
+        ("gate_band", C.c_float),
     ]
 
 

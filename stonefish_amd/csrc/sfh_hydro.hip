@@ -3050,6 +3050,7 @@ struct SurNorm {
     float in_shift[SFH_SUR_FEATURES], in_scale[SFH_SUR_FEATURES];
     float in_lo[SFH_SUR_FEATURES], in_hi[SFH_SUR_FEATURES];
     float out_shift[6], out_scale[6];
+    float gate_band;
 };
 
 __device__ __forceinline__ uint16_t bf16_bits(float f)
@@ -3298,7 +3299,10 @@ __device__ __forceinline__ void sur_input_row(const double* f, const SurNorm& nm
 }
 
 // Out of distribution: a feature outside the training range, or jets and
-// pipes in the ocean (the features carry only the uniform current).
+// pipes in the ocean (the features carry only the uniform current); and the
+// error-aware gate: a body crossing the surface whose depth feature is in the
+// band above the training minimum where the network's training error was
+// large.
 __device__ __forceinline__ uint8_t sur_ood(const double* f, int pos, const SurNorm& nm, const OceanK& oc)
 {
     uint8_t o = oc.ncur > 0 ? 1 : 0;
@@ -3306,6 +3310,7 @@ __device__ __forceinline__ uint8_t sur_ood(const double* f, int pos, const SurNo
         const float v = (float)f[k];
         if (!(v >= nm.in_lo[k] && v <= nm.in_hi[k])) o = 1;   // also catches NaN
     }
+    if (pos == 2 && nm.gate_band > 0.f && (float)f[9] < nm.in_lo[9] + nm.gate_band) o = 1;
     return pos > 0 ? o : (uint8_t)0;   // out of the water (or not hydrodynamic): the result is known
 }
 
@@ -3421,6 +3426,7 @@ static SurNorm sur_norm(const sfh_surrogate_norm* n)
     memcpy(k.in_hi, n->in_hi, sizeof(k.in_hi));
     memcpy(k.out_shift, n->out_shift, sizeof(k.out_shift));
     memcpy(k.out_scale, n->out_scale, sizeof(k.out_scale));
+    k.gate_band = n->gate_band;
     return k;
 }
 

@@ -331,6 +331,43 @@ def test_surrogate_beats_untrained_on_held_out(gpu):
     net.close()
 
 
+def test_error_aware_gate(gpu):
+    """gate_band flags exactly the bodies crossing the surface whose depth
+    feature is below in_lo[9] + gate_band (FP32, as the kernel compares), in
+    sfh_surrogate_features and in the one-launch plan; 0 turns it off."""
+    from stonefish_amd import surrogate as S
+    batch, ocean, hulls, grid = _small_fleet(waves=True, n_hulls=2, per_hull=256)
+    rng = np.random.default_rng(41)
+    st = torch.from_numpy(S.sample_states(rng, batch.n, xy_range=(0.0, 256.0), depth_range=(-0.5, 1.5))).cuda()
+    raw = torch.zeros(batch.n, 13, dtype=torch.float32, device="cuda")
+    pos = torch.zeros(batch.n, dtype=torch.int8, device="cuda")
+    S.features(batch, ocean, st, 0, batch.n, raw=raw, pos=pos)
+    torch.cuda.synchronize()
+    r, p = raw.cpu().numpy(), pos.cpu().numpy()
+    norm = S.Norm.fit(r, np.ones((batch.n, 6)))   # every body inside the training range
+    cross = p == 2
+    assert cross.sum() > 20 and (p == 1).sum() > 20
+    norm.gate_band = float(np.median(r[cross, 9])) - float(norm.in_lo[9])
+    expect = cross & (r[:, 9] < np.float32(norm.in_lo[9]) + np.float32(norm.gate_band))
+    assert 0 < expect.sum() < cross.sum()
+    rows = (batch.n + 31) // 32 * 32
+    x = torch.zeros(rows, 32, dtype=torch.bfloat16, device="cuda")
+    ood = torch.zeros(batch.n, dtype=torch.uint8, device="cuda")
+    S.features(batch, ocean, st, 0, batch.n, norm=norm, rows=rows, x=x, ood=ood)
+    torch.cuda.synchronize()
+    assert np.array_equal(ood.cpu().numpy().astype(bool), expect)
+    plan = S.Plan(batch, [S.Net(1024, seed=4)], [norm], [batch.n])
+    ft = torch.zeros(batch.n, 6, dtype=torch.float64, device="cuda")
+    o2 = torch.zeros(batch.n, dtype=torch.uint8, device="cuda")
+    plan.run(ocean, st, ft, ood=o2)
+    torch.cuda.synchronize()
+    assert torch.equal(o2, ood)
+    norm.gate_band = 0.0
+    S.features(batch, ocean, st, 0, batch.n, norm=norm, rows=rows, x=x, ood=ood)
+    torch.cuda.synchronize()
+    assert int(ood.sum()) == 0
+
+
 def test_one_launch_step_matches_separate_kernels(gpu):
     """sfh_surrogate_run (features + network + decode in one launch, two
     groups with their own networks and norms) against features, the fused
