@@ -538,6 +538,7 @@ def rollout_section(args, sea) -> dict:
     rng = np.random.default_rng(77)
     states = torch.from_numpy(S.sample_states(rng, n, (0.0, sea.wave_len), (0.0, 2.5), v_max=1.0,
                                               w_max=0.5)).cuda()
+    states0 = states.clone()   # the rollout integrates in place; the gated rollout starts from the same states
     # exact kernel alone on the same bodies (graph replay of one evaluation),
     # in the configuration the autotuner picks for the whole batch; the
     # rollout's batch evaluates only the few out-of-distribution bodies per
@@ -578,6 +579,50 @@ def rollout_section(args, sea) -> dict:
     t_ro = (time.perf_counter() - t1) / steps
     summ = ro.audit_summary()
     ood_frac = float(torch.cat([a[2] for a in ro.audits]).float().mean()) if ro.audits else None
+    rnd = lambda d: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in d.items()}   # noqa: E731
+    gated = None
+    if args.sur_gate_tau > 0:
+        # the same rollout with the error-aware gate fitted on the training
+        # steps: the flagged compounds (out of distribution, or barely wet
+        # where the training error exceeded tau) take the exact kernel
+        d, e = S.gate_errors(ds, 0, net, norm, range(min(4, ds.train_steps)))
+        band = S.fit_gate(d, e, float(norm.in_lo[9]), args.sur_gate_tau)
+        gnorm = S.Norm(**{k: getattr(norm, k) for k in ("in_shift", "in_scale", "in_lo", "in_hi", "out_shift",
+                                                        "out_scale")}, gate_band=band)
+        held_g = S.evaluate(ds, 0, net, gnorm, delivered=True)
+        rg = S.MixedRollout(batch, ocean, net, gnorm, states0, auv.mass, auv.inertia, dt=1e-3, audit_every=100,
+                            audit_batch=exact)
+        for _ in range(3):
+            rg.step()
+        torch.cuda.synchronize()
+        if not args.no_autotune:
+            gmask = rg.ood.clone()
+            st_now = rg.states.clone()
+            pts = batch.autotune(ocean, st_now.data_ptr(), out.data_ptr(), stream, d_active=gmask.data_ptr())
+            torch.cuda.synchronize()
+            tune["rollout_masked_gated"] = {"chosen": batch.get_opts(), "points": pts,
+                                            "active_bodies": int(gmask.sum())}
+        rg.capture()
+        rg.audits.clear()
+        rg.k = 0
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            rg.step()
+        torch.cuda.synchronize()
+        t_rg = (time.perf_counter() - t1) / steps
+        sg = rg.audit_summary()
+        gated = {
+            "tau": args.sur_gate_tau,
+            "depth_below_m": round(float(norm.in_lo[9]) + band, 3),
+            "ms_per_step": round(1e3 * t_rg, 4),
+            "speedup_vs_exact": round(t_exact / t_rg, 2),
+            "flagged_fraction_at_audits": (float(torch.cat([a[2] for a in rg.audits]).float().mean())
+                                           if rg.audits else None),
+            "audit_network_error": rnd({k: v for k, v in sg.items() if k != "delivered"}),
+            "audit_delivered_error": rnd(sg.get("delivered", {})),
+            "held_out_network_error": rnd({k: v for k, v in held_g.items() if k != "delivered"}),
+            "held_out_delivered_error": rnd(held_g["delivered"]),
+        }
     return {
         "workload": f"{n} AUV compounds (8,192 + 4 x 1,024 triangles, {nfaces} faces), 1 kHz, "
                     f"1024 x 1024 PM wave grid, audit every 100 steps, {steps} steps",
@@ -593,6 +638,7 @@ def rollout_section(args, sea) -> dict:
                                   for k, v in summ.get("delivered", {}).items()},
         "ood_fraction_at_audits": ood_frac,
         "held_out_error": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in held.items()},
+        "gated": gated,
         "seconds": round(time.time() - t0, 1),
     }
 
