@@ -1252,13 +1252,14 @@ __device__ __forceinline__ void wave_lds_sync()
 
 // kEpilogueWaves wavefronts per workgroup; wavefront w of W takes the bodies
 // w, w + W, w + 2W, ...: lane k reads the position of the k-th of the
-// wavefront's next 64 bodies in one load, and the wavefront sums the chunk
-// partials of those with face work (positions 1 and 2; the prologue wrote the
-// others' rows, and those of bodies without faces), in order, into LDS slots;
-// every kEpilogueBatch bodies (and at the end of the 64) lane j finishes the
-// row of the body in slot j — world rotation, moment about the CoG, buoyancy,
-// CorrectHydrodynamicForces, the write mask — so a wavefront with several
-// bodies runs the finishing arithmetic once per batch, not once per body.
+// wavefront's next 64 bodies in one load.  Of those with face work (positions
+// 1 and 2; the prologue wrote the others' rows, and those of bodies without
+// faces), a body of one chunk is finished by its own lane from the chunk's
+// partial; the larger ones are summed by the whole wavefront, one after the
+// other, into LDS slots, and every kEpilogueBatch bodies (and at the end
+// of the 64) lane j finishes the row of the body in slot j — world rotation,
+// moment about the CoG, buoyancy, CorrectHydrodynamicForces, the write mask —
+// so the finishing arithmetic runs once per batch, not once per body.
 // A masked step with few active bodies costs two loads per 64 bodies.
 // It also zeroes the work list's counts for the next step's prologue.
 static constexpr int kEpilogueWaves = 4;
@@ -1278,7 +1279,23 @@ __global__ __launch_bounds__(64 * kEpilogueWaves, 4) void k_epilogue(
         const uint32_t bl = base + (uint32_t)lane * W;
         int p = -3;
         if (bl < n) p = bpos[bl];
-        uint64_t m = __ballot((p == 1 || p == 2) && st[bl].nchunks);
+        uint32_t nck = 0;
+        if (p == 1 || p == 2) nck = st[bl].nchunks;
+        // a body of one chunk: the chunk's partial is its sum, and its lane
+        // finishes the row, every such body of the 64 at once
+        if (nck == 1) {
+            const double2* p2 = (const double2*)(partials + (size_t)st[bl].chunk0 * kChunkDoubles);
+            double P[23];
+#pragma unroll
+            for (int k = 0; k < 11; ++k) {
+                const double2 x = p2[k];
+                P[2 * k] = x.x;
+                P[2 * k + 1] = x.y;
+            }
+            P[22] = ((const double*)p2)[22];
+            epilogue_finish(st[bl], dyn[bl], oc, P, out + (size_t)bl * SFH_OUT_DOUBLES);
+        }
+        uint64_t m = __ballot(nck > 1);
         uint32_t mine = 0;    // lane j < nb: the body in slot j
         uint32_t first = 0;   // the body in slot 0 (wave-uniform)
         int nb = 0;
