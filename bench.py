@@ -263,7 +263,7 @@ def single_body_section(args) -> dict:
         sv = np.zeros(2)
         settings = H.HydrodynamicsSettings(True, True)
 
-        def call(i):
+        def call(i, outs=outs, sv=sv):
             if submerged:
                 H.ComputeHydrodynamicForcesSubmerged(gm, go, st[i, :16], st[i, :16], st[i, 16:19], st[i, 19:22],
                                                      *outs[2:])
@@ -276,6 +276,22 @@ def single_body_section(args) -> dict:
         for i in range(nsteps):
             call(i)
         statics_rate = nsteps / (time.perf_counter() - t0)
+        # the same calls from a pool of host threads, one task per body-step,
+        # as the reference's pool threads call the statics (each thread takes
+        # a workspace with its own stream from the library's per-device pool)
+        from concurrent.futures import ThreadPoolExecutor
+        nthr = BOX_CPU_SHARE
+
+        def worker(tid):
+            o, s2 = [np.zeros(3) for _ in range(6)], np.zeros(2)
+            for i in range(tid, nsteps, nthr):
+                call(i, o, s2)
+
+        with ThreadPoolExecutor(nthr) as ex:
+            list(ex.map(worker, range(nthr)))   # every thread's first call sets up its workspace
+            t0 = time.perf_counter()
+            list(ex.map(worker, range(nthr)))
+            statics_pool_rate = nsteps / (time.perf_counter() - t0)
         # (2) all steps as one batch of the body's states (body level)
         desc = H.BodyDesc(mesh=gm, Cd=b.Cd, Cf=b.Cf, volume=b.volume, surface=b.surface)
         batch = H.BodyBatch([desc] * nsteps)
@@ -329,7 +345,9 @@ def single_body_section(args) -> dict:
             "gpu_per_body_statics": {"value": round(statics_rate, 1), "unit": "body-steps/s",
                                      "call": "ComputeHydrodynamicForcesSubmerged" if submerged
                                      else "ComputeHydrodynamicForcesSurface",
-                                     "note": "one synchronous call per step from one host thread, host buffers"},
+                                     "note": "one synchronous call per step from one host thread, host buffers",
+                                     "pool_threads": nthr,
+                                     "pool_value": round(statics_pool_rate, 1)},
             "gpu_batch": {"value": round(batch_rate, 1), "unit": "body-steps/s", "ms_per_replay": round(sec * 1e3, 4),
                           "face_pass_ms": round(face_ms, 5), "step_kernels_ms": round(total_ms, 5),
                           "roofline": {"bound": "hbm", "achieved": round(alg / (face_ms * 1e-3) / 1e9, 1),
