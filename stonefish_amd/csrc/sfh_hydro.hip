@@ -2670,6 +2670,13 @@ struct SingleHead {
     BodyDyn dyn;
     uint32_t sidx;
     int8_t bpos;
+    double state[38];   // T_CG, T_C, v, omega of a call whose state came by value
+};
+
+// A per-body call's state passed by value to the setup kernel (no copy
+// command of its own).
+struct StateArg {
+    double v[38];
 };
 
 __host__ __device__ constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -2696,11 +2703,12 @@ __host__ __device__ constexpr SingleLayout single_layout(uint32_t F)
 // counts (the buffer may hold another call's).
 __global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H, Chunk* __restrict__ chunks,
                                                       uint32_t* __restrict__ counts, const uint2* __restrict__ blkwin,
-                                                      BodyStatic st)
+                                                      BodyStatic st, StateArg sa, int with_state)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t csize = 64u * kSingleK;
     const uint32_t nc = st.nchunks;
+    if (with_state && t < 38) H->state[t] = sa.v[t];
     if (t == 0) {
         H->st = st;
         H->sidx = 0;
@@ -2730,8 +2738,11 @@ __global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H
 // One evaluation of `m` with `flags` from the 38-double state d_state into
 // the output row d_out, on stream s, in the buffer `buf` laid out for the
 // mesh's face count.  No allocation.
+// d_state: the state in device memory, or nullptr with h_state: the state's
+// 38 doubles on the host, passed to the setup kernel by value.
 static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags, const sfh_ocean_t* o,
-                         const double* d_state, double* d_out, hipStream_t s, uint32_t waves)
+                         const double* d_state, double* d_out, hipStream_t s, uint32_t waves,
+                         const double* h_state = nullptr)
 {
     const SingleLayout L = single_layout(m->nf);
     SingleHead* H = (SingleHead*)buf;
@@ -2741,9 +2752,17 @@ static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags
     st.chunk0 = 0;
     st.nchunks = single_nchunks(m->nf);
     st.sidx = 0;
-    const uint32_t nt = std::max(st.nchunks, 1u);
+    const uint32_t nt = std::max(st.nchunks, 38u);
+    StateArg sa;
+    const int with_state = d_state == nullptr;
+    if (with_state) {
+        memcpy(sa.v, h_state, sizeof(sa.v));
+        d_state = H->state;
+    } else {
+        memset(sa.v, 0, sizeof(sa.v));
+    }
     hipLaunchKernelGGL(k_single_setup, dim3((nt + 255) / 256), dim3(256), 0, s, H, chunks, lists + st.nchunks,
-                       m->d_blkwin, st);
+                       m->d_blkwin, st, sa, with_state);
     // a batch view of the buffer: no allocation (empty vectors, borrowed pointers)
     sfh_batch_t b;
     b.n = 1;
@@ -2858,12 +2877,11 @@ static int single_call(sfh_mesh_t* m, uint32_t flags, const sfh_ocean_t* o, cons
         memcpy(h + 16, T_C, 16 * sizeof(double));
         memcpy(h + 32, v, 3 * sizeof(double));
         memcpy(h + 35, w, 3 * sizeof(double));
-        hipError_t e = hipMemcpyAsync(ws->d_io, h, 38 * sizeof(double), hipMemcpyHostToDevice, ws->stream);
-        if (e != hipSuccess) rc = fail(SFH_ERR_HIP, "state upload: %s", hipGetErrorString(e));
-        if (rc == SFH_OK) rc = single_launch(ws->d_buf, m, flags, o, ws->d_io, ws->d_io + 38, ws->stream, ws->waves);
+        // the state goes to the device as the setup kernel's argument
+        rc = single_launch(ws->d_buf, m, flags, o, nullptr, ws->d_io + 38, ws->stream, ws->waves, h);
         if (rc == SFH_OK) {
-            e = hipMemcpyAsync(h + 38, ws->d_io + 38, SFH_OUT_DOUBLES * sizeof(double), hipMemcpyDeviceToHost,
-                               ws->stream);
+            hipError_t e = hipMemcpyAsync(h + 38, ws->d_io + 38, SFH_OUT_DOUBLES * sizeof(double),
+                                          hipMemcpyDeviceToHost, ws->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(ws->stream);
             if (e != hipSuccess) rc = fail(SFH_ERR_HIP, "per-body call: %s", hipGetErrorString(e));
             else memcpy(out, h + 38, SFH_OUT_DOUBLES * sizeof(double));
