@@ -289,6 +289,26 @@ def test_batch_determinism_and_graph(gpu):
         assert torch.equal(gout, eager), rep
 
 
+def test_rows_independent_of_launch_shape(gpu):
+    """The face launch's wavefront count (and with it the epilogue's, whose
+    wavefronts finish several bodies at once or one at a time) and one or two
+    face launches change the speed, not the bits (the faces per lane decide
+    which faces a lane sums in FP32, so they are held fixed)."""
+    log("case: launch shapes, 64 bodies up to 20,000 faces")
+    w, go, oo, descs, obodies = build_fleet(64, f_hi=20000, waves=True, seed=5)
+    batch = H.BodyBatch(descs, faces_per_lane=16)
+    st = WL.random_states(w, 1)[0]
+    ref = None
+    for waves, split in ((64, 0), (1024, 0), (16384, 1), (4096, 1), (8, 0)):
+        batch.set_opts(16, waves, split)
+        g = batch.compute_host(go, st)
+        if ref is None:
+            ref = g
+            assert np.bincount(g[:, 21].astype(int) + 1, minlength=4)[2:].min() > 0   # inside and crossing bodies
+        else:
+            assert np.array_equal(g.view(np.uint64), ref.view(np.uint64)), (waves, split)
+
+
 def test_wave_grid_builder(gpu):
     import torch
     comps = W.pm_components(64, 12.0, 1024.0, seed=8)
