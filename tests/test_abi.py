@@ -75,3 +75,34 @@ def test_current_argument_checks_without_device():
         o.add_jet([2, 0, 1], [0, 1, 0], 0.2, 1.0)
     assert b"at most 4" in lib.sfh_last_error()
     o.close()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror of a struct in include/sfhydro.h has the C
+    compiler's size and field offsets (a field added on one side only, or a
+    padding difference, would shift every later field)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = [getattr(N, n) for n in sorted(dir(N)) if n.startswith("sfh_")
+               and isinstance(getattr(N, n), type) and issubclass(getattr(N, n), C.Structure)]
+    assert len(structs) >= 10
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "sfhydro.h"', "int main(void) {"]
+    for cls in structs:
+        nm = cls.__name__
+        lines.append(f'printf("{nm} %zu\\n", sizeof({nm}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{nm}.{f[0]} %zu\\n", offsetof({nm}, {f[0]}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines()
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for cls in structs:
+        assert int(got[cls.__name__]) == C.sizeof(cls), cls.__name__
+        for f in cls._fields_:
+            assert int(got[f"{cls.__name__}.{f[0]}"]) == getattr(cls, f[0]).offset, (cls.__name__, f[0])
