@@ -664,13 +664,15 @@ __device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_
 
 // Every lane of the grid reaches the list build (lanes past the last body
 // contribute nothing), so the scans see whole wavefronts and workgroups.
-__global__ __launch_bounds__(kPrologueBlock) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
-                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active,
-                           const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
-                           int8_t* __restrict__ bpos, double* __restrict__ out, uint32_t* __restrict__ list,
-                           uint32_t nc, uint32_t* __restrict__ counts)
+// The prologue of the bodies of one workgroup (gt: the thread's index in the
+// grid); every thread of the workgroup calls it (the list build's barriers).
+__device__ __forceinline__ void prologue_group(const BodyStatic* __restrict__ st, const double* __restrict__ states,
+                                               int state_kind, uint32_t n, const OceanK& oc,
+                                               const uint8_t* __restrict__ active, const uint32_t* __restrict__ sidx,
+                                               BodyDyn* __restrict__ dyn, int8_t* __restrict__ bpos,
+                                               double* __restrict__ out, uint32_t* __restrict__ list, uint32_t nc,
+                                               uint32_t* __restrict__ counts, uint32_t gt)
 {
-    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t b = gt / kPrologueLanes;
     const int sub = (int)(gt % kPrologueLanes);
     int pos = -3;
@@ -682,6 +684,16 @@ __global__ __launch_bounds__(kPrologueBlock) void k_prologue(const BodyStatic* _
         else c_cross = st[b].nchunks;
     }
     list_append(list, counts, nc, c_sub, c_cross, chunk0, sub);
+}
+
+__global__ __launch_bounds__(kPrologueBlock) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
+                           uint32_t n, OceanK oc, const uint8_t* __restrict__ active,
+                           const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
+                           int8_t* __restrict__ bpos, double* __restrict__ out, uint32_t* __restrict__ list,
+                           uint32_t nc, uint32_t* __restrict__ counts)
+{
+    prologue_group(st, states, state_kind, n, oc, active, sidx, dyn, bpos, out, list, nc, counts,
+                   blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Jets and pipes at a mesh-frame point (Jet.cpp:54-75, Pipe.cpp:57-78): the
@@ -2339,7 +2351,7 @@ static int launch_faces(sfh_batch_t* b, const OceanK& oc, hipStream_t s)
 // the prologue (ev[0]), around the face launches (ev[1], ev[2]) and after the
 // last launch (ev[3]).
 static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_states, int kind, double* d_out,
-                        hipStream_t s, bool timed, const uint8_t* active = nullptr)
+                        hipStream_t s, bool timed, const uint8_t* active = nullptr, bool skip_prologue = false)
 {
     if (b->n == 0) return SFH_OK;
     // every launch of the step runs or none does: the epilogue zeroes the
@@ -2350,8 +2362,10 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
     double* rows = b->compound ? b->d_part_out : d_out;
     const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + kPrologueBlock - 1) / kPrologueBlock);
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
-    hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(kPrologueBlock), 0, s, b->d_static, d_states, kind, b->n, oc, active,
-                       b->d_sidx, b->d_dyn, b->d_bpos, rows, b->d_lists, b->nchunks, b->d_lists + b->nchunks);
+    if (!skip_prologue)
+        hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(kPrologueBlock), 0, s, b->d_static, d_states, kind, b->n, oc,
+                           active, b->d_sidx, b->d_dyn, b->d_bpos, rows, b->d_lists, b->nchunks,
+                           b->d_lists + b->nchunks);
     if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
     int rc = launch_faces(b, oc, s);
     if (rc != SFH_OK) return rc;
@@ -2701,9 +2715,14 @@ __host__ __device__ constexpr SingleLayout single_layout(uint32_t F)
 // Writes the body's static record and its chunks with their vertex windows
 // (from the mesh's per-64-face block windows), and zeroes the work list's
 // counts (the buffer may hold another call's).
-__global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H, Chunk* __restrict__ chunks,
+// The setup of a per-body call and, in its first workgroup once the record,
+// the state and the counts are written, the body's prologue (one launch
+// instead of two; the batch's own prologue launch is skipped).
+__global__ __launch_bounds__(kPrologueBlock) void k_single_setup(SingleHead* __restrict__ H, Chunk* __restrict__ chunks,
                                                       uint32_t* __restrict__ counts, const uint2* __restrict__ blkwin,
-                                                      BodyStatic st, StateArg sa, int with_state)
+                                                      BodyStatic st, StateArg sa, int with_state,
+                                                      const double* __restrict__ states, OceanK oc,
+                                                      double* __restrict__ out, uint32_t* __restrict__ list)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t csize = 64u * kSingleK;
@@ -2733,6 +2752,11 @@ __global__ __launch_bounds__(256) void k_single_setup(SingleHead* __restrict__ H
         c.pad[0] = c.pad[1] = c.pad[2] = 0;
         chunks[t] = c;
     }
+    if (blockIdx.x == 0) {
+        __syncthreads();   // the record, the state and the zeroed counts are this workgroup's writes
+        prologue_group(&H->st, states, 1, 1u, oc, nullptr, &H->sidx, &H->dyn, &H->bpos, out, list, nc, counts,
+                       threadIdx.x);
+    }
 }
 
 // One evaluation of `m` with `flags` from the 38-double state d_state into
@@ -2761,8 +2785,9 @@ static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags
     } else {
         memset(sa.v, 0, sizeof(sa.v));
     }
-    hipLaunchKernelGGL(k_single_setup, dim3((nt + 255) / 256), dim3(256), 0, s, H, chunks, lists + st.nchunks,
-                       m->d_blkwin, st, sa, with_state);
+    hipLaunchKernelGGL(k_single_setup, dim3((nt + kPrologueBlock - 1) / kPrologueBlock), dim3(kPrologueBlock), 0, s,
+                       H, chunks, lists + st.nchunks, m->d_blkwin, st, sa, with_state, d_state, make_oceank(o), d_out,
+                       lists);
     // a batch view of the buffer: no allocation (empty vectors, borrowed pointers)
     sfh_batch_t b;
     b.n = 1;
@@ -2779,7 +2804,7 @@ static int single_launch(unsigned char* buf, const sfh_mesh_t* m, uint32_t flags
     b.d_lists = lists;
     b.d_dyn = &H->dyn;
     b.d_partials = (double*)(buf + L.partials);
-    return batch_launch(&b, o, d_state, 1, d_out, s, false);
+    return batch_launch(&b, o, d_state, 1, d_out, s, false, nullptr, true);   // the prologue ran in the setup
 }
 
 // ---- per-body reference-shaped calls ------------------------------------------
