@@ -2674,9 +2674,10 @@ extern "C" int sfh_batch_kernel_ms(sfh_batch b, double* face_ms, double* total_m
 // workspace of the per-body statics (device memory from hipMalloc, grown on
 // demand) or the no-alloc entry point's file-scope array.  One kernel writes
 // the static record and the chunks from its arguments and the mesh's block
-// windows, and the step is that kernel, the prologue (which gives the body
-// the position its call names: the Surface call the crossing loop, the
-// Submerged and aerodynamic calls the submerged one) and the face pass.
+// windows and, in its first workgroup, runs the body's prologue (which gives
+// the body the position its call names: the Surface call the crossing loop,
+// the Submerged and aerodynamic calls the submerged one); the step is that
+// kernel, the face pass and the epilogue.
 static constexpr uint32_t kSingleK = 8;   // faces per lane of per-body calls
 
 struct SingleHead {
