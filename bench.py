@@ -182,6 +182,16 @@ def measured_traffic():
     return None
 
 
+def _section(name: str, fn):
+    """Runs an optional section of the report; a Python-level failure in it is
+    recorded in its place (and logged) instead of costing the headline line."""
+    try:
+        return fn()
+    except Exception as e:   # noqa: BLE001 - reported in the JSON line, not swallowed
+        log(f"{name} section failed: {e!r}")
+        return {"error": repr(e)}
+
+
 def _time_replays(fn, k: int) -> float:
     """Seconds per call of fn over k calls (after 3 untimed), bracketed by syncs."""
     import torch
@@ -948,22 +958,24 @@ def main() -> int:
         out_h = out.cpu().numpy()
         # the last eager launch used state set (count-1) % nsets
         kk = (max(10, min(args.steps, 50)) - 1) % nsets
-        res["parity"] = parity_sample(w, states_h[kk], grid.cpu().numpy(), out_h)
+        res["parity"] = _section("parity", lambda: parity_sample(w, states_h[kk], grid.cpu().numpy(), out_h))
         log(f"parity sample in {time.time() - t0:.1f}s")
     if not args.no_cpu:
         t0 = time.time()
-        res["cpu_baseline"] = cpu_baseline(w, states_h[0], grid.cpu().numpy())
+        res["cpu_baseline"] = _section("cpu_baseline", lambda: cpu_baseline(w, states_h[0], grid.cpu().numpy()))
         log(f"cpu baseline in {time.time() - t0:.1f}s")
     if not args.no_single:
         t0 = time.time()
         graphs = None
-        res["single_body"] = single_body_section(args)
+        res["single_body"] = _section("single_body", lambda: single_body_section(args))
         log(f"single-body workloads in {time.time() - t0:.1f}s")
     if not args.no_surrogate:
         graphs = None
-        res["surrogate"] = surrogate_section(args)
-        res["surrogate"]["speedup_vs_fleet_exact"] = round(res["surrogate"]["surrogate_body_steps_per_s"] / value, 2)
-        res["rollout"] = rollout_section(args, w.sea)
+        res["surrogate"] = _section("surrogate", lambda: surrogate_section(args))
+        if "surrogate_body_steps_per_s" in res["surrogate"]:
+            res["surrogate"]["speedup_vs_fleet_exact"] = round(res["surrogate"]["surrogate_body_steps_per_s"] / value,
+                                                               2)
+        res["rollout"] = _section("rollout", lambda: rollout_section(args, w.sea))
     print(json.dumps(res), flush=True)
     return 0
 
