@@ -626,13 +626,23 @@ __device__ __forceinline__ int prologue_body(const BodyStatic* __restrict__ st, 
 // order of the atomics, the results do not: a chunk's partial has its own
 // slot and the epilogue sums a body's partials in chunk order.
 static constexpr int kPrologueBlock = 256;
+// Whole unmasked steps of large batches use workgroups of 1,024 (128 bodies):
+// every workgroup reserves its list ranges with one atomic add per list, and
+// the atomics of all workgroups on the two counters serialise (the compound
+// batch's 81,920 part bodies: 1,280 reservations per list instead of 5,120,
+// its step 8-14 us faster); with few bodies, or a mask leaving most
+// workgroups without work, the smaller workgroups hide the prologue's latency
+// chain better (the fleet 2.5 us, the rollout's masked step 2 us faster).
+static constexpr int kPrologueBlockLarge = 1024;
+static constexpr uint32_t kPrologueLargeBodies = 32768;
 
+template <int BLOCK>
 __device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_t* __restrict__ counts,
                                             uint32_t nc, uint32_t c_sub, uint32_t c_cross, uint32_t chunk0,
                                             int sub)
 {
-    __shared__ uint32_t wsum[2][kPrologueBlock / 64];
-    __shared__ uint32_t wbase[2][kPrologueBlock / 64];
+    __shared__ uint32_t wsum[2][BLOCK / 64];
+    __shared__ uint32_t wbase[2][BLOCK / 64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const uint32_t v0 = sub == 0 ? c_sub : 0u, v1 = sub == 0 ? c_cross : 0u;
@@ -646,9 +656,9 @@ __device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_
     if (threadIdx.x < 2) {
         const int k = threadIdx.x;
         uint32_t run = 0;
-        for (int j = 0; j < kPrologueBlock / 64; ++j) { wbase[k][j] = run; run += wsum[k][j]; }
+        for (int j = 0; j < BLOCK / 64; ++j) { wbase[k][j] = run; run += wsum[k][j]; }
         const uint32_t base = run ? atomicAdd(counts + k, run) : 0u;
-        for (int j = 0; j < kPrologueBlock / 64; ++j) wbase[k][j] += base;
+        for (int j = 0; j < BLOCK / 64; ++j) wbase[k][j] += base;
     }
     __syncthreads();
     const int lead = lane & ~(kPrologueLanes - 1);
@@ -664,8 +674,10 @@ __device__ __forceinline__ void list_append(uint32_t* __restrict__ list, uint32_
 
 // Every lane of the grid reaches the list build (lanes past the last body
 // contribute nothing), so the scans see whole wavefronts and workgroups.
-// The prologue of the bodies of one workgroup (gt: the thread's index in the
-// grid); every thread of the workgroup calls it (the list build's barriers).
+// The prologue of the bodies of one workgroup of BLOCK threads (gt: the
+// thread's index in the grid); every thread of the workgroup calls it (the
+// list build's barriers).
+template <int BLOCK>
 __device__ __forceinline__ void prologue_group(const BodyStatic* __restrict__ st, const double* __restrict__ states,
                                                int state_kind, uint32_t n, const OceanK& oc,
                                                const uint8_t* __restrict__ active, const uint32_t* __restrict__ sidx,
@@ -683,17 +695,18 @@ __device__ __forceinline__ void prologue_group(const BodyStatic* __restrict__ st
         if (pos == 1) c_sub = st[b].nchunks;
         else c_cross = st[b].nchunks;
     }
-    list_append(list, counts, nc, c_sub, c_cross, chunk0, sub);
+    list_append<BLOCK>(list, counts, nc, c_sub, c_cross, chunk0, sub);
 }
 
-__global__ __launch_bounds__(kPrologueBlock) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_prologue(const BodyStatic* __restrict__ st, const double* __restrict__ states, int state_kind,
                            uint32_t n, OceanK oc, const uint8_t* __restrict__ active,
                            const uint32_t* __restrict__ sidx, BodyDyn* __restrict__ dyn,
                            int8_t* __restrict__ bpos, double* __restrict__ out, uint32_t* __restrict__ list,
                            uint32_t nc, uint32_t* __restrict__ counts)
 {
-    prologue_group(st, states, state_kind, n, oc, active, sidx, dyn, bpos, out, list, nc, counts,
-                   blockIdx.x * blockDim.x + threadIdx.x);
+    prologue_group<BLOCK>(st, states, state_kind, n, oc, active, sidx, dyn, bpos, out, list, nc, counts,
+                          blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Jets and pipes at a mesh-frame point (Jet.cpp:54-75, Pipe.cpp:57-78): the
@@ -2360,12 +2373,20 @@ static int batch_launch(sfh_batch_t* b, const sfh_ocean_t* o, const double* d_st
         return fail(SFH_ERR_INVALID, "bad K");
     OceanK oc = make_oceank(o);
     double* rows = b->compound ? b->d_part_out : d_out;
-    const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + kPrologueBlock - 1) / kPrologueBlock);
     if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
-    if (!skip_prologue)
-        hipLaunchKernelGGL(k_prologue, dim3(blocks), dim3(kPrologueBlock), 0, s, b->d_static, d_states, kind, b->n, oc,
-                           active, b->d_sidx, b->d_dyn, b->d_bpos, rows, b->d_lists, b->nchunks,
-                           b->d_lists + b->nchunks);
+    if (!skip_prologue) {
+        const bool large = !active && b->n >= kPrologueLargeBodies;
+        const uint32_t blk = large ? kPrologueBlockLarge : kPrologueBlock;
+        const uint32_t blocks = (uint32_t)(((uint64_t)b->n * kPrologueLanes + blk - 1) / blk);
+        if (large)
+            hipLaunchKernelGGL(k_prologue<kPrologueBlockLarge>, dim3(blocks), dim3(blk), 0, s, b->d_static, d_states,
+                               kind, b->n, oc, active, b->d_sidx, b->d_dyn, b->d_bpos, rows, b->d_lists, b->nchunks,
+                               b->d_lists + b->nchunks);
+        else
+            hipLaunchKernelGGL(k_prologue<kPrologueBlock>, dim3(blocks), dim3(blk), 0, s, b->d_static, d_states, kind,
+                               b->n, oc, active, b->d_sidx, b->d_dyn, b->d_bpos, rows, b->d_lists, b->nchunks,
+                               b->d_lists + b->nchunks);
+    }
     if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
     int rc = launch_faces(b, oc, s);
     if (rc != SFH_OK) return rc;
@@ -2755,8 +2776,8 @@ __global__ __launch_bounds__(kPrologueBlock) void k_single_setup(SingleHead* __r
     }
     if (blockIdx.x == 0) {
         __syncthreads();   // the record, the state and the zeroed counts are this workgroup's writes
-        prologue_group(&H->st, states, 1, 1u, oc, nullptr, &H->sidx, &H->dyn, &H->bpos, out, list, nc, counts,
-                       threadIdx.x);
+        prologue_group<kPrologueBlock>(&H->st, states, 1, 1u, oc, nullptr, &H->sidx, &H->dyn, &H->bpos, out, list,
+                                       nc, counts, threadIdx.x);
     }
 }
 
